@@ -1,0 +1,207 @@
+/*
+ * o3d_biot.h -- C ABI of libo3dbiot.so: direct regularised Biot-Savart velocity
+ * and velocity-gradient sums on one AMD Instinct MI355X (gfx950).
+ *
+ * Drop-in for the hot path of Omega3D (Applied-Scientific-Research/Omega3D):
+ *   points_affect_points  src/Influence.h:67-551
+ *   panels_affect_points  src/Influence.h:557-1099
+ * and its external-solver plugin point
+ *   external_vel_solver_f_ / external_vel_solver_d_   src/Influence.h:33-48
+ *
+ * Conventions shared by every entry point (the reference's, Influence.h:296-307,
+ * ElementBase.h:170-208, Points.h:252-277):
+ *   - arrays are structure-of-arrays, float32, caller-owned;
+ *   - results are ACCUMULATED (+=) into the caller's velocity and gradient
+ *     arrays, unscaled: the caller applies 1/(4 pi) and the freestream later;
+ *   - gradient component k = 3*j + i holds du_i/dx_j (ux vx wx uy vy wy uz vz wz);
+ *   - sources and targets may be the same arrays (a collection acting on itself);
+ *   - when they are, the self pair j == i is INCLUDED, as in the reference,
+ *     unless o3d_opts.self_term = O3D_SELF_EXCLUDE.
+ * Every entry point returns an o3d_status; nothing asserts or aborts across the
+ * ABI.  o3d_last_error() describes the last failure on the calling thread.
+ */
+#ifndef O3D_BIOT_H
+#define O3D_BIOT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define O3D_ABI_VERSION 1
+
+/* regularised cores, runtime-selected (the reference selects one at compile
+ * time with a #define, CoreFunc.h:35-39) */
+enum o3d_core {
+  O3D_CORE_RM = 1,   /* Rosenhead-Moore           CoreFunc.h:43-83   */
+  O3D_CORE_WL = 2,   /* Winckelmans-Leonard       CoreFunc.h:241-289 */
+  O3D_CORE_EXP = 3,  /* compact exponential       CoreFunc.h:86-238  */
+  O3D_CORE_V2 = 4    /* Vatistas n=2              CoreFunc.h:292-341 */
+};
+
+/* results_t values of ResultsType.h:27-33 that this path serves */
+enum o3d_results { O3D_VELONLY = 1, O3D_VELANDGRAD = 2 };
+
+enum o3d_precision {
+  O3D_PREC_FP32 = 0,         /* FP32 pairs, plain FP32 accumulation per target            */
+  O3D_PREC_COMPENSATED = 1,  /* FP32 pairs; per-tile FP32 partials folded with two-sum for
+                                the velocity and in FP64 for the gradient tensor           */
+  O3D_PREC_FP64 = 2          /* the FP64 oracle kernel: every operation in double          */
+};
+
+enum o3d_self { O3D_SELF_INCLUDE = 0, O3D_SELF_EXCLUDE = 1 };
+
+enum o3d_status {
+  O3D_OK = 0,
+  O3D_EINVAL = 1,          /* bad argument (null pointer, bad enum, size mismatch)  */
+  O3D_ENOMEM = 2,          /* device allocation failed                              */
+  O3D_EDEVICE = 3,         /* HIP runtime / kernel error                            */
+  O3D_EINJECTED = 4,       /* failure forced by o3d_inject_fault (tests)            */
+  O3D_EADMIT = 5,          /* request does not fit the serving memory budget        */
+  O3D_EBUSY = 6,           /* try again: no free slot                               */
+  O3D_ECANCELLED = 7,      /* request cancelled or timed out before it ran          */
+  O3D_ENODEV = 8,          /* no usable gfx950 device                               */
+  O3D_ENOTFOUND = 9        /* unknown handle / request id                           */
+};
+
+/* SoA view of a Points<float> collection (ElementBase.h:43-462, Points.h:50-1054).
+ * As a SOURCE:  x, s and r are read (r may not be NULL).
+ * As a TARGET:  x is read, r is read when non-NULL (thick-cored particles; NULL =
+ *               inert field points, the reference's is_inert() branch), u is
+ *               accumulated, ug is accumulated when results == O3D_VELANDGRAD. */
+typedef struct o3d_points {
+  int64_t n;
+  const float* x[3];
+  const float* s[3];
+  const float* r;
+  float* u[3];
+  float* ug[9];
+} o3d_points;
+
+/* SoA view of a Surfaces<float> of flat triangles (Surfaces.h:50-251).
+ * ts is the ABSOLUTE vortex strength (sheet strength x area, Surfaces.h:309-335),
+ * ss the sheet SOURCE strength (NULL = none); idx holds 3 node indices per panel. */
+typedef struct o3d_panels {
+  int64_t np;          /* panels */
+  int64_t nn;          /* nodes  */
+  const float* x[3];
+  const uint32_t* idx;
+  const float* ts[3];
+  const float* area;
+  const float* ss;
+} o3d_panels;
+
+typedef struct o3d_opts {
+  int32_t core;        /* enum o3d_core                                              */
+  int32_t results;     /* enum o3d_results                                           */
+  int32_t precision;   /* enum o3d_precision                                         */
+  int32_t self_term;   /* enum o3d_self (exclusion needs src and targ to be the same
+                          collection: same n, source i == target i)                  */
+  int32_t tile;        /* sources per LDS tile (64..1024, multiple of 64); 0 = tuned */
+  int32_t tpt;         /* targets per thread (1, 2, 4); 0 = tuned                    */
+  int32_t maxlev;      /* panels: recursion depth (reference RECURSIVE_LEVELS = 3);
+                          negative selects the default                               */
+  int32_t reserved;
+  void* stream;        /* hipStream_t for the *_dev calls; NULL = null stream        */
+} o3d_opts;
+
+typedef struct o3d_stats {
+  double seconds;      /* wall time of the call (host calls) or of the kernels      */
+  double flops;        /* the reference's own count: N_t*(12 + F*N_s) etc.           */
+  double interactions; /* target-source pairs evaluated                              */
+} o3d_stats;
+
+/* Fill *opts with the defaults: WL core (the reference's compiled default),
+ * vel+grad, compensated, self included, tuned tile/tpt, maxlev 3. */
+void o3d_default_opts(o3d_opts* opts);
+
+/* Pick the device (ordinal) for this thread and create the library's context.
+ * Optional: every call initialises lazily on the current device. */
+int o3d_init(int device);
+const char* o3d_last_error(void);
+int o3d_abi_version(void);
+
+/* ---- the reference's call shapes ------------------------------------------ */
+
+/* points_affect_points (Influence.h:67-551) on HOST arrays: copies in, runs on the
+ * GPU, synchronises, accumulates into targ->u / targ->ug.  stats may be NULL. */
+int o3d_points_affect_points(const o3d_points* src, o3d_points* targ,
+                             const o3d_opts* opts, o3d_stats* stats);
+
+/* panels_affect_points (Influence.h:557-1099), recursive centroid rule of
+ * rkernel_2vs_0p(g) (Kernels.h:1028-1211), on HOST arrays.  Target radii are
+ * ignored, as in the reference (point kernels for every target, Influence.h:873). */
+int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ,
+                             const o3d_opts* opts, o3d_stats* stats);
+
+/* Same two calls on DEVICE pointers, enqueued on opts->stream, no host sync.
+ * stats->seconds is not filled (the call returns before the kernels end). */
+int o3d_points_affect_points_dev(const o3d_points* src, o3d_points* targ,
+                                 const o3d_opts* opts, o3d_stats* stats);
+int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ,
+                                 const o3d_opts* opts, o3d_stats* stats);
+
+/* Double-precision collections (the reference's Points<double>): always run the
+ * FP64 kernel, whatever opts->precision says.  Host and device variants. */
+typedef struct o3d_points64 {
+  int64_t n;
+  const double* x[3];
+  const double* s[3];
+  const double* r;
+  double* u[3];
+  double* ug[9];
+} o3d_points64;
+int o3d_points_affect_points64(const o3d_points64* src, o3d_points64* targ,
+                               const o3d_opts* opts, o3d_stats* stats);
+int o3d_points_affect_points64_dev(const o3d_points64* src, o3d_points64* targ,
+                                   const o3d_opts* opts, o3d_stats* stats);
+
+/* The reference's external-solver plugin ABI (Influence.h:33-48, called at
+ * :96-104 with gradients): host arrays, counts by pointer, point targets (no
+ * target radius), results accumulated; returns the flop count.  The core is the
+ * one set by o3d_set_external_core (default WL, the reference's compiled core).
+ * _f_ runs the compensated FP32 kernel, _d_ the FP64 kernel.  On error they
+ * return a negative value and o3d_last_error() says why. */
+float external_vel_solver_f_(int* ns, const float* sx, const float* sy, const float* sz,
+                             const float* ssx, const float* ssy, const float* ssz,
+                             const float* sr,
+                             int* nt, const float* tx, const float* ty, const float* tz,
+                             float* tu, float* tv, float* tw, float* tux,
+                             float* tvx, float* twx, float* tuy, float* tvy,
+                             float* twy, float* tuz, float* tvz, float* twz);
+float external_vel_solver_d_(int* ns, const double* sx, const double* sy, const double* sz,
+                             const double* ssx, const double* ssy, const double* ssz,
+                             const double* sr,
+                             int* nt, const double* tx, const double* ty, const double* tz,
+                             double* tu, double* tv, double* tw, double* tux,
+                             double* tvx, double* twx, double* tuy, double* tvy,
+                             double* twy, double* tuz, double* tvz, double* twz);
+int o3d_set_external_core(int core);
+
+/* Per-interaction flop count of the reference for (core, results, target kind,
+ * vortex-only or vortex+source): Kernels.h flops_0v_0bg etc. -1 if invalid. */
+int o3d_flops_per_interaction(int core, int results, int thick_target, int vortex_source);
+
+/* ---- device plumbing (no counterpart in the reference: device-resident use) -- */
+int o3d_dev_malloc(void** ptr, size_t bytes);
+int o3d_dev_free(void* ptr);
+int o3d_memcpy_htod(void* dst, const void* src, size_t bytes);
+int o3d_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int o3d_memset_dev(void* dst, int value, size_t bytes);
+int o3d_device_synchronize(void);
+int o3d_stream_create(void** stream);
+int o3d_stream_destroy(void* stream);
+int o3d_stream_synchronize(void* stream);
+int o3d_event_create(void** ev);
+int o3d_event_destroy(void* ev);
+int o3d_event_record(void* ev, void* stream);
+int o3d_event_elapsed_ms(float* ms, void* start, void* stop);
+/* device name, CU count, clock (kHz), HBM bytes */
+int o3d_device_info(char* name, int name_len, int* cus, int* clock_khz, size_t* hbm_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* O3D_BIOT_H */

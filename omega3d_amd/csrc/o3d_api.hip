@@ -1,0 +1,528 @@
+// o3d_api.hip -- the C ABI of libo3dbiot.so (declared in include/o3d_biot.h).
+//
+// Host layer between callers shaped like Omega3D's Influence.h and the gfx950
+// kernels: argument checks (status codes instead of the reference's asserts,
+// Influence.h:369,537), staging of host arrays through a grow-only device pool,
+// tile / targets-per-thread selection, and the flop accounting of
+// Influence.h:75,310,475 (N_t * (12 + F * N_s) with F from Kernels.h).
+#include "o3d_api_internal.hpp"
+
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace o3d {
+
+// ---- error reporting ---------------------------------------------------------
+static thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+void clear_error() { g_err.clear(); }
+const char* last_error() { return g_err.c_str(); }
+
+// ---- device context ------------------------------------------------------------
+struct DevPool {
+  std::vector<std::pair<void*, size_t>> bufs;  // slot -> (ptr, bytes)
+};
+
+struct Context {
+  std::mutex mu;
+  bool ready = false;
+  int device = -1;
+  int status = O3D_OK;
+  hipDeviceProp_t prop{};
+  hipStream_t stream = nullptr;
+  DevPool pool;
+};
+
+static Context g_ctx;
+static int g_ext_core = CORE_WL;
+
+static int init_locked(int device) {
+  if (g_ctx.ready && (device < 0 || device == g_ctx.device)) return O3D_OK;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) return fail(O3D_ENODEV, "no HIP device visible (%s)", hipGetErrorString(e));
+  if (device < 0) {
+    if (hipGetDevice(&device) != hipSuccess) device = 0;
+  }
+  if (device >= ndev) return fail(O3D_ENODEV, "device %d out of range (%d devices)", device, ndev);
+  HIPCHK(hipSetDevice(device));
+  HIPCHK(hipGetDeviceProperties(&g_ctx.prop, device));
+  if (std::strncmp(g_ctx.prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(O3D_ENODEV, "device %d is %s; this library is built for gfx950 only", device, g_ctx.prop.gcnArchName);
+  if (g_ctx.stream == nullptr) HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  g_ctx.device = device;
+  g_ctx.ready = true;
+  return O3D_OK;
+}
+
+int ensure_init() {
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  if (g_ctx.ready) {
+    int cur = -1;
+    if (hipGetDevice(&cur) == hipSuccess && cur != g_ctx.device) HIPCHK(hipSetDevice(g_ctx.device));
+    return O3D_OK;
+  }
+  return init_locked(-1);
+}
+
+int num_cus() { return g_ctx.ready ? g_ctx.prop.multiProcessorCount : 256; }
+
+// grow-only pool of device scratch buffers, one per slot; callers hold g_pool_mu
+static std::mutex g_pool_mu;
+static int pool_get(int slot, size_t bytes, void** out) {
+  auto& b = g_ctx.pool.bufs;
+  if ((int)b.size() <= slot) b.resize(slot + 1, {nullptr, 0});
+  if (b[slot].second < bytes) {
+    if (b[slot].first) {
+      HIPCHK(hipDeviceSynchronize());
+      HIPCHK(hipFree(b[slot].first));
+      b[slot] = {nullptr, 0};
+    }
+    size_t want = bytes + bytes / 4 + 256;
+    if (hipMalloc(&b[slot].first, want) != hipSuccess) {
+      b[slot] = {nullptr, 0};
+      return fail(O3D_ENOMEM, "device allocation of %zu bytes failed", want);
+    }
+    b[slot].second = want;
+  }
+  *out = b[slot].first;
+  return O3D_OK;
+}
+
+// ---- launch configuration -------------------------------------------------------
+// Targets per thread: enough workgroups to cover every CU several times, more
+// register reuse of each staged source when the target count allows it.
+void choose_config(const Tuning& req, int64_t nt, int64_t ns, bool grad, int prec, int* tile, int* tpt) {
+  int tl = req.tile > 0 ? req.tile : 256;
+  int tp = req.tpt;
+  if (tp <= 0) {
+    const int64_t cus = num_cus();
+    tp = 1;
+    if (prec != PREC_FP64 && nt >= (int64_t)BLOCK * 2 * cus * 4) tp = 2;
+    if (!grad && nt >= (int64_t)BLOCK * 4 * cus * 4) tp = 4;
+  }
+  (void)ns;
+  *tile = tl;
+  *tpt = tp;
+}
+
+hipError_t dispatch_p2p(int core, const P2PArgs& a, bool thick, bool grad, int prec, int tpt, hipStream_t st) {
+  switch (core) {
+    case CORE_RM: return launch_p2p_rm(a, thick, grad, prec, tpt, st);
+    case CORE_WL: return launch_p2p_wl(a, thick, grad, prec, tpt, st);
+    case CORE_EXP: return launch_p2p_exp(a, thick, grad, prec, tpt, st);
+    case CORE_V2: return launch_p2p_v2(a, thick, grad, prec, tpt, st);
+  }
+  return hipErrorInvalidValue;
+}
+hipError_t dispatch_p2p64(int core, const P2PArgs64& a, bool thick, bool grad, hipStream_t st) {
+  switch (core) {
+    case CORE_RM: return launch_p2p64_rm(a, thick, grad, st);
+    case CORE_WL: return launch_p2p64_wl(a, thick, grad, st);
+    case CORE_EXP: return launch_p2p64_exp(a, thick, grad, st);
+    case CORE_V2: return launch_p2p64_v2(a, thick, grad, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+int check_opts(const o3d_opts* o) {
+  if (!o) return fail(O3D_EINVAL, "opts is NULL");
+  if (o->core < 1 || o->core > 4) return fail(O3D_EINVAL, "unknown core %d", o->core);
+  if (o->results != O3D_VELONLY && o->results != O3D_VELANDGRAD)
+    return fail(O3D_EINVAL, "results type %d is not served (velonly=1, velandgrad=2)", o->results);
+  if (o->precision < 0 || o->precision > 2) return fail(O3D_EINVAL, "unknown precision %d", o->precision);
+  if (o->self_term != O3D_SELF_INCLUDE && o->self_term != O3D_SELF_EXCLUDE)
+    return fail(O3D_EINVAL, "self_term must be 0 or 1");
+  if (o->tile != 0 && (o->tile < 64 || o->tile > 1024 || o->tile % 64 != 0))
+    return fail(O3D_EINVAL, "tile %d: must be 0 or a multiple of 64 in [64, 1024]", o->tile);
+  if (o->tpt != 0 && o->tpt != 1 && o->tpt != 2 && o->tpt != 4) return fail(O3D_EINVAL, "tpt %d: must be 0, 1, 2 or 4", o->tpt);
+  return O3D_OK;
+}
+
+template <class P>
+int check_points(const P* p, bool as_src, bool as_tgt, bool grad, const char* what) {
+  if (!p) return fail(O3D_EINVAL, "%s is NULL", what);
+  if (p->n < 0) return fail(O3D_EINVAL, "%s: negative count", what);
+  if (p->n == 0) return O3D_OK;
+  for (int d = 0; d < 3; ++d)
+    if (!p->x[d]) return fail(O3D_EINVAL, "%s: position component %d is NULL", what, d);
+  if (as_src) {
+    for (int d = 0; d < 3; ++d)
+      if (!p->s[d]) return fail(O3D_EINVAL, "%s: strength component %d is NULL", what, d);
+    if (!p->r) return fail(O3D_EINVAL, "%s: source core radii are NULL", what);
+  }
+  if (as_tgt) {
+    for (int d = 0; d < 3; ++d)
+      if (!p->u[d]) return fail(O3D_EINVAL, "%s: velocity component %d is NULL", what, d);
+    if (grad)
+      for (int k = 0; k < 9; ++k)
+        if (!p->ug[k])
+          return fail(O3D_EINVAL, "%s: velocity-gradient component %d is NULL but results = velandgrad "
+                                  "(the reference asserts here, Influence.h:369)", what, k);
+  }
+  return O3D_OK;
+}
+
+template <class P>
+int check_self(const P* src, const P* targ, const o3d_opts* o) {
+  if (o->self_term == O3D_SELF_EXCLUDE && src->n != targ->n)
+    return fail(O3D_EINVAL, "self_term exclude needs sources and targets to be one collection (n %lld vs %lld)",
+                (long long)src->n, (long long)targ->n);
+  return O3D_OK;
+}
+
+double flops_of(int core, bool grad, bool thick, int64_t nt, int64_t ns) {
+  const int f = o3d_flops_per_interaction(core, grad ? O3D_VELANDGRAD : O3D_VELONLY, thick, 0);
+  return (double)nt * ((grad ? 12.0 : 3.0) + (double)f * (double)ns);
+}
+
+// device-pointer point->point; all pointers device, float
+int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun) {
+  const bool grad = o->results == O3D_VELANDGRAD;
+  const bool thick = targ->r != nullptr;
+  if (src->n == 0 || targ->n == 0) return O3D_OK;
+  if (o->precision == PREC_FP64) return fail(O3D_EINVAL, "internal: FP64 routed to p2p_dev");
+  P2PArgs a{};
+  a.s = SrcView{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
+  a.t.x = targ->x[0]; a.t.y = targ->x[1]; a.t.z = targ->x[2]; a.t.r = targ->r;
+  a.t.u = targ->u[0]; a.t.v = targ->u[1]; a.t.w = targ->u[2];
+  for (int k = 0; k < 9; ++k) a.t.g[k] = grad ? targ->ug[k] : nullptr;
+  a.ns = src->n;
+  a.nt = targ->n;
+  a.self_excl = o->self_term == O3D_SELF_EXCLUDE;
+  Tuning req{o->tile, o->tpt};
+  if (tun) req = *tun;
+  int tile, tpt;
+  choose_config(req, a.nt, a.ns, grad, o->precision, &tile, &tpt);
+  a.tile = tile;
+  hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
+  if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
+  return O3D_OK;
+}
+
+int p2p64_dev(const o3d_points64* src, o3d_points64* targ, const o3d_opts* o, hipStream_t st) {
+  const bool grad = o->results == O3D_VELANDGRAD;
+  const bool thick = targ->r != nullptr;
+  if (src->n == 0 || targ->n == 0) return O3D_OK;
+  P2PArgs64 a{};
+  a.s = SrcView64{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
+  a.t.x = targ->x[0]; a.t.y = targ->x[1]; a.t.z = targ->x[2]; a.t.r = targ->r;
+  a.t.u = targ->u[0]; a.t.v = targ->u[1]; a.t.w = targ->u[2];
+  for (int k = 0; k < 9; ++k) a.t.g[k] = grad ? targ->ug[k] : nullptr;
+  a.ns = src->n;
+  a.nt = targ->n;
+  a.tile = 128;
+  a.self_excl = o->self_term == O3D_SELF_EXCLUDE;
+  hipError_t e = dispatch_p2p64(o->core, a, thick, grad, st);
+  if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p64 launch failed: %s", hipGetErrorString(e));
+  return O3D_OK;
+}
+
+// host arrays -> pool -> kernel -> host arrays (accumulate)
+template <class T, class P>
+int p2p_host_impl(const P* src, P* targ, const o3d_opts* o, o3d_stats* stats) {
+  const bool grad = o->results == O3D_VELANDGRAD;
+  const bool thick = targ->r != nullptr;
+  const int64_t ns = src->n, nt = targ->n;
+  if (ns == 0 || nt == 0) return O3D_OK;
+  std::lock_guard<std::mutex> lk(g_pool_mu);
+  hipStream_t st = g_ctx.stream;
+  const size_t sb = sizeof(T) * (size_t)ns, tb = sizeof(T) * (size_t)nt;
+  void* dsrc;
+  void* dtgt;
+  int rc;
+  if ((rc = pool_get(0, 7 * sb, &dsrc)) != O3D_OK) return rc;
+  if ((rc = pool_get(1, (grad ? 16 : 7) * tb, &dtgt)) != O3D_OK) return rc;
+  T* s = (T*)dsrc;
+  T* t = (T*)dtgt;
+  P ds{}, dt{};
+  ds.n = ns;
+  for (int d = 0; d < 3; ++d) { ds.x[d] = s + d * ns; ds.s[d] = s + (3 + d) * ns; }
+  ds.r = s + 6 * ns;
+  for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync((void*)ds.x[d], src->x[d], sb, hipMemcpyHostToDevice, st));
+  for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync((void*)ds.s[d], src->s[d], sb, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync((void*)ds.r, src->r, sb, hipMemcpyHostToDevice, st));
+  dt.n = nt;
+  for (int d = 0; d < 3; ++d) { dt.x[d] = t + d * nt; dt.u[d] = t + (3 + d) * nt; }
+  dt.r = thick ? t + 6 * nt : nullptr;
+  if (grad) for (int k = 0; k < 9; ++k) dt.ug[k] = t + (7 + k) * nt;
+  for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync((void*)dt.x[d], targ->x[d], tb, hipMemcpyHostToDevice, st));
+  if (thick) HIPCHK(hipMemcpyAsync((void*)dt.r, targ->r, tb, hipMemcpyHostToDevice, st));
+  for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync(dt.u[d], targ->u[d], tb, hipMemcpyHostToDevice, st));
+  if (grad) for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(dt.ug[k], targ->ug[k], tb, hipMemcpyHostToDevice, st));
+  if constexpr (sizeof(T) == 4) rc = p2p_dev((const o3d_points*)&ds, (o3d_points*)&dt, o, st, nullptr);
+  else rc = p2p64_dev((const o3d_points64*)&ds, (o3d_points64*)&dt, o, st);
+  if (rc != O3D_OK) return rc;
+  for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync(targ->u[d], dt.u[d], tb, hipMemcpyDeviceToHost, st));
+  if (grad) for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(targ->ug[k], dt.ug[k], tb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  (void)stats;
+  return O3D_OK;
+}
+
+// float host arrays through the FP64 kernel: promote, run, accumulate back
+int p2p_host_fp64_from_f32(const o3d_points* src, o3d_points* targ, const o3d_opts* o) {
+  const bool grad = o->results == O3D_VELANDGRAD;
+  const int64_t ns = src->n, nt = targ->n;
+  std::vector<double> sb(7 * (size_t)ns), tb(16 * (size_t)nt, 0.0);
+  o3d_points64 s{}, t{};
+  s.n = ns;
+  for (int d = 0; d < 3; ++d) {
+    s.x[d] = sb.data() + d * ns;
+    s.s[d] = sb.data() + (3 + d) * ns;
+    for (int64_t j = 0; j < ns; ++j) {
+      sb[d * ns + j] = src->x[d][j];
+      sb[(3 + d) * ns + j] = src->s[d][j];
+    }
+  }
+  s.r = sb.data() + 6 * ns;
+  for (int64_t j = 0; j < ns; ++j) sb[6 * ns + j] = src->r[j];
+  t.n = nt;
+  for (int d = 0; d < 3; ++d) {
+    t.x[d] = tb.data() + d * nt;
+    t.u[d] = tb.data() + (3 + d) * nt;
+    for (int64_t j = 0; j < nt; ++j) tb[d * nt + j] = targ->x[d][j];
+  }
+  t.r = nullptr;
+  if (targ->r) {
+    t.r = tb.data() + 6 * nt;
+    for (int64_t j = 0; j < nt; ++j) tb[6 * nt + j] = targ->r[j];
+  }
+  if (grad) for (int k = 0; k < 9; ++k) t.ug[k] = tb.data() + (7 + k) * nt;
+  int rc = p2p_host_impl<double>(&s, &t, o, nullptr);
+  if (rc != O3D_OK) return rc;
+  for (int d = 0; d < 3; ++d)
+    for (int64_t j = 0; j < nt; ++j) targ->u[d][j] = (float)((double)targ->u[d][j] + t.u[d][j]);
+  if (grad)
+    for (int k = 0; k < 9; ++k)
+      for (int64_t j = 0; j < nt; ++j) targ->ug[k][j] = (float)((double)targ->ug[k][j] + t.ug[k][j]);
+  return O3D_OK;
+}
+
+void fill_stats(o3d_stats* stats, double secs, double flops, double inter) {
+  if (!stats) return;
+  stats->seconds = secs;
+  stats->flops = flops;
+  stats->interactions = inter;
+}
+
+int ext_core() { return g_ext_core; }
+void set_ext_core(int c) { g_ext_core = c; }
+Context& ctx() { return g_ctx; }
+hipStream_t lib_stream() { return g_ctx.stream; }
+std::mutex& pool_mutex() { return g_pool_mu; }
+int pool_buffer(int slot, size_t bytes, void** out) { return pool_get(slot, bytes, out); }
+
+}  // namespace o3d
+
+using namespace o3d;
+
+extern "C" {
+
+int o3d_abi_version(void) { return O3D_ABI_VERSION; }
+const char* o3d_last_error(void) { return last_error(); }
+
+void o3d_default_opts(o3d_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->core = O3D_CORE_WL;
+  o->results = O3D_VELANDGRAD;
+  o->precision = O3D_PREC_COMPENSATED;
+  o->self_term = O3D_SELF_INCLUDE;
+  o->maxlev = -1;
+}
+
+int o3d_init(int device) {
+  std::lock_guard<std::mutex> lk(g_ctx.mu);
+  clear_error();
+  return init_locked(device);
+}
+
+int o3d_flops_per_interaction(int core, int results, int thick, int vs) {
+  // CoreFunc.h flops_tv_grads / tp_grads / tv_nograds / tp_nograds, and the
+  // Kernels.h pair counts 54 (0v_*g), 79 (0vs_*g), 23 (0v_*), 29 (0vs_*)
+  static const int tvg[5] = {0, 9, 16, 14, 13}, tpg[5] = {0, 7, 14, 11, 10};
+  static const int tvn[5] = {0, 7, 10, 12, 11}, tpn[5] = {0, 5, 8, 9, 8};
+  if (core < 1 || core > 4) return -1;
+  if (results == O3D_VELANDGRAD) return (vs ? 79 : 54) + (thick ? tvg[core] : tpg[core]);
+  if (results == O3D_VELONLY) return (vs ? 29 : 23) + (thick ? tvn[core] : tpn[core]);
+  return -1;
+}
+
+int o3d_points_affect_points(const o3d_points* src, o3d_points* targ, const o3d_opts* o, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  const bool grad = o->results == O3D_VELANDGRAD;
+  if ((rc = check_points(src, true, false, grad, "src")) != O3D_OK) return rc;
+  if ((rc = check_points(targ, false, true, grad, "targ")) != O3D_OK) return rc;
+  if ((rc = check_self(src, targ, o)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  if (o->precision == PREC_FP64) rc = src->n && targ->n ? p2p_host_fp64_from_f32(src, targ, o) : O3D_OK;
+  else rc = p2p_host_impl<float>(src, targ, o, stats);
+  if (rc != O3D_OK) return rc;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, secs, flops_of(o->core, grad, targ->r != nullptr, targ->n, src->n),
+             (double)targ->n * (double)src->n);
+  return O3D_OK;
+}
+
+int o3d_points_affect_points_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  const bool grad = o->results == O3D_VELANDGRAD;
+  if ((rc = check_points(src, true, false, grad, "src")) != O3D_OK) return rc;
+  if ((rc = check_points(targ, false, true, grad, "targ")) != O3D_OK) return rc;
+  if ((rc = check_self(src, targ, o)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  if (o->precision == PREC_FP64)
+    return fail(O3D_EINVAL, "FP64 on device float arrays: use o3d_points_affect_points64_dev with double arrays");
+  rc = p2p_dev(src, targ, o, (hipStream_t)o->stream, nullptr);
+  if (rc != O3D_OK) return rc;
+  fill_stats(stats, 0.0, flops_of(o->core, grad, targ->r != nullptr, targ->n, src->n), (double)targ->n * (double)src->n);
+  return O3D_OK;
+}
+
+int o3d_points_affect_points64(const o3d_points64* src, o3d_points64* targ, const o3d_opts* o, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  const bool grad = o->results == O3D_VELANDGRAD;
+  if ((rc = check_points(src, true, false, grad, "src")) != O3D_OK) return rc;
+  if ((rc = check_points(targ, false, true, grad, "targ")) != O3D_OK) return rc;
+  if ((rc = check_self(src, targ, o)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  rc = p2p_host_impl<double>(src, targ, o, stats);
+  if (rc != O3D_OK) return rc;
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, secs, flops_of(o->core, grad, targ->r != nullptr, targ->n, src->n), (double)targ->n * (double)src->n);
+  return O3D_OK;
+}
+
+int o3d_points_affect_points64_dev(const o3d_points64* src, o3d_points64* targ, const o3d_opts* o, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  const bool grad = o->results == O3D_VELANDGRAD;
+  if ((rc = check_points(src, true, false, grad, "src")) != O3D_OK) return rc;
+  if ((rc = check_points(targ, false, true, grad, "targ")) != O3D_OK) return rc;
+  if ((rc = check_self(src, targ, o)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  rc = p2p64_dev(src, targ, o, (hipStream_t)o->stream);
+  if (rc != O3D_OK) return rc;
+  fill_stats(stats, 0.0, flops_of(o->core, grad, targ->r != nullptr, targ->n, src->n), (double)targ->n * (double)src->n);
+  return O3D_OK;
+}
+
+int o3d_set_external_core(int core) {
+  if (core < 1 || core > 4) return fail(O3D_EINVAL, "unknown core %d", core);
+  set_ext_core(core);
+  return O3D_OK;
+}
+
+float external_vel_solver_f_(int* ns, const float* sx, const float* sy, const float* sz,
+                             const float* ssx, const float* ssy, const float* ssz, const float* sr,
+                             int* nt, const float* tx, const float* ty, const float* tz,
+                             float* tu, float* tv, float* tw, float* tux, float* tvx, float* twx,
+                             float* tuy, float* tvy, float* twy, float* tuz, float* tvz, float* twz) {
+  if (!ns || !nt) { fail(O3D_EINVAL, "counts are NULL"); return -1.0f; }
+  o3d_points s{}, t{};
+  s.n = *ns; s.x[0] = sx; s.x[1] = sy; s.x[2] = sz; s.s[0] = ssx; s.s[1] = ssy; s.s[2] = ssz; s.r = sr;
+  t.n = *nt; t.x[0] = tx; t.x[1] = ty; t.x[2] = tz; t.r = nullptr;
+  t.u[0] = tu; t.u[1] = tv; t.u[2] = tw;
+  float* g[9] = {tux, tvx, twx, tuy, tvy, twy, tuz, tvz, twz};
+  for (int k = 0; k < 9; ++k) t.ug[k] = g[k];
+  o3d_opts o;
+  o3d_default_opts(&o);
+  o.core = ext_core();
+  o3d_stats st{};
+  if (o3d_points_affect_points(&s, &t, &o, &st) != O3D_OK) return -1.0f;
+  return (float)st.flops;
+}
+
+float external_vel_solver_d_(int* ns, const double* sx, const double* sy, const double* sz,
+                             const double* ssx, const double* ssy, const double* ssz, const double* sr,
+                             int* nt, const double* tx, const double* ty, const double* tz,
+                             double* tu, double* tv, double* tw, double* tux, double* tvx, double* twx,
+                             double* tuy, double* tvy, double* twy, double* tuz, double* tvz, double* twz) {
+  if (!ns || !nt) { fail(O3D_EINVAL, "counts are NULL"); return -1.0f; }
+  o3d_points64 s{}, t{};
+  s.n = *ns; s.x[0] = sx; s.x[1] = sy; s.x[2] = sz; s.s[0] = ssx; s.s[1] = ssy; s.s[2] = ssz; s.r = sr;
+  t.n = *nt; t.x[0] = tx; t.x[1] = ty; t.x[2] = tz; t.r = nullptr;
+  t.u[0] = tu; t.u[1] = tv; t.u[2] = tw;
+  double* g[9] = {tux, tvx, twx, tuy, tvy, twy, tuz, tvz, twz};
+  for (int k = 0; k < 9; ++k) t.ug[k] = g[k];
+  o3d_opts o;
+  o3d_default_opts(&o);
+  o.core = ext_core();
+  o.precision = O3D_PREC_FP64;
+  o3d_stats st{};
+  if (o3d_points_affect_points64(&s, &t, &o, &st) != O3D_OK) return -1.0f;
+  return (float)st.flops;
+}
+
+// ---- device plumbing --------------------------------------------------------------
+int o3d_dev_malloc(void** p, size_t bytes) {
+  clear_error();
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (!p) return fail(O3D_EINVAL, "NULL out pointer");
+  if (hipMalloc(p, bytes ? bytes : 1) != hipSuccess) return fail(O3D_ENOMEM, "hipMalloc(%zu) failed", bytes);
+  return O3D_OK;
+}
+int o3d_dev_free(void* p) { HIPCHK(hipFree(p)); return O3D_OK; }
+int o3d_memcpy_htod(void* d, const void* s, size_t b) { HIPCHK(hipMemcpy(d, s, b, hipMemcpyHostToDevice)); return O3D_OK; }
+int o3d_memcpy_dtoh(void* d, const void* s, size_t b) { HIPCHK(hipMemcpy(d, s, b, hipMemcpyDeviceToHost)); return O3D_OK; }
+int o3d_memset_dev(void* d, int v, size_t b) { HIPCHK(hipMemset(d, v, b)); return O3D_OK; }
+int o3d_device_synchronize(void) { HIPCHK(hipDeviceSynchronize()); return O3D_OK; }
+int o3d_stream_create(void** s) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIPCHK(hipStreamCreateWithFlags((hipStream_t*)s, hipStreamNonBlocking));
+  return O3D_OK;
+}
+int o3d_stream_destroy(void* s) { HIPCHK(hipStreamDestroy((hipStream_t)s)); return O3D_OK; }
+int o3d_stream_synchronize(void* s) { HIPCHK(hipStreamSynchronize((hipStream_t)s)); return O3D_OK; }
+int o3d_event_create(void** e) {
+  int rc = ensure_init();
+  if (rc) return rc;
+  HIPCHK(hipEventCreate((hipEvent_t*)e));
+  return O3D_OK;
+}
+int o3d_event_destroy(void* e) { HIPCHK(hipEventDestroy((hipEvent_t)e)); return O3D_OK; }
+int o3d_event_record(void* e, void* s) { HIPCHK(hipEventRecord((hipEvent_t)e, (hipStream_t)s)); return O3D_OK; }
+int o3d_event_elapsed_ms(float* ms, void* a, void* b) {
+  HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
+  return O3D_OK;
+}
+int o3d_device_info(char* name, int len, int* cus, int* clk, size_t* hbm) {
+  clear_error();
+  int rc = ensure_init();
+  if (rc) return rc;
+  if (name && len > 0) {
+    std::snprintf(name, (size_t)len, "%s %s", g_ctx.prop.name, g_ctx.prop.gcnArchName);
+  }
+  if (cus) *cus = g_ctx.prop.multiProcessorCount;
+  if (clk) *clk = g_ctx.prop.clockRate;
+  if (hbm) *hbm = g_ctx.prop.totalGlobalMem;
+  return O3D_OK;
+}
+
+}  // extern "C"

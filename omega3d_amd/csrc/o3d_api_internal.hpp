@@ -1,0 +1,37 @@
+// o3d_api_internal.hpp -- declarations shared by the host-side translation units.
+#pragma once
+
+#include "o3d_biot.h"
+#include "p2p_kernels_decl.hpp"
+
+#include <mutex>
+
+namespace o3d {
+
+int fail(int code, const char* fmt, ...);
+void clear_error();
+const char* last_error();
+int ensure_init();
+int num_cus();
+hipStream_t lib_stream();
+std::mutex& pool_mutex();
+int pool_buffer(int slot, size_t bytes, void** out);
+
+struct Tuning {
+  int tile;
+  int tpt;
+};
+void choose_config(const Tuning& req, int64_t nt, int64_t ns, bool grad, int prec, int* tile, int* tpt);
+hipError_t dispatch_p2p(int core, const P2PArgs& a, bool thick, bool grad, int prec, int tpt, hipStream_t st);
+double flops_of(int core, bool grad, bool thick, int64_t nt, int64_t ns);
+int check_opts(const o3d_opts* o);
+void fill_stats(o3d_stats* stats, double secs, double flops, double inter);
+int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);
+
+}  // namespace o3d
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return ::o3d::fail(O3D_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)

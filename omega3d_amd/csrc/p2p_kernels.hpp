@@ -1,0 +1,448 @@
+// p2p_kernels.hpp -- point -> point direct sums (velocity and velocity gradient)
+// for gfx950.  Included once per core by p2p_<core>.hip.
+//
+// Math follows Omega3D's pair kernels and cores (Kernels.h:50-112, 155-290;
+// CoreFunc.h:43-341); the restructuring for the GPU is:
+//   * the skew part of the gradient, sum_j K_j skew(alpha_j), is accumulated as
+//     one 3-vector W = sum_j K_j alpha_j and spread into the six off-diagonal
+//     entries once per target (3 FMAs per pair instead of 6);
+//   * RM / WL / V2 use one v_rsq_f32 per pair; the compact-exponential core
+//     evaluates its far branch (q = rho^3/(ss^3+st^3) > 16, CoreFunc.h:121,169)
+//     first and enters the exp/near-field code only when some lane of the wave
+//     needs it; every lane's result is independent of the other lanes;
+//   * for q < 1 the exponential core uses Taylor series of (1-e^-q)/q and of
+//     (e^-q - (1-e^-q)/q)/q instead of 1 - exp(-q), which cancels in FP32.
+//
+// Layout: sources are staged per tile in LDS as two float4 per source,
+// {x, y, z, core_term(sigma)} and {ax, ay, az, 0}; every lane of the workgroup
+// reads the same source (LDS broadcast, no bank conflicts).  Targets live in
+// VGPRs: TPT targets per lane, strided by the workgroup size so that loads and
+// stores stay coalesced.
+#pragma once
+#include "o3d_common.hpp"
+
+namespace o3d {
+
+// ---- compact exponential: the rare path (exp, series and near branch) -------
+__device__ __forceinline__ float horner_f(float q) {
+  // (1 - e^-q)/q = sum_n (-1)^n q^n/(n+1)!,  n = 0..10   (|trunc| < 3e-9 for q < 1)
+  float p = (float)(1.0 / 39916800.0);
+  p = fmaf(p, q, (float)(-1.0 / 3628800.0));
+  p = fmaf(p, q, (float)(1.0 / 362880.0));
+  p = fmaf(p, q, (float)(-1.0 / 40320.0));
+  p = fmaf(p, q, (float)(1.0 / 5040.0));
+  p = fmaf(p, q, (float)(-1.0 / 720.0));
+  p = fmaf(p, q, (float)(1.0 / 120.0));
+  p = fmaf(p, q, (float)(-1.0 / 24.0));
+  p = fmaf(p, q, (float)(1.0 / 6.0));
+  p = fmaf(p, q, -0.5f);
+  return fmaf(p, q, 1.0f);
+}
+__device__ __forceinline__ float horner_h(float q) {
+  // (f(q) - e^-q)/q = sum_m (-1)^m q^m (m+1)/(m+2)!,  m = 0..10
+  float p = (float)(1.0 / 43545600.0);
+  p = fmaf(p, q, (float)(-1.0 / 3991680.0));
+  p = fmaf(p, q, (float)(1.0 / 403200.0));
+  p = fmaf(p, q, (float)(-1.0 / 45360.0));
+  p = fmaf(p, q, (float)(1.0 / 5760.0));
+  p = fmaf(p, q, (float)(-1.0 / 840.0));
+  p = fmaf(p, q, (float)(1.0 / 144.0));
+  p = fmaf(p, q, (float)(-1.0 / 30.0));
+  p = fmaf(p, q, 0.125f);
+  p = fmaf(p, q, (float)(-1.0 / 3.0));
+  return fmaf(p, q, 0.5f);
+}
+
+// K = (1 - exp(-q))/rho^3,  B = 3 (c e^-q - K)/rho^2  with c = 1/cs, q = rho^3 c;
+// q < 0.001: K = c, B = -1.5 rho c^2 (CoreFunc.h:124-126, 172-174).
+__device__ __forceinline__ void exp_slow(float d2m, float ir, float ir2, float ood3, float cs,
+                                         bool far, float& K, float& B) {
+  const float c = frcp(cs);
+  const float dist = d2m * ir;
+  const float q = (d2m * dist) * c;
+  const float e = fexp2(q * -1.44269504088896341f);
+  const float kg = fmaf(-ood3, e, ood3);
+  const float bg = 3.0f * fmaf(c, e, -kg) * ir2;
+  const float ks = c * horner_f(q);
+  const float bs = (-3.0f * c) * (c * dist) * horner_h(q);
+  const bool small = q < 1.0f;
+  float kn = small ? ks : kg;
+  float bn = small ? bs : bg;
+  const bool near = q < 0.001f;
+  kn = near ? c : kn;
+  bn = near ? (-1.5f * dist) * c * c : bn;
+  K = far ? K : kn;
+  B = far ? B : bn;
+}
+
+// K and B of one pair from rho^2 (d2) and the pair's core parameter st
+// (sum of the two particles' core_term).
+template <int CORE, bool GRAD>
+__device__ __forceinline__ void kb(float d2, float st, float& K, float& B) {
+  if (CORE == CORE_RM) {                       // CoreFunc.h:65-71
+    const float r2 = d2 + st;
+    const float ir = frsq(r2);
+    const float ir2 = ir * ir;
+    K = ir2 * ir;
+    B = K * (ir2 * -3.0f);
+  } else if (CORE == CORE_WL) {                // CoreFunc.h:265-274
+    const float dd = d2 + st;
+    const float top = fmaf(2.5f, st, d2);
+    const float ir = frsq(dd);
+    const float ir2 = ir * ir;
+    const float dn5 = ir2 * ir2 * ir;
+    K = top * dn5;
+    B = dn5 * fmaf(-5.0f, top * ir2, 2.0f);
+  } else if (CORE == CORE_V2) {                // CoreFunc.h:317-328
+    const float den = fmaf(d2, d2, st);
+    const float rs = frsq(den);
+    K = rs * fsqrt(rs);
+    B = K * (rs * -3.0f);
+  } else {                                     // CoreFunc.h:225-236
+    const float d2m = fmaxf(d2, 1.17549435e-38f);
+    const float ir = frsq(d2m);
+    const float ir2 = ir * ir;
+    const float ood3 = ir2 * ir;
+    K = ood3;
+    B = ood3 * (ir2 * -3.0f);
+    const bool far = ood3 * st < 0.0625f;      // rho^3/(ss^3+st^3) > 16
+    if (__builtin_expect(wave_any(!far), 0)) exp_slow(d2m, ir, ir2, ood3, st, far, K, B);
+  }
+}
+
+// acc: [0..2] velocity, [3..11] sum_j B_j d_j (x) (alpha_j x d_j) in ug order,
+//      [12..14] W = sum_j K_j alpha_j
+template <int CORE, bool GRAD>
+__device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt, const float4 P,
+                                         const float4 A, float* acc) {
+  const float dx = tx - P.x;
+  const float dy = ty - P.y;
+  const float dz = tz - P.z;
+  const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+  float K, B;
+  kb<CORE, GRAD>(d2, P.w + tt, K, B);
+  const float cx = fmaf(dz, A.y, -dy * A.z);
+  const float cy = fmaf(dx, A.z, -dz * A.x);
+  const float cz = fmaf(dy, A.x, -dx * A.y);
+  acc[0] = fmaf(K, cx, acc[0]);
+  acc[1] = fmaf(K, cy, acc[1]);
+  acc[2] = fmaf(K, cz, acc[2]);
+  if (GRAD) {
+    const float bx = B * cx, by = B * cy, bz = B * cz;
+    acc[3] = fmaf(dx, bx, acc[3]);
+    acc[4] = fmaf(dx, by, acc[4]);
+    acc[5] = fmaf(dx, bz, acc[5]);
+    acc[6] = fmaf(dy, bx, acc[6]);
+    acc[7] = fmaf(dy, by, acc[7]);
+    acc[8] = fmaf(dy, bz, acc[8]);
+    acc[9] = fmaf(dz, bx, acc[9]);
+    acc[10] = fmaf(dz, by, acc[10]);
+    acc[11] = fmaf(dz, bz, acc[11]);
+    acc[12] = fmaf(K, A.x, acc[12]);
+    acc[13] = fmaf(K, A.y, acc[13]);
+    acc[14] = fmaf(K, A.z, acc[14]);
+  }
+}
+
+// skew(W) added to the dyad part gives the gradient (Kernels.h:183-192):
+// g1 += Wz, g2 -= Wy, g3 -= Wz, g5 += Wx, g6 += Wy, g7 -= Wx
+__device__ __forceinline__ void fold_tile_comp(float* acc, float* hi, float* lo, double* gd, bool grad) {
+  two_sum(hi[0], lo[0], acc[0]);
+  two_sum(hi[1], lo[1], acc[1]);
+  two_sum(hi[2], lo[2], acc[2]);
+  acc[0] = acc[1] = acc[2] = 0.0f;
+  if (grad) {
+    const double wx = acc[12], wy = acc[13], wz = acc[14];
+    gd[0] += (double)acc[3];
+    gd[1] += (double)acc[4] + wz;
+    gd[2] += (double)acc[5] - wy;
+    gd[3] += (double)acc[6] - wz;
+    gd[4] += (double)acc[7];
+    gd[5] += (double)acc[8] + wx;
+    gd[6] += (double)acc[9] + wy;
+    gd[7] += (double)acc[10] - wx;
+    gd[8] += (double)acc[11];
+#pragma unroll
+    for (int c = 3; c < 15; ++c) acc[c] = 0.0f;
+  }
+}
+
+// stage sources [s0, s0+tile) into LDS as {x,y,z,term} {ax,ay,az,0}
+template <int CORE>
+__device__ __forceinline__ void stage_tile(const SrcView& s, int64_t s0, int cnt, int tile, float4* sh) {
+  for (int j = threadIdx.x; j < tile; j += BLOCK) {
+    float4 P = make_float4(0.f, 0.f, 0.f, 1.f), A = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j < cnt) {
+      const int64_t g = s0 + j;
+      P = make_float4(s.x[g], s.y[g], s.z[g], core_term<CORE>(s.r[g]));
+      A = make_float4(s.ax[g], s.ay[g], s.az[g], 0.f);
+    }
+    sh[2 * j] = P;
+    sh[2 * j + 1] = A;
+  }
+}
+
+// One chunk of BLOCK*TPT targets [t_base, t_base + BLOCK*TPT) against sources
+// [0, ns) of (s, t).  Shared by the single-collection and the batched kernels so
+// that a target's result does not depend on which launch computed it.
+template <int CORE, bool THICK, bool GRAD, int PREC, int TPT>
+__device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, int64_t ns, int64_t nt,
+                                          int64_t t_base, int tile, bool self_excl, float4* sh) {
+  const int tid = threadIdx.x;
+  float tx[TPT], ty[TPT], tz[TPT], tt[TPT];
+  int64_t ti[TPT];
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) {
+    const int64_t i = t_base + tid + (int64_t)k * BLOCK;
+    ti[k] = i;
+    const int64_t ic = i < nt ? i : nt - 1;    // idle lanes mirror a real target
+    tx[k] = t.x[ic]; ty[k] = t.y[ic]; tz[k] = t.z[ic];
+    tt[k] = THICK ? core_term<CORE>(t.r[ic]) : 0.0f;
+  }
+  float acc[TPT][15];
+  float hi[TPT][3], lo[TPT][3];
+  double gd[TPT][9];
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) {
+#pragma unroll
+    for (int c = 0; c < 15; ++c) acc[k][c] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { hi[k][c] = 0.0f; lo[k][c] = 0.0f; }
+#pragma unroll
+    for (int c = 0; c < 9; ++c) gd[k][c] = 0.0;
+  }
+
+  for (int64_t s0 = 0; s0 < ns; s0 += tile) {
+    const int cnt = (int)((ns - s0) < tile ? (ns - s0) : tile);
+    __syncthreads();
+    stage_tile<CORE>(s, s0, cnt, tile, sh);
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const float4 P = sh[2 * j];
+      const float4 A = sh[2 * j + 1];
+#pragma unroll
+      for (int k = 0; k < TPT; ++k) pair_acc<CORE, GRAD>(tx[k], ty[k], tz[k], tt[k], P, A, acc[k]);
+    }
+    if (PREC == PREC_COMP) {
+#pragma unroll
+      for (int k = 0; k < TPT; ++k) fold_tile_comp(acc[k], hi[k], lo[k], gd[k], GRAD);
+    }
+  }
+
+  // self pair j == i: velocity and dyad parts vanish at d = 0; only K(0) alpha_i
+  // entered W.  Evaluated with the same code as in the loop.
+  float ks[TPT], sax[TPT], say[TPT], saz[TPT];
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) {
+    ks[k] = 0.0f; sax[k] = say[k] = saz[k] = 0.0f;
+    if (GRAD && self_excl) {
+      const int64_t ic = ti[k] < nt ? ti[k] : nt - 1;
+      float B;
+      kb<CORE, GRAD>(0.0f, core_term<CORE>(s.r[ic]) + tt[k], ks[k], B);
+      sax[k] = s.ax[ic]; say[k] = s.ay[ic]; saz[k] = s.az[ic];
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) {
+    const int64_t i = ti[k];
+    if (i >= nt) continue;
+    if (PREC == PREC_FP32) {
+      t.u[i] += acc[k][0];
+      t.v[i] += acc[k][1];
+      t.w[i] += acc[k][2];
+      if (GRAD) {
+        const float wx = fmaf(-ks[k], sax[k], acc[k][12]);
+        const float wy = fmaf(-ks[k], say[k], acc[k][13]);
+        const float wz = fmaf(-ks[k], saz[k], acc[k][14]);
+        t.g[0][i] += acc[k][3];
+        t.g[1][i] += acc[k][4] + wz;
+        t.g[2][i] += acc[k][5] - wy;
+        t.g[3][i] += acc[k][6] - wz;
+        t.g[4][i] += acc[k][7];
+        t.g[5][i] += acc[k][8] + wx;
+        t.g[6][i] += acc[k][9] + wy;
+        t.g[7][i] += acc[k][10] - wx;
+        t.g[8][i] += acc[k][11];
+      }
+    } else {
+      t.u[i] = (float)((double)t.u[i] + (double)hi[k][0] + (double)lo[k][0]);
+      t.v[i] = (float)((double)t.v[i] + (double)hi[k][1] + (double)lo[k][1]);
+      t.w[i] = (float)((double)t.w[i] + (double)hi[k][2] + (double)lo[k][2]);
+      if (GRAD) {
+        const double kx = (double)ks[k] * sax[k], ky = (double)ks[k] * say[k], kz = (double)ks[k] * saz[k];
+        t.g[0][i] = (float)((double)t.g[0][i] + gd[k][0]);
+        t.g[1][i] = (float)((double)t.g[1][i] + (gd[k][1] - kz));
+        t.g[2][i] = (float)((double)t.g[2][i] + (gd[k][2] + ky));
+        t.g[3][i] = (float)((double)t.g[3][i] + (gd[k][3] + kz));
+        t.g[4][i] = (float)((double)t.g[4][i] + gd[k][4]);
+        t.g[5][i] = (float)((double)t.g[5][i] + (gd[k][5] - kx));
+        t.g[6][i] = (float)((double)t.g[6][i] + (gd[k][6] - ky));
+        t.g[7][i] = (float)((double)t.g[7][i] + (gd[k][7] + kx));
+        t.g[8][i] = (float)((double)t.g[8][i] + gd[k][8]);
+      }
+    }
+  }
+}
+
+template <int CORE, bool THICK, bool GRAD, int PREC, int TPT>
+__global__ __launch_bounds__(BLOCK) void k_p2p(P2PArgs a) {
+  extern __shared__ float4 sh[];
+  const int64_t t_base = (int64_t)blockIdx.x * (BLOCK * TPT);
+  p2p_chunk<CORE, THICK, GRAD, PREC, TPT>(a.s, a.t, a.ns, a.nt, t_base, a.tile, a.self_excl != 0, sh);
+}
+
+// ===========================================================================
+// FP64 oracle kernel: the reference's formulas (CoreFunc.h, Kernels.h) with
+// every operation in double, including its branches; double arrays in and out.
+// ===========================================================================
+
+// r2/d2 etc. exactly as CoreFunc.h orders them; sr, tr are the two core radii
+template <int CORE, bool THICK, bool GRAD>
+__device__ __forceinline__ void kb64(double d2, double sr, double tr, double& K, double& B) {
+  if (CORE == CORE_RM) {
+    const double r2 = THICK ? d2 + sr * sr + tr * tr : d2 + sr * sr;
+    K = 1.0 / (r2 * sqrt(r2));
+    B = -3.0 * K * (1.0 / r2);
+  } else if (CORE == CORE_WL) {
+    const double r2 = THICK ? sr * sr + tr * tr : sr * sr;
+    const double dd = d2 + r2;
+    const double top = d2 + 2.5 * r2;
+    const double dn5 = 1.0 / (dd * dd * sqrt(dd));
+    K = top * dn5;
+    B = 2.0 * dn5 - 5.0 * top * dn5 / dd;
+  } else if (CORE == CORE_V2) {
+    const double s2 = sr * sr, t2 = tr * tr;
+    const double den = THICK ? d2 * d2 + s2 * s2 + t2 * t2 : d2 * d2 + s2 * s2;
+    const double q = sqrt(den);
+    K = 1.0 / (q * sqrt(q));
+    B = -3.0 * K * (1.0 / sqrt(den));
+  } else {
+    const double dist = sqrt(d2);
+    const double cf = 1.0 / (THICK ? sr * sr * sr + tr * tr * tr : sr * sr * sr);
+    if (GRAD) {
+      const double d3 = d2 * dist;
+      const double q = d3 * cf;
+      const double ood3 = 1.0 / d3;
+      if (q > 16.0) { K = ood3; B = -3.0 * K / d2; }
+      else if (q < (double)0.001f) { K = cf; B = -1.5 * dist * K * K; }
+      else { const double e = exp(-q); K = ood3 * (1.0 - e); B = 3.0 * (cf * e - K) / d2; }
+    } else {
+      const double ood3 = 1.0 / (d2 * dist);
+      const double q = cf / ood3;
+      B = 0.0;
+      if (q > 16.0) K = ood3;
+      else if (q < (double)0.001f) K = cf;
+      else K = ood3 * (1.0 - exp(-q));
+    }
+  }
+}
+
+template <int CORE, bool THICK, bool GRAD>
+__global__ __launch_bounds__(BLOCK) void k_p2p64(P2PArgs64 a) {
+  extern __shared__ double4 sh64[];
+  const int tid = threadIdx.x;
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + tid;
+  const int64_t ic = i < a.nt ? i : a.nt - 1;
+  const double tx = a.t.x[ic], ty = a.t.y[ic], tz = a.t.z[ic];
+  const double tr = THICK ? a.t.r[ic] : 0.0;
+  double acc[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) acc[c] = 0.0;
+  for (int64_t s0 = 0; s0 < a.ns; s0 += a.tile) {
+    const int cnt = (int)((a.ns - s0) < a.tile ? (a.ns - s0) : a.tile);
+    __syncthreads();
+    for (int j = tid; j < cnt; j += BLOCK) {
+      const int64_t g = s0 + j;
+      sh64[2 * j] = make_double4(a.s.x[g], a.s.y[g], a.s.z[g], a.s.r[g]);
+      sh64[2 * j + 1] = make_double4(a.s.ax[g], a.s.ay[g], a.s.az[g], 0.0);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      if (a.self_excl && s0 + j == i) continue;
+      const double4 P = sh64[2 * j];
+      const double4 A = sh64[2 * j + 1];
+      const double dx = tx - P.x, dy = ty - P.y, dz = tz - P.z;
+      const double d2 = dx * dx + dy * dy + dz * dz;
+      double K, B;
+      kb64<CORE, THICK, GRAD>(d2, P.w, tr, K, B);
+      double cx = dz * A.y - dy * A.z;
+      double cy = dx * A.z - dz * A.x;
+      double cz = dy * A.x - dx * A.y;
+      acc[0] += K * cx; acc[1] += K * cy; acc[2] += K * cz;
+      if (GRAD) {
+        cx *= B; cy *= B; cz *= B;
+        acc[3] += dx * cx;
+        acc[4] += dx * cy + A.z * K;
+        acc[5] += dx * cz - A.y * K;
+        acc[6] += dy * cx - A.z * K;
+        acc[7] += dy * cy;
+        acc[8] += dy * cz + A.x * K;
+        acc[9] += dz * cx + A.y * K;
+        acc[10] += dz * cy - A.x * K;
+        acc[11] += dz * cz;
+      }
+    }
+  }
+  if (i < a.nt) {
+    a.t.u[i] += acc[0]; a.t.v[i] += acc[1]; a.t.w[i] += acc[2];
+    if (GRAD) {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) a.t.g[c][i] += acc[3 + c];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers (one translation unit per core instantiates them)
+// ---------------------------------------------------------------------------
+template <int CORE, bool THICK, bool GRAD, int PREC>
+hipError_t launch_p2p_prec(const P2PArgs& a, int tpt, hipStream_t st) {
+  const size_t lds = (size_t)a.tile * 2 * sizeof(float4);
+  const int64_t per = (int64_t)BLOCK * tpt;
+  const dim3 grid((unsigned)((a.nt + per - 1) / per));
+  if (tpt == 1) hipLaunchKernelGGL((k_p2p<CORE, THICK, GRAD, PREC, 1>), grid, dim3(BLOCK), lds, st, a);
+  else if (tpt == 2) hipLaunchKernelGGL((k_p2p<CORE, THICK, GRAD, PREC, 2>), grid, dim3(BLOCK), lds, st, a);
+  else if (tpt == 4) hipLaunchKernelGGL((k_p2p<CORE, THICK, GRAD, PREC, 4>), grid, dim3(BLOCK), lds, st, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int CORE>
+hipError_t launch_p2p(const P2PArgs& a, bool thick, bool grad, int prec, int tpt, hipStream_t st) {
+  if (a.nt <= 0 || a.ns <= 0) return hipSuccess;
+#define O3D_L(TH, GR)                                                                        \
+  if (thick == TH && grad == GR) {                                                           \
+    if (prec == PREC_FP32) return launch_p2p_prec<CORE, TH, GR, PREC_FP32>(a, tpt, st);     \
+    if (prec == PREC_COMP) return launch_p2p_prec<CORE, TH, GR, PREC_COMP>(a, tpt, st);     \
+    return hipErrorInvalidValue;                                                             \
+  }
+  O3D_L(true, true) O3D_L(true, false) O3D_L(false, true) O3D_L(false, false)
+#undef O3D_L
+  return hipErrorInvalidValue;
+}
+
+template <int CORE>
+hipError_t launch_p2p64(const P2PArgs64& a, bool thick, bool grad, hipStream_t st) {
+  if (a.nt <= 0 || a.ns <= 0) return hipSuccess;
+  const size_t lds = (size_t)a.tile * 2 * sizeof(double4);
+  const dim3 grid((unsigned)((a.nt + BLOCK - 1) / BLOCK));
+  if (thick && grad) hipLaunchKernelGGL((k_p2p64<CORE, true, true>), grid, dim3(BLOCK), lds, st, a);
+  else if (thick) hipLaunchKernelGGL((k_p2p64<CORE, true, false>), grid, dim3(BLOCK), lds, st, a);
+  else if (grad) hipLaunchKernelGGL((k_p2p64<CORE, false, true>), grid, dim3(BLOCK), lds, st, a);
+  else hipLaunchKernelGGL((k_p2p64<CORE, false, false>), grid, dim3(BLOCK), lds, st, a);
+  return hipGetLastError();
+}
+
+}  // namespace o3d
+
+#define O3D_INSTANTIATE_P2P(CORE, NAME)                                                        \
+  namespace o3d {                                                                              \
+  hipError_t launch_p2p_##NAME(const P2PArgs& a, bool thick, bool grad, int prec, int tpt,     \
+                               hipStream_t st) {                                               \
+    return launch_p2p<CORE>(a, thick, grad, prec, tpt, st);                                    \
+  }                                                                                            \
+  hipError_t launch_p2p64_##NAME(const P2PArgs64& a, bool thick, bool grad, hipStream_t st) {  \
+    return launch_p2p64<CORE>(a, thick, grad, st);                                             \
+  }                                                                                            \
+  }

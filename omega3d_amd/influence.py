@@ -1,0 +1,202 @@
+"""Python mirror of Omega3D's influence interface for the direct Biot-Savart path.
+
+Reference shapes (Influence.h, ExecEnv.h, ResultsType.h, ElementBase.h, Points.h):
+  points_affect_points(src, targ, restype, env)   Influence.h:67-551
+  panels_affect_points(src, targ, restype, env)   Influence.h:557-1099
+  ExecEnv / accel_t                               ExecEnv.h:26-123
+  ResultsType / results_t                         ResultsType.h:27-95
+  zero_vels / finalize_vels (1/4pi, freestream)   ElementBase.h:170-208, Points.h:252-277
+
+Every call goes to the gfx950 kernels through libo3dbiot.so; there is no CPU
+path.  Results accumulate into the target's vel/grad arrays, unscaled, exactly
+as in the reference; finalize_vels applies 1/(4 pi) and the freestream.
+
+Array conventions: pos (3, n), str (3, n), rad (n,), vel (3, n), grad (9, n)
+with grad[3*j + i] = du_i/dx_j; panels as in scenes.Panels.
+"""
+import ctypes as C
+import enum
+import math
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib, make_opts
+
+
+class accel_t(enum.IntEnum):
+    cpu_x86 = 1
+    cpu_vc = 2
+    gpu_opengl = 3
+    gpu_cuda = 4
+    gpu_hip = 5      # this library
+
+
+class results_t(enum.IntEnum):
+    velonly = 1
+    velandgrad = 2
+    psionly = 3
+    velandvort = 4
+    velandshear = 5
+
+
+class ResultsType:
+    def __init__(self, rtype=results_t.velonly):
+        self.rtype = results_t(rtype)
+
+    def compute_grad(self):
+        return self.rtype == results_t.velandgrad
+
+    def get_type(self):
+        return self.rtype
+
+
+class ExecEnv:
+    """Execution environment; the default accelerator here is gpu_hip.  `core`,
+    `precision`, `self_term`, `tile`, `tpt` carry what the reference fixes at
+    compile time or does not have."""
+
+    def __init__(self, internal=True, useomp=True, summation=1, accel=accel_t.gpu_hip,
+                 core="wl", precision="comp", self_term=0, tile=0, tpt=0, maxlev=-1):
+        self.internal, self.useomp, self.summation, self.accel = internal, useomp, summation, accel_t(accel)
+        self.core, self.precision, self.self_term = core, precision, self_term
+        self.tile, self.tpt, self.maxlev = tile, tpt, maxlev
+
+    def to_string(self):
+        return " HIP-accelerated direct sums" if self.accel == accel_t.gpu_hip else " unsupported accelerator"
+
+
+class Points:
+    """SoA particles or field points.  rad None = inert field points (point targets);
+    grad is allocated when with_grad (the reference gives inert+lagrangian points no
+    gradient storage, Points.h:131-139)."""
+
+    def __init__(self, pos, str_=None, rad=None, with_grad=True, dtype=np.float32):
+        self.pos = np.ascontiguousarray(pos, dtype=dtype)
+        n = self.pos.shape[1]
+        self.str = None if str_ is None else np.ascontiguousarray(str_, dtype=dtype)
+        self.rad = None if rad is None else np.ascontiguousarray(rad, dtype=dtype)
+        self.vel = np.zeros((3, n), dtype)
+        self.grad = np.zeros((9, n), dtype) if with_grad else None
+
+    @property
+    def n(self):
+        return self.pos.shape[1]
+
+    def is_inert(self):
+        return self.rad is None
+
+    def zero_vels(self):
+        self.vel[:] = 0
+        if self.grad is not None:
+            self.grad[:] = 0
+
+    def finalize_vels(self, fs=(0.0, 0.0, 0.0)):
+        # ElementBase.h:187-208 (velocity, factor in double) and Points.h:265-277
+        # (gradient, factor as the storage type)
+        factor = 0.25 / math.pi
+        for d in range(3):
+            self.vel[d] = (fs[d] + self.vel[d].astype(np.float64) * factor).astype(self.vel.dtype)
+        if self.grad is not None:
+            self.grad *= self.grad.dtype.type(factor)
+
+
+def _pts_struct(pos, str_, rad, vel, grad, cls=_lib.Points):
+    p = cls()
+    p.n = pos.shape[1]
+    for d in range(3):
+        p.x[d] = pos[d].ctypes.data
+        if str_ is not None:
+            p.s[d] = str_[d].ctypes.data
+        if vel is not None:
+            p.u[d] = vel[d].ctypes.data
+    p.r = rad.ctypes.data if rad is not None else None
+    if grad is not None:
+        for k in range(9):
+            p.ug[k] = grad[k].ctypes.data
+    return p
+
+
+def _c32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _c64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, precision="comp", self_term=0,
+        tile=0, tpt=0, vel=None, grad=None, stats=False):
+    """Host-array point->point sum on the GPU.  Accumulates into vel/grad (new zero
+    arrays when not given) and returns them (and the o3d_stats when stats=True)."""
+    f = _c64 if precision == "fp64" else _c32
+    spos, sstr, srad, tpos, trad = f(spos), f(sstr), f(srad), f(tpos), f(trad)
+    dt = spos.dtype
+    nt = tpos.shape[1]
+    vel = np.zeros((3, nt), dt) if vel is None else vel
+    grad = (np.zeros((9, nt), dt) if grad is None else grad) if grads else None
+    o = make_opts(core, grads, precision, self_term, tile, tpt)
+    st = _lib.Stats()
+    if dt == np.float64:
+        s = _pts_struct(spos, sstr, srad, None, None, _lib.Points64)
+        t = _pts_struct(tpos, None, trad, vel, grad, _lib.Points64)
+        check(lib.o3d_points_affect_points64(C.byref(s), C.byref(t), C.byref(o), C.byref(st)))
+    else:
+        s = _pts_struct(spos, sstr, srad, None, None)
+        t = _pts_struct(tpos, None, trad, vel, grad)
+        check(lib.o3d_points_affect_points(C.byref(s), C.byref(t), C.byref(o), C.byref(st)))
+    return (vel, grad, st) if stats else (vel, grad)
+
+
+def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, precision="comp", maxlev=-1,
+        vel=None, grad=None, stats=False):
+    """Host-array panel->point sum (recursive centroid rule) on the GPU."""
+    nodes, ts, area, ss, tpos = _c32(nodes), _c32(ts), _c32(area), _c32(ss), _c32(tpos)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    nt = tpos.shape[1]
+    vel = np.zeros((3, nt), np.float32) if vel is None else vel
+    grad = (np.zeros((9, nt), np.float32) if grad is None else grad) if grads else None
+    p = _lib.Panels()
+    p.np = area.shape[0]
+    p.nn = nodes.shape[1]
+    for d in range(3):
+        p.x[d] = nodes[d].ctypes.data
+        p.ts[d] = ts[d].ctypes.data
+    p.idx = idx.ctypes.data
+    p.area = area.ctypes.data
+    p.ss = ss.ctypes.data if ss is not None else None
+    t = _pts_struct(tpos, None, None, vel, grad)
+    o = make_opts(core, grads, precision, 0, 0, 0, maxlev)
+    st = _lib.Stats()
+    check(lib.o3d_panels_affect_points(C.byref(p), C.byref(t), C.byref(o), C.byref(st)))
+    return (vel, grad, st) if stats else (vel, grad)
+
+
+def points_affect_points(src: Points, targ: Points, restype: ResultsType, env: ExecEnv):
+    """Influence.h:67-551 on the GPU: accumulate src's influence into targ."""
+    grads = restype.compute_grad()
+    if grads and targ.grad is None:
+        raise _lib.O3DError(1, "targets have no velocity-gradient storage (Influence.h:369)")
+    if restype.get_type() not in (results_t.velonly, results_t.velandgrad):
+        raise _lib.O3DError(1, f"results type {restype.get_type().name} is not served by this path")
+    if env.accel != accel_t.gpu_hip:
+        raise _lib.O3DError(1, f"accelerator {env.accel.name} is not provided by this library")
+    p2p(env.core, src.pos, src.str, src.rad, targ.pos, targ.rad, grads, env.precision,
+        env.self_term, env.tile, env.tpt, vel=targ.vel, grad=targ.grad if grads else None)
+
+
+def panels_affect_points(src, targ: Points, restype: ResultsType, env: ExecEnv):
+    """Influence.h:557-1099 on the GPU (point kernels for every target)."""
+    grads = restype.compute_grad() and targ.grad is not None
+    pan(env.core, src.nodes, src.idx, src.ts, src.area, src.ss, targ.pos, grads, env.precision,
+        env.maxlev, vel=targ.vel, grad=targ.grad if grads else None)
+
+
+def external_vel_solver_f(spos, sstr, srad, tpos, vel, grad):
+    """The reference's external-solver ABI (Influence.h:33-48) through ctypes."""
+    ns = C.c_int(spos.shape[1])
+    nt = C.c_int(tpos.shape[1])
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    return lib.external_vel_solver_f_(C.byref(ns), P(spos[0]), P(spos[1]), P(spos[2]), P(sstr[0]), P(sstr[1]),
+                                      P(sstr[2]), P(srad), C.byref(nt), P(tpos[0]), P(tpos[1]), P(tpos[2]),
+                                      P(vel[0]), P(vel[1]), P(vel[2]), *[P(grad[k]) for k in range(9)])

@@ -1,0 +1,212 @@
+"""THE ORACLE -- test infrastructure only.
+
+ctypes front-end over ``liboracle.so`` (a CPU restatement of Omega3D's direct
+Biot-Savart path, see ``o3d_oracle.cpp``) and, when it has been built, over the
+reference's own scalar kernels in ``_ref/libo3dref_<core>.so``.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s cpu_baseline
+leg may import this package, and only as the checker / the timed CPU baseline.
+The product (``omega3d_amd``) never imports it.
+
+Array conventions (shared with the product's Python mirror):
+  source points : pos (3, N) float32, str (3, N) float32, rad (N,) float32
+  targets       : pos (3, M) float32, rad (M,) float32 or None for point targets
+  results       : vel (3, M), grad (9, M) with grad[3*j + i] = du_i/dx_j,
+                  accumulated and unscaled (no 1/4pi), as Influence.h:296-307.
+  panels        : nodes (3, Nn) float32, idx (3*P,) uint32, ts (3, P) absolute
+                  vortex strength, area (P,), ss (P,) sheet source strength or None.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CORES = {"rm": 1, "wl": 2, "exp": 3, "v2": 4}
+RECURSIVE_LEVELS = 3  # Influence.h:23
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_vp = C.c_void_p
+_i64 = C.c_int64
+
+
+def _load(path):
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} not built (run make -C oracle)")
+    return C.CDLL(path)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load(os.path.join(HERE, "liboracle.so"))
+        for name in ("orc_p2p_f", "orc_p2p_d", "orc_pan_f", "orc_pan_d", "orc_p2p_vcstyle"):
+            getattr(_lib, name).restype = C.c_int
+        _lib.orc_flops_pair.restype = C.c_int
+        _lib.orc_flops_pair.argtypes = [C.c_int] * 4
+        _lib.orc_max_threads.restype = C.c_int
+        _lib.orc_have_avx2.restype = C.c_int
+    return _lib
+
+
+def ref_path(core):
+    return os.path.join(HERE, "_ref", f"libo3dref_{core}.so")
+
+
+_refs = {}
+
+
+def ref_lib(core):
+    """The reference's own kernels for ``core`` (None when oracle/_ref is not built)."""
+    if core not in _refs:
+        p = ref_path(core)
+        _refs[core] = C.CDLL(p) if os.path.exists(p) else None
+        if _refs[core] is not None:
+            for name in ("ref_p2p_f", "ref_p2p_d", "ref_pan_f", "ref_pan_d", "ref_flops"):
+                getattr(_refs[core], name).restype = C.c_int
+    return _refs[core]
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_vp) if a is not None else None
+
+
+def _ptrs9(g):
+    if g is None:
+        return None
+    arr = (_vp * 9)(*[g[k].ctypes.data for k in range(9)])
+    return arr
+
+
+def _core_id(core):
+    return CORES[core] if isinstance(core, str) else int(core)
+
+
+def _as(a, dt):
+    return None if a is None else np.ascontiguousarray(a, dtype=dt)
+
+
+def _prep(dt, spos, sstr, srad, tpos, trad, grads):
+    spos, sstr, srad = _as(spos, dt), _as(sstr, dt), _as(srad, dt)
+    tpos, trad = _as(tpos, dt), _as(trad, dt)
+    nt = tpos.shape[1]
+    vel = np.zeros((3, nt), dt)
+    grad = np.zeros((9, nt), dt) if grads else None
+    return spos, sstr, srad, tpos, trad, vel, grad
+
+
+def flops_pair(core, grads=True, thick=True, vs=False):
+    return lib().orc_flops_pair(_core_id(core), int(grads), int(thick), int(vs))
+
+
+def p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, mode="f32d", nthreads=0):
+    """points_affect_points restated.  mode: 'f32f' (S=A=float), 'f32d' (S=float,
+    A=double), 'f64' (FP64 oracle: every operation in double)."""
+    cid = _core_id(core)
+    if mode == "f64":
+        spos, sstr, srad, tpos, trad, vel, grad = _prep(np.float64, spos, sstr, srad, tpos, trad, grads)
+        rc = lib().orc_p2p_d(cid, int(grads), _i64(spos.shape[1]), _ptr(spos[0]), _ptr(spos[1]), _ptr(spos[2]),
+                             _ptr(srad), _ptr(sstr[0]), _ptr(sstr[1]), _ptr(sstr[2]),
+                             _i64(tpos.shape[1]), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]), _ptr(trad),
+                             _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad), int(nthreads))
+    else:
+        m = {"f32f": 0, "f32d": 1}[mode]
+        spos, sstr, srad, tpos, trad, vel, grad = _prep(np.float32, spos, sstr, srad, tpos, trad, grads)
+        rc = lib().orc_p2p_f(cid, int(grads), m, _i64(spos.shape[1]), _ptr(spos[0]), _ptr(spos[1]), _ptr(spos[2]),
+                             _ptr(srad), _ptr(sstr[0]), _ptr(sstr[1]), _ptr(sstr[2]),
+                             _i64(tpos.shape[1]), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]), _ptr(trad),
+                             _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle p2p failed rc={rc}")
+    return vel, grad
+
+
+def p2p_vcstyle(core, spos, sstr, srad, tpos, trad, t0=0, t1=None, nthreads=0):
+    """The Vc-style CPU baseline (thick targets, velocity + gradient) over targets [t0, t1)."""
+    spos, sstr, srad, tpos, trad, vel, grad = _prep(np.float32, spos, sstr, srad, tpos, trad, True)
+    nt = tpos.shape[1]
+    t1 = nt if t1 is None else t1
+    rc = lib().orc_p2p_vcstyle(_core_id(core), _i64(spos.shape[1]), _ptr(spos[0]), _ptr(spos[1]), _ptr(spos[2]),
+                               _ptr(srad), _ptr(sstr[0]), _ptr(sstr[1]), _ptr(sstr[2]),
+                               _i64(nt), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]), _ptr(trad),
+                               _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad),
+                               _i64(t0), _i64(t1), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle vcstyle failed rc={rc}")
+    return vel, grad
+
+
+def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, mode="f32d", maxlev=RECURSIVE_LEVELS, nthreads=0):
+    """panels_affect_points restated (recursive centroid rule, Kernels.h:1120-1211)."""
+    cid = _core_id(core)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    dt = np.float64 if mode == "f64" else np.float32
+    nodes, ts, area, ss, tpos = _as(nodes, dt), _as(ts, dt), _as(area, dt), _as(ss, dt), _as(tpos, dt)
+    nt = tpos.shape[1]
+    vel = np.zeros((3, nt), dt)
+    grad = np.zeros((9, nt), dt) if grads else None
+    args = [_i64(area.shape[0]), _ptr(nodes[0]), _ptr(nodes[1]), _ptr(nodes[2]), _ptr(idx),
+            _ptr(ts[0]), _ptr(ts[1]), _ptr(ts[2]), _ptr(area), _ptr(ss),
+            _i64(nt), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]),
+            _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad), int(nthreads)]
+    if mode == "f64":
+        rc = lib().orc_pan_d(cid, int(grads), int(maxlev), *args)
+    else:
+        rc = lib().orc_pan_f(cid, int(grads), {"f32f": 0, "f32d": 1}[mode], int(maxlev), *args)
+    if rc != 0:
+        raise RuntimeError(f"oracle pan failed rc={rc}")
+    return vel, grad
+
+
+# ---- the reference's own kernels (oracle/_ref), same conventions -------------
+
+def ref_p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, mode="f32d"):
+    L = ref_lib(core)
+    if L is None:
+        raise FileNotFoundError(ref_path(core))
+    if mode == "f64":
+        spos, sstr, srad, tpos, trad, vel, grad = _prep(np.float64, spos, sstr, srad, tpos, trad, grads)
+        rc = L.ref_p2p_d(int(grads), _i64(spos.shape[1]), _ptr(spos[0]), _ptr(spos[1]), _ptr(spos[2]),
+                         _ptr(srad), _ptr(sstr[0]), _ptr(sstr[1]), _ptr(sstr[2]),
+                         _i64(tpos.shape[1]), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]), _ptr(trad),
+                         _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad))
+    else:
+        spos, sstr, srad, tpos, trad, vel, grad = _prep(np.float32, spos, sstr, srad, tpos, trad, grads)
+        rc = L.ref_p2p_f(int(grads), {"f32f": 0, "f32d": 1}[mode], _i64(spos.shape[1]),
+                         _ptr(spos[0]), _ptr(spos[1]), _ptr(spos[2]),
+                         _ptr(srad), _ptr(sstr[0]), _ptr(sstr[1]), _ptr(sstr[2]),
+                         _i64(tpos.shape[1]), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]), _ptr(trad),
+                         _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad))
+    if rc != 0:
+        raise RuntimeError(f"ref p2p failed rc={rc}")
+    return vel, grad
+
+
+def ref_pan(core, nodes, idx, ts, area, ss, tpos, grads=True, mode="f32d", maxlev=RECURSIVE_LEVELS):
+    L = ref_lib(core)
+    if L is None:
+        raise FileNotFoundError(ref_path(core))
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    dt = np.float64 if mode == "f64" else np.float32
+    nodes, ts, area, ss, tpos = _as(nodes, dt), _as(ts, dt), _as(area, dt), _as(ss, dt), _as(tpos, dt)
+    nt = tpos.shape[1]
+    vel = np.zeros((3, nt), dt)
+    grad = np.zeros((9, nt), dt) if grads else None
+    args = [_i64(area.shape[0]), _ptr(nodes[0]), _ptr(nodes[1]), _ptr(nodes[2]), _ptr(idx),
+            _ptr(ts[0]), _ptr(ts[1]), _ptr(ts[2]), _ptr(area), _ptr(ss),
+            _i64(nt), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]),
+            _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad)]
+    if mode == "f64":
+        rc = L.ref_pan_d(int(grads), int(maxlev), *args)
+    else:
+        rc = L.ref_pan_f(int(grads), {"f32f": 0, "f32d": 1}[mode], int(maxlev), *args)
+    if rc != 0:
+        raise RuntimeError(f"ref pan failed rc={rc}")
+    return vel, grad
+
+
+def ref_flops(core, which):
+    return ref_lib(core).ref_flops(int(which))

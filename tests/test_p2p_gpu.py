@@ -1,0 +1,225 @@
+"""GPU parity of the point->point kernels against the oracle, through the C ABI.
+
+Bars (normwise, omega3d_amd.metrics):
+  * against the reference's scalar semantics (oracle 'f32d': float pairs, double
+    accumulation, the non-Vc build of Simulation.h:44-46):  max rel <= 1e-5
+  * compensated FP32 against the FP64 oracle:                  max rel <= 1e-6
+  * plain FP32 against the FP64 oracle:                        max rel <= 2e-5 at N <= 8192
+  * the FP64 kernel against the CPU FP64 oracle:               max rel <= 1e-12
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from omega3d_amd import scenes
+from omega3d_amd.metrics import rel_errors, self_term_gradient
+
+pytestmark = pytest.mark.gpu
+
+CORES = ["rm", "wl", "exp", "v2"]
+
+
+@pytest.fixture(scope="module")
+def infl():
+    from omega3d_amd import _lib, influence
+    assert _lib.lib.o3d_init(-1) == 0, _lib.last_error()
+    return influence
+
+
+def _case(seed, ns=1000, nt=300, spread=0.5, rmin=0.02, rmax=0.1):
+    rng = np.random.default_rng(seed)
+    sp = rng.uniform(-spread, spread, (3, ns)).astype(np.float32)
+    ss = (rng.normal(size=(3, ns)) / ns).astype(np.float32)
+    sr = rng.uniform(rmin, rmax, ns).astype(np.float32)
+    tp = np.concatenate([sp[:, : nt // 3], rng.uniform(-spread, spread, (3, nt - nt // 3)).astype(np.float32)], 1)
+    tr = rng.uniform(rmin, rmax, nt).astype(np.float32)
+    return sp, ss, sr, tp, tr
+
+
+@pytest.mark.parametrize("core", CORES)
+@pytest.mark.parametrize("thick", [True, False])
+@pytest.mark.parametrize("grads", [True, False])
+@pytest.mark.parametrize("prec", ["fp32", "comp"])
+def test_p2p_parity_random(infl, core, thick, grads, prec):
+    sp, ss, sr, tp, tr = _case(1 + CORES.index(core))
+    trr = tr if thick else None
+    v, g = infl.p2p(core, sp, ss, sr, tp, trr, grads, prec)
+    rv, rg = oracle.p2p(core, sp, ss, sr, tp, trr, grads, "f32d")
+    ov, og = oracle.p2p(core, sp, ss, sr, tp, trr, grads, "f64")
+    e_ref = rel_errors(v, rv)[0]
+    e_64 = rel_errors(v, ov)[0]
+    print(f"{core} thick={thick} grads={grads} {prec}: vel vs ref {e_ref:.2e} vs fp64 {e_64:.2e}")
+    assert e_ref <= 1e-5
+    assert e_64 <= (1e-6 if prec == "comp" else 2e-5)
+    if grads:
+        ge_ref = rel_errors(g, rg)[0]
+        ge_64 = rel_errors(g, og)[0]
+        print(f"   grad vs ref {ge_ref:.2e} vs fp64 {ge_64:.2e}")
+        assert ge_ref <= 1e-5
+        assert ge_64 <= (1e-6 if prec == "comp" else 2e-5)
+
+
+@pytest.mark.parametrize("core", CORES)
+def test_fp64_kernel_matches_cpu_fp64(infl, core):
+    sp, ss, sr, tp, tr = _case(11, ns=700, nt=257)
+    for thick in (True, False):
+        for grads in (True, False):
+            v, g = infl.p2p(core, sp, ss, sr, tp, tr if thick else None, grads, "fp64")
+            ov, og = oracle.p2p(core, sp, ss, sr, tp, tr if thick else None, grads, "f64")
+            assert rel_errors(v, ov)[0] <= 1e-12
+            if grads:
+                assert rel_errors(g, og)[0] <= 1e-12
+
+
+def test_ring_c0_rm_self_excluded(infl):
+    """The single-ring workload (N = 8,192, RM core, sigma 0.05, self pair excluded)
+    against the reference with its self term removed in closed form."""
+    p = scenes.ring()
+    v, g = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", self_term=1)
+    rv, rg = oracle.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "f32d")
+    ov, og = oracle.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "f64")
+    corr = self_term_gradient("rm", p.str, p.rad)
+    rg = rg - corr
+    og = og - corr
+    ev, eg = rel_errors(v, ov), rel_errors(g, og)
+    print("c0 comp vs fp64: vel", ev, "grad", eg)
+    assert ev[0] <= 1e-6 and eg[0] <= 1e-6
+    assert rel_errors(v, rv)[0] <= 1e-5 and rel_errors(g, rg)[0] <= 1e-5
+    # plain FP32 as well
+    v2, g2 = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "fp32", self_term=1)
+    print("c0 fp32 vs fp64: vel", rel_errors(v2, ov), "grad", rel_errors(g2, og))
+    assert rel_errors(v2, ov)[0] <= 2e-5 and rel_errors(g2, og)[0] <= 2e-5
+    # FP64 kernel excludes the pair exactly
+    v3, g3 = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "fp64", self_term=1)
+    assert rel_errors(g3, og)[0] <= 1e-12 and rel_errors(v3, ov)[0] <= 1e-12
+
+
+@pytest.mark.parametrize("tile", [64, 128, 256, 512, 1024])
+@pytest.mark.parametrize("tpt", [1, 2, 4])
+def test_tile_and_tpt_variants(infl, tile, tpt):
+    sp, ss, sr, tp, tr = _case(21, ns=1500, nt=1100)
+    v, g = infl.p2p("exp", sp, ss, sr, tp, tr, True, "comp", tile=tile, tpt=tpt)
+    ov, og = oracle.p2p("exp", sp, ss, sr, tp, tr, True, "f64")
+    assert rel_errors(v, ov)[0] <= 1e-6 and rel_errors(g, og)[0] <= 1e-6
+
+
+@pytest.mark.parametrize("ns,nt", [(1, 1), (1, 300), (300, 1), (65, 63), (257, 1025), (0, 10), (10, 0)])
+def test_edge_sizes(infl, ns, nt):
+    rng = np.random.default_rng(ns * 1000 + nt)
+    sp = rng.uniform(-1, 1, (3, ns)).astype(np.float32)
+    ss = rng.normal(size=(3, ns)).astype(np.float32)
+    sr = rng.uniform(0.05, 0.2, ns).astype(np.float32)
+    tp = rng.uniform(-1, 1, (3, nt)).astype(np.float32)
+    tr = rng.uniform(0.05, 0.2, nt).astype(np.float32)
+    v, g = infl.p2p("wl", sp, ss, sr, tp, tr, True, "comp")
+    if ns == 0 or nt == 0:
+        assert np.all(v == 0) and np.all(g == 0)
+        return
+    ov, og = oracle.p2p("wl", sp, ss, sr, tp, tr, True, "f64")
+    assert rel_errors(v, ov)[0] <= 1e-6 and rel_errors(g, og)[0] <= 1e-6
+
+
+def test_accumulate_semantics(infl):
+    """Results are added into what the arrays already hold (Influence.h:296-307)."""
+    sp, ss, sr, tp, tr = _case(31, ns=400, nt=200)
+    v0 = np.random.default_rng(1).normal(size=(3, 200)).astype(np.float32)
+    g0 = np.random.default_rng(2).normal(size=(9, 200)).astype(np.float32)
+    for prec in ("fp32", "comp", "fp64"):
+        dt = np.float64 if prec == "fp64" else np.float32
+        v, g = infl.p2p("rm", sp, ss, sr, tp, tr, True, prec, vel=v0.astype(dt), grad=g0.astype(dt))
+        ov, og = oracle.p2p("rm", sp, ss, sr, tp, tr, True, "f64")
+        assert rel_errors(v - v0, ov)[0] <= 1e-5
+        assert rel_errors(g - g0, og)[0] <= 1e-5
+    # two calls = twice the influence
+    v, g = infl.p2p("rm", sp, ss, sr, tp, tr, True, "comp")
+    v, g = infl.p2p("rm", sp, ss, sr, tp, tr, True, "comp", vel=v, grad=g)
+    ov, og = oracle.p2p("rm", sp, ss, sr, tp, tr, True, "f64")
+    assert rel_errors(v, 2 * ov)[0] <= 1e-6
+
+
+def test_reference_object_api(infl):
+    """points_affect_points / ExecEnv / ResultsType / finalize_vels, as a caller of
+    the reference would drive them (Convection.h:131-182)."""
+    p = scenes.ring(2000, sigma=0.04, seed=3)
+    vort = infl.Points(p.pos, p.str, p.rad)
+    env = infl.ExecEnv(core="exp", precision="comp")
+    vort.zero_vels()
+    infl.points_affect_points(vort, vort, infl.ResultsType(infl.results_t.velandgrad), env)
+    vort.finalize_vels((0.1, 0.0, 0.0))
+    ov, og = oracle.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, "f64")
+    f = 0.25 / np.pi
+    want_v = ov * f + np.array([[0.1], [0.0], [0.0]])
+    assert rel_errors(vort.vel, want_v)[0] <= 1e-6
+    assert rel_errors(vort.grad, og * f)[0] <= 1e-6
+    # inert field points with gradients, velocity-only request
+    fld = infl.Points(np.random.default_rng(4).uniform(-1, 1, (3, 100)).astype(np.float32))
+    infl.points_affect_points(vort, fld, infl.ResultsType(infl.results_t.velonly), env)
+    ov2, _ = oracle.p2p("exp", p.pos, p.str, p.rad, fld.pos, None, False, "f64")
+    assert rel_errors(fld.vel, ov2)[0] <= 1e-6
+    assert np.all(fld.grad == 0)
+
+
+def test_external_solver_abi(infl):
+    """external_vel_solver_f_/_d_ (Influence.h:33-48): point targets, accumulate,
+    flop count returned; core chosen with o3d_set_external_core (default WL)."""
+    from omega3d_amd import _lib
+    sp, ss, sr, tp, _ = _case(41, ns=500, nt=130)
+    vel = np.zeros((3, 130), np.float32)
+    grad = np.zeros((9, 130), np.float32)
+    fl = infl.external_vel_solver_f(sp, ss, sr, tp, vel, grad)
+    ov, og = oracle.p2p("wl", sp, ss, sr, tp, None, True, "f64")
+    assert rel_errors(vel, ov)[0] <= 1e-6 and rel_errors(grad, og)[0] <= 1e-6
+    assert fl == pytest.approx(130 * (12 + 68 * 500), rel=1e-6)
+    assert _lib.lib.o3d_set_external_core(1) == 0
+    d = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+    sp64, ss64, sr64, tp64 = d(sp), d(ss), d(sr), d(tp)
+    vel64 = np.zeros((3, 130)); grad64 = np.zeros((9, 130))
+    P = lambda a: a.ctypes.data_as(C.c_void_p)
+    ns, nt = C.c_int(500), C.c_int(130)
+    fl = _lib.lib.external_vel_solver_d_(C.byref(ns), P(sp64[0]), P(sp64[1]), P(sp64[2]), P(ss64[0]), P(ss64[1]),
+                                         P(ss64[2]), P(sr64), C.byref(nt), P(tp64[0]), P(tp64[1]), P(tp64[2]),
+                                         P(vel64[0]), P(vel64[1]), P(vel64[2]), *[P(grad64[k]) for k in range(9)])
+    ov, og = oracle.p2p("rm", sp, ss, sr, tp, None, True, "f64")
+    assert rel_errors(vel64, ov)[0] <= 1e-12 and rel_errors(grad64, og)[0] <= 1e-12
+    assert _lib.lib.o3d_set_external_core(2) == 0
+
+
+def test_device_pointer_entry(infl):
+    """o3d_points_affect_points_dev on device-resident SoA arrays."""
+    from omega3d_amd import _lib
+    sp, ss, sr, tp, tr = _case(51, ns=3000, nt=2000)
+    D = _lib.DeviceArray.from_host
+    dsp, dss, dsr, dtp, dtr = D(sp), D(ss), D(sr), D(tp), D(tr)
+    dv = D(np.zeros((3, 2000), np.float32))
+    dg = D(np.zeros((9, 2000), np.float32))
+    s = _lib.Points(); t = _lib.Points()
+    s.n, t.n = 3000, 2000
+    for d in range(3):
+        s.x[d], s.s[d], t.x[d], t.u[d] = dsp.row(d), dss.row(d), dtp.row(d), dv.row(d)
+    s.r, t.r = dsr.ptr, dtr.ptr
+    for k in range(9):
+        t.ug[k] = dg.row(k)
+    o = _lib.make_opts("v2", True, "comp")
+    _lib.check(_lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(t), C.byref(o), None))
+    _lib.synchronize()
+    ov, og = oracle.p2p("v2", sp, ss, sr, tp, tr, True, "f64")
+    assert rel_errors(dv.to_host(), ov)[0] <= 1e-6
+    assert rel_errors(dg.to_host(), og)[0] <= 1e-6
+
+
+def test_colliding_rings_subset_property(infl):
+    """At the full two-ring size (N = 1,048,576, EXP core, sigma 0.02) the GPU result on
+    a 512-target subset against all sources matches the FP64 oracle; the subset of a
+    full self-interaction equals the full evaluation restricted to those targets."""
+    c = scenes.colliding_rings()
+    sel = np.arange(0, c.n, c.n // 512)[:512]
+    tp, tr = c.pos[:, sel], c.rad[sel]
+    v, g = infl.p2p("exp", c.pos, c.str, c.rad, tp, tr, True, "comp")
+    ov, og = oracle.p2p("exp", c.pos, c.str, c.rad, tp, tr, True, "f64")
+    ev, eg = rel_errors(v, ov), rel_errors(g, og)
+    print("c1 subset comp vs fp64", ev, eg)
+    assert ev[0] <= 1e-6 and eg[0] <= 1e-6
+    vp, gp = infl.p2p("exp", c.pos, c.str, c.rad, tp, tr, True, "fp32")
+    print("c1 subset plain fp32 vs fp64", rel_errors(vp, ov), rel_errors(gp, og))
