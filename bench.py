@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--error-targets", type=int, default=16384)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--tpt", type=int, default=0)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="also run the known-traffic copy kernel on this many MiB (counter calibration)")
     return ap.parse_args()
 
 
@@ -97,6 +99,16 @@ def main():
         wall = time.perf_counter() - t
         kms = np.array([a.elapsed_ms(b) for a, b in evs])
         return wall, kms
+
+    if args.copy_probe_mb:
+        nb = args.copy_probe_mb << 20
+        a_, b_ = _lib.DeviceArray(nb // 4), _lib.DeviceArray(nb // 4)
+        a_.zero()
+        for _ in range(3):
+            _lib.check(L.o3d_copy_probe(b_.ptr, a_.ptr, nb, None))
+        _lib.synchronize()
+        log(f"bench: copy probe {args.copy_probe_mb} MiB x3 done")
+        del a_, b_
 
     log(f"bench: compensated FP32, {args.warmup} warmup + {args.steps} timed steps")
     wall, kms = run("comp", args.steps, args.warmup)

@@ -184,6 +184,48 @@ int o3d_set_external_core(int core);
  * vortex-only or vortex+source): Kernels.h flops_0v_0bg etc. -1 if invalid. */
 int o3d_flops_per_interaction(int core, int results, int thick_target, int vortex_source);
 
+/* ---- many independent collections in one persistent launch ------------------
+ * (no counterpart in the reference, which evaluates one collection per call).
+ * Instance i owns particles [off[i], off[i] + n[i]) of shared SoA DEVICE arrays
+ * and acts on itself (sources = targets = its particles), velocity + gradient,
+ * thick-cored targets.  A device-side work queue of (instance, 256*tpt-target
+ * chunk) items, longest instances first, is drained by a persistent grid through
+ * an atomic counter.  Each target's result is the same number whether its
+ * instance runs alone or in a batch (same tile, fixed per-target order). */
+typedef struct o3d_batch o3d_batch;
+
+/* cores / precs / self_terms: per-instance arrays (NULL = the opts value for all).
+ * opts->tile / opts->tpt: 0 = defaults (see o3d_batch_tune). */
+int o3d_batch_create(o3d_batch** b, int32_t ninst, const int64_t* off, const int64_t* n,
+                     const int32_t* cores, const int32_t* precs, const int32_t* self_terms,
+                     const o3d_opts* opts);
+int o3d_batch_destroy(o3d_batch* b);
+/* Zero the instances' outputs and evaluate every instance, on `stream`.
+ * all: the shared device arrays (x, s, r, u, ug with all->n = total particles). */
+int o3d_batch_run(o3d_batch* b, const o3d_points* all, void* stream);
+/* The same work as one ordinary launch per instance (the comparison baseline). */
+int o3d_batch_run_separate(o3d_batch* b, const o3d_points* all, void* stream);
+/* Capture o3d_batch_run into a HIP graph (zero-fill kernel, queue reset kernel,
+ * persistent kernel) and replay it; the graph reads the queue from device memory. */
+int o3d_batch_capture(o3d_batch* b, const o3d_points* all, void* stream);
+int o3d_batch_replay(o3d_batch* b, void* stream);
+/* Autotuner: time every (tile, tpt) candidate `reps` times on the real batch and
+ * keep the fastest in b; reports it and its mean time.  NULL lists = the default
+ * candidates tile {128, 256, 512} x tpt {1, 2, 4}. */
+int o3d_batch_tune(o3d_batch* b, const o3d_points* all, void* stream,
+                   const int32_t* tiles, int32_t ntiles, const int32_t* tpts, int32_t ntpts,
+                   int32_t reps, int32_t* best_tile, int32_t* best_tpt, float* best_ms);
+int o3d_batch_config(const o3d_batch* b, int32_t* tile, int32_t* tpt, int32_t* grid, int64_t* items);
+
+/* Autotuner for single-collection calls: time the candidates on the given device
+ * collection and record the winner for (core, results, precision, target kind,
+ * size bucket); later calls with tile = tpt = 0 use it.  The table can be saved
+ * and loaded as text. */
+int o3d_tune_p2p(const o3d_points* src, o3d_points* targ, const o3d_opts* opts, int32_t reps,
+                 int32_t* best_tile, int32_t* best_tpt, float* best_ms);
+int o3d_tune_save(const char* path);
+int o3d_tune_load(const char* path);
+
 /* ---- device plumbing (no counterpart in the reference: device-resident use) -- */
 int o3d_dev_malloc(void** ptr, size_t bytes);
 int o3d_dev_free(void* ptr);
@@ -198,6 +240,10 @@ int o3d_event_create(void** ev);
 int o3d_event_destroy(void* ev);
 int o3d_event_record(void* ev, void* stream);
 int o3d_event_elapsed_ms(float* ms, void* start, void* stop);
+/* A streaming float4 copy kernel of exactly `bytes` read and `bytes` written
+ * (bytes a multiple of 16): the known-traffic reference that HBM byte counters
+ * (FETCH_SIZE, WRITE_SIZE) are calibrated against. */
+int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream);
 /* device name, CU count, clock (kHz), HBM bytes */
 int o3d_device_info(char* name, int name_len, int* cus, int* clock_khz, size_t* hbm_bytes);
 

@@ -54,7 +54,25 @@ def build(verbose=False, jobs=None):
     if p.returncode != 0:
         raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
     os.replace(tmp, LIB)
+    build_cpp_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(ROOT, "tests", "cpp", "drive_influence.cpp")
+DRIVER = os.path.join(BUILD, "drive_influence")
+
+
+def build_cpp_driver():
+    """The C++ caller of include/omega3d/Influence.hpp used by the GPU tests: plain
+    g++ against the C ABI, linked to libo3dbiot.so by rpath."""
+    if not os.path.exists(DRIVER_SRC):
+        return None
+    cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-I" + os.path.join(ROOT, "include"),
+           DRIVER_SRC, "-o", DRIVER, "-L" + HERE, "-lo3dbiot", "-Wl,-rpath,$ORIGIN/.."]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("C++ driver build failed:\n" + p.stdout + p.stderr)
+    return DRIVER
 
 
 def clean():

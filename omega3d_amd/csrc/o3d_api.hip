@@ -7,6 +7,7 @@
 // Influence.h:75,310,475 (N_t * (12 + F * N_s) with F from Kernels.h).
 #include "o3d_api_internal.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -206,6 +207,10 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   a.self_excl = o->self_term == O3D_SELF_EXCLUDE;
   Tuning req{o->tile, o->tpt};
   if (tun) req = *tun;
+  else if (o->tile == 0 && o->tpt == 0) {
+    Tuning t;
+    if (tuned_lookup(o->core, o->results, o->precision, thick, a.nt, &t)) req = t;
+  }
   int tile, tpt;
   choose_config(req, a.nt, a.ns, grad, o->precision, &tile, &tpt);
   a.tile = tile;
@@ -326,6 +331,11 @@ Context& ctx() { return g_ctx; }
 hipStream_t lib_stream() { return g_ctx.stream; }
 std::mutex& pool_mutex() { return g_pool_mu; }
 int pool_buffer(int slot, size_t bytes, void** out) { return pool_get(slot, bytes, out); }
+
+__global__ void k_copy_probe(float4* __restrict__ d, const float4* __restrict__ s, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) d[i] = s[i];
+}
 
 }  // namespace o3d
 
@@ -512,6 +522,20 @@ int o3d_event_elapsed_ms(float* ms, void* a, void* b) {
   HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b));
   return O3D_OK;
 }
+int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream) {
+  clear_error();
+  if (!dst || !src || bytes % 16) return fail(O3D_EINVAL, "copy probe needs device buffers and bytes %% 16 == 0");
+  int rc = ensure_init();
+  if (rc) return rc;
+  const int64_t n = (int64_t)(bytes / 16);
+  const int blocks = (int)std::min<int64_t>((n + BLOCK - 1) / BLOCK, 8192);
+  if (n > 0)
+    hipLaunchKernelGGL(k_copy_probe, dim3(std::max(blocks, 1)), dim3(BLOCK), 0, (hipStream_t)stream, (float4*)dst,
+                       (const float4*)src, n);
+  HIPCHK(hipGetLastError());
+  return O3D_OK;
+}
+
 int o3d_device_info(char* name, int len, int* cus, int* clk, size_t* hbm) {
   clear_error();
   int rc = ensure_init();

@@ -21,6 +21,7 @@ struct Tuning {
   int tile;
   int tpt;
 };
+bool tuned_lookup(int core, int results, int prec, bool thick, int64_t nt, Tuning* out);
 void choose_config(const Tuning& req, int64_t nt, int64_t ns, bool grad, int prec, int* tile, int* tpt);
 hipError_t dispatch_p2p(int core, const P2PArgs& a, bool thick, bool grad, int prec, int tpt, hipStream_t st);
 double flops_of(int core, bool grad, bool thick, int64_t nt, int64_t ns);

@@ -1,0 +1,313 @@
+// pan_kernels.hpp -- panel -> point direct sums (velocity and gradient) for gfx950:
+// the recursive centroid rule of Omega3D's rkernel_2vs_0p / rkernel_2vs_0pg
+// (Kernels.h:1028-1211) over flat triangles, as driven by panels_affect_points
+// (Influence.h:557-1099).
+//
+// Per (target, panel) pair the reference walks a 4-ary subdivision tree: a node
+// (sub-triangle) is a leaf when its centroid is more than 4 sqrt(area) from the
+// target or at depth maxlev; a leaf is a singular (sigma = 0) vortex + source
+// point at the centroid carrying the node's share of the panel strength.
+//
+// GPU form: one lane per target; panels are staged per tile in LDS; each lane
+// walks the tree for its target in depth-first order with an explicit (level,
+// path) cursor -- no recursion, no stack -- recomputing a node's vertices from
+// the panel's corners along its path.  Every quantity that decides the tree
+// shape (midpoints, centroid, sqrt(area), distance) is computed in correctly
+// rounded IEEE arithmetic with FMA contraction off, in the reference's order,
+// so the GPU subdivides exactly where the reference does.  At the root the
+// decision is taken from a fast distance estimate whenever it is unambiguous
+// and from the exact expression otherwise.
+//
+// At sigma = 0 every core (RM, WL, EXP, V2) is the singular kernel K = 1/rho^3,
+// B = -3/rho^5 (CoreFunc.h formulas with sr = 0), so the panel path has one
+// leaf kernel whatever core the caller selected.
+#pragma once
+#include "o3d_common.hpp"
+
+namespace o3d {
+
+struct PanArgs {
+  const float* nx; const float* ny; const float* nz;    // nodes
+  const uint32_t* idx;                                  // 3 per panel
+  const float* tsx; const float* tsy; const float* tsz; // absolute vortex strength
+  const float* area;
+  const float* ss;                                      // sheet source strength or nullptr
+  int64_t np;
+  const float* tx; const float* ty; const float* tz;    // targets
+  void* u; void* v; void* w; void* g[9];                // float (FP32/COMP) or double (FP64) outputs
+  int64_t nt;
+  int32_t tile;
+  int32_t maxlev;
+};
+
+constexpr int PAN_F4 = 6;   // float4 per staged panel
+
+// exact reference arithmetic for the tree decisions (Kernels.h:1140-1160)
+template <class R>
+__device__ __forceinline__ void centroid_exact(R x0, R y0, R z0, R x1, R y1, R z1, R x2, R y2, R z2,
+                                               R& cx, R& cy, R& cz) {
+#pragma clang fp contract(off)
+  cx = (x0 + x1 + x2) / R(3);
+  cy = (y0 + y1 + y2) / R(3);
+  cz = (z0 + z1 + z2) / R(3);
+}
+template <class R>
+__device__ __forceinline__ R dist_exact(R tx, R ty, R tz, R cx, R cy, R cz) {
+#pragma clang fp contract(off)
+  const R ex = tx - cx, ey = ty - cy, ez = tz - cz;
+  return sqrt(ex * ex + ey * ey + ez * ez);
+}
+template <class R>
+__device__ __forceinline__ R mid(R a, R b) {
+#pragma clang fp contract(off)
+  return R(0.5) * (a + b);
+}
+
+// child c of triangle (v0, v1, v2) over the six nodes [v0, m01, v1, m02, m12, v2]:
+// {0,1,3}, {1,2,4}, {1,4,3}, {3,4,5}   (Kernels.h:1185-1192)
+template <class R>
+__device__ __forceinline__ void child_of(R* a, int c) {
+  R n1[3], n3[3], n4[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    n1[d] = mid(a[d], a[3 + d]);
+    n3[d] = mid(a[d], a[6 + d]);
+    n4[d] = mid(a[3 + d], a[6 + d]);
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const R v0 = a[d], v1 = a[3 + d], v2 = a[6 + d];
+    const R p = c == 0 ? v0 : (c == 3 ? n3[d] : n1[d]);
+    const R q = c == 0 ? n1[d] : (c == 1 ? v1 : n4[d]);
+    const R r = c == 0 ? n3[d] : (c == 1 ? n4[d] : (c == 2 ? n3[d] : v2));
+    a[d] = p; a[3 + d] = q; a[6 + d] = r;
+  }
+}
+
+// leaf: singular vortex (g) + source (gs) point at c acting on target t.
+// acc: [0..2] u, [3..11] sum B d_j (g x d + gs d)_i, [12..14] sum K g, [15] sum K gs
+template <class R, bool GRAD>
+__device__ __forceinline__ void leaf_acc(R tx, R ty, R tz, R cx, R cy, R cz, R gx, R gy, R gz, R gs,
+                                         R* acc) {
+  const R dx = tx - cx, dy = ty - cy, dz = tz - cz;
+  R K, B;
+  if constexpr (sizeof(R) == 4) {
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    const float ir = frsq(d2);
+    const float ir2 = ir * ir;
+    K = ir2 * ir;
+    B = K * (ir2 * -3.0f);
+  } else {
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    K = 1.0 / (d2 * sqrt(d2));
+    B = -3.0 * K / d2;
+  }
+  const R cvx = dz * gy - dy * gz + dx * gs;
+  const R cvy = dx * gz - dz * gx + dy * gs;
+  const R cvz = dy * gx - dx * gy + dz * gs;
+  acc[0] += K * cvx; acc[1] += K * cvy; acc[2] += K * cvz;
+  if (GRAD) {
+    const R bx = B * cvx, by = B * cvy, bz = B * cvz;
+    acc[3] += dx * bx; acc[4] += dx * by; acc[5] += dx * bz;
+    acc[6] += dy * bx; acc[7] += dy * by; acc[8] += dy * bz;
+    acc[9] += dz * bx; acc[10] += dz * by; acc[11] += dz * bz;
+    acc[12] += K * gx; acc[13] += K * gy; acc[14] += K * gz;
+    acc[15] += K * gs;
+  }
+}
+
+// gradient from the 16 accumulators: dyad + skew(W) + diag(S)
+template <class R>
+__device__ __forceinline__ void grad_of(const R* a, R* g) {
+  g[0] = a[3] + a[15];
+  g[1] = a[4] + a[14];
+  g[2] = a[5] - a[13];
+  g[3] = a[6] - a[14];
+  g[4] = a[7] + a[15];
+  g[5] = a[8] + a[12];
+  g[6] = a[9] + a[13];
+  g[7] = a[10] - a[12];
+  g[8] = a[11] + a[15];
+}
+
+// stage panels [p0, p0 + cnt) into LDS:
+//   f0 = {x0,y0,z0,x1}  f1 = {y1,z1,x2,y2}
+//   f2 = {z2, level-0 absolute vortex strength (ts/area)*area, float}
+//   f3 = {level-0 absolute source strength ss*area, area, sqrt(area), 0}
+//   f4 = {cx, cy, cz, 0} (root centroid)   f5 = {tsx, tsy, tsz, ss} (raw inputs)
+// The reference passes the sheet strength ts/area and multiplies it back by the
+// area at level 0 (Influence.h:737, Kernels.h:1136-1146); that round trip is kept.
+__device__ __forceinline__ void stage_panels(const PanArgs& a, int64_t p0, int cnt, float4* sh) {
+  for (int j = threadIdx.x; j < cnt; j += BLOCK) {
+    const int64_t p = p0 + j;
+    const uint32_t i0 = a.idx[3 * p], i1 = a.idx[3 * p + 1], i2 = a.idx[3 * p + 2];
+    const float x0 = a.nx[i0], y0 = a.ny[i0], z0 = a.nz[i0];
+    const float x1 = a.nx[i1], y1 = a.ny[i1], z1 = a.nz[i1];
+    const float x2 = a.nx[i2], y2 = a.ny[i2], z2 = a.nz[i2];
+    const float ar = a.area[p];
+    const float tx = a.tsx[p], ty = a.tsy[p], tz = a.tsz[p];
+    const float sg = a.ss ? a.ss[p] : 0.0f;
+    float cx, cy, cz, gx, gy, gz, gs;
+    centroid_exact(x0, y0, z0, x1, y1, z1, x2, y2, z2, cx, cy, cz);
+    {
+#pragma clang fp contract(off)
+      gx = (tx / ar) * ar; gy = (ty / ar) * ar; gz = (tz / ar) * ar; gs = sg * ar;
+    }
+    float4* s = sh + PAN_F4 * j;
+    s[0] = make_float4(x0, y0, z0, x1);
+    s[1] = make_float4(y1, z1, x2, y2);
+    s[2] = make_float4(z2, gx, gy, gz);
+    s[3] = make_float4(gs, ar, sqrtf(ar), 0.0f);
+    s[4] = make_float4(cx, cy, cz, 0.0f);
+    s[5] = make_float4(tx, ty, tz, sg);
+  }
+}
+
+// one (target, panel) tree walk, accumulating into acc
+template <class R, bool GRAD>
+__device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, int maxlev, bool active, R* acc) {
+  const float4 f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
+  const R ar = f3.y;
+  // level-0 absolute strengths (Kernels.h:1136-1146)
+  R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+  if constexpr (sizeof(R) == 4) {
+    gx = f2.y; gy = f2.z; gz = f2.w; gs = f3.x;
+    cx0 = f4.x; cy0 = f4.y; cz0 = f4.z;
+    lim = f3.z * 4.0f;
+  } else {
+    const float4 f5 = s[5];
+    {
+#pragma clang fp contract(off)
+      gx = (R(f5.x) / ar) * ar; gy = (R(f5.y) / ar) * ar; gz = (R(f5.z) / ar) * ar; gs = R(f5.w) * ar;
+    }
+    centroid_exact<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, cx0, cy0, cz0);
+    lim = sqrt(ar) * R(4);
+  }
+  bool leaf0;
+  if constexpr (sizeof(R) == 4) {
+    const float ex = tx - cx0, ey = ty - cy0, ez = tz - cz0;
+    const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+    const float dfast = d2 * frsq(d2);
+    const bool sure_far = dfast > lim * 1.0000010f;
+    const bool sure_near = dfast < lim * 0.9999990f;
+    leaf0 = sure_far;
+    const bool unsure = !sure_far && !sure_near;
+    if (wave_any(unsure && active)) {
+      if (unsure) leaf0 = dist_exact<float>(tx, ty, tz, cx0, cy0, cz0) > lim;
+    }
+  } else {
+    leaf0 = dist_exact<double>(tx, ty, tz, cx0, cy0, cz0) > lim;
+  }
+  leaf0 = leaf0 || maxlev == 0;
+  if (!wave_any(!leaf0 && active)) {
+    if (active) leaf_acc<R, GRAD>(tx, ty, tz, cx0, cy0, cz0, gx, gy, gz, gs, acc);
+    return;
+  }
+  // depth-first walk; lev = 0 is the root (already decided above)
+  int lev = 0, path = 0;
+  bool done = !active;
+  bool first = true;
+  while (wave_any(!done)) {
+    if (!done) {
+      bool leaf;
+      R cx, cy, cz;
+      R scale = R(1);
+      if (first) {
+        leaf = leaf0;
+        cx = cx0; cy = cy0; cz = cz0;
+      } else {
+        R t[9] = {R(f0.x), R(f0.y), R(f0.z), R(f0.w), R(f1.x), R(f1.y), R(f1.z), R(f1.w), R(f2.x)};
+        R sa = ar;
+        for (int l = 1; l <= lev; ++l) {
+          child_of<R>(t, (path >> (2 * (l - 1))) & 3);
+          sa = R(0.25) * sa;
+          scale = R(0.25) * scale;
+        }
+        centroid_exact<R>(t[0], t[1], t[2], t[3], t[4], t[5], t[6], t[7], t[8], cx, cy, cz);
+        leaf = lev == maxlev || dist_exact<R>(tx, ty, tz, cx, cy, cz) > sqrt(sa) * R(4);
+      }
+      first = false;
+      if (leaf) {
+        if (lev == 0) leaf_acc<R, GRAD>(tx, ty, tz, cx, cy, cz, gx, gy, gz, gs, acc);
+        else leaf_acc<R, GRAD>(tx, ty, tz, cx, cy, cz, gx * scale, gy * scale, gz * scale, gs * scale, acc);
+        // advance to the next node in preorder after this subtree
+        while (lev > 0) {
+          const int sh = 2 * (lev - 1);
+          const int c = (path >> sh) & 3;
+          if (c < 3) { path += 1 << sh; break; }
+          path &= ~(3 << sh);
+          --lev;
+        }
+        if (lev == 0) done = true;
+      } else {
+        ++lev;   // first child: digit 0
+      }
+    }
+  }
+}
+
+template <bool GRAD, int PREC>
+__global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
+  using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
+  extern __shared__ float4 shp[];
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = i < a.nt;
+  const int64_t ic = active ? i : a.nt - 1;
+  const R tx = a.tx[ic], ty = a.ty[ic], tz = a.tz[ic];
+  R acc[16];
+  float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
+  double gd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int c = 0; c < 16; ++c) acc[c] = R(0);
+  for (int64_t p0 = 0; p0 < a.np; p0 += a.tile) {
+    const int cnt = (int)((a.np - p0) < a.tile ? (a.np - p0) : a.tile);
+    __syncthreads();
+    stage_panels(a, p0, cnt, shp);
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) walk_panel<R, GRAD>(tx, ty, tz, shp + PAN_F4 * j, a.maxlev, active, acc);
+    if (PREC == PREC_COMP) {
+      two_sum(hi[0], lo[0], (float)acc[0]);
+      two_sum(hi[1], lo[1], (float)acc[1]);
+      two_sum(hi[2], lo[2], (float)acc[2]);
+      if (GRAD) {
+        R g[9];
+        grad_of<R>(acc, g);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) gd[k] += (double)g[k];
+      }
+#pragma unroll
+      for (int c = 0; c < 16; ++c) acc[c] = R(0);
+    }
+  }
+  if (!active) return;
+  if (PREC == PREC_FP64) {
+    double* u = (double*)a.u; double* v = (double*)a.v; double* w = (double*)a.w;
+    u[i] += acc[0]; v[i] += acc[1]; w[i] += acc[2];
+    if (GRAD) {
+      R g[9];
+      grad_of<R>(acc, g);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) ((double*)a.g[k])[i] += g[k];
+    }
+  } else if (PREC == PREC_FP32) {
+    float* u = (float*)a.u; float* v = (float*)a.v; float* w = (float*)a.w;
+    u[i] += (float)acc[0]; v[i] += (float)acc[1]; w[i] += (float)acc[2];
+    if (GRAD) {
+      R g[9];
+      grad_of<R>(acc, g);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) ((float*)a.g[k])[i] += (float)g[k];
+    }
+  } else {
+    float* u = (float*)a.u; float* v = (float*)a.v; float* w = (float*)a.w;
+    u[i] = (float)((double)u[i] + (double)hi[0] + (double)lo[0]);
+    v[i] = (float)((double)v[i] + (double)hi[1] + (double)lo[1]);
+    w[i] = (float)((double)w[i] + (double)hi[2] + (double)lo[2]);
+    if (GRAD) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) ((float*)a.g[k])[i] = (float)((double)((float*)a.g[k])[i] + gd[k]);
+    }
+  }
+}
+
+}  // namespace o3d

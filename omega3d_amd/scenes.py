@@ -90,9 +90,9 @@ def _torus_points(rng, n, R, a):
     """n points exactly uniform in volume inside a torus around the z axis:
     theta ~ U[0,2pi), cross-section (r, phi) with r = a sqrt(u), accepted with
     probability (R + r cos phi)/(R + a)."""
-    out_t = []
-    out_r = []
-    out_p = []
+    out_t = [np.zeros(0)]
+    out_r = [np.zeros(0)]
+    out_p = [np.zeros(0)]
     have = 0
     while have < n:
         m = int((n - have) * 1.25) + 64
@@ -115,7 +115,7 @@ def ring(n=8192, R=1.0, a=0.1, sigma=0.05, gamma=1.0, seed=SEED_BASE + 0, stream
     """Single vortex ring in the xy-plane; |alpha| = gamma/n (total circulation gamma)."""
     rng = Rng(seed, stream)
     pos, tang = _torus_points(rng, n, R, a)
-    mag = np.full(n, gamma / n)
+    mag = np.full(n, gamma / max(n, 1))
     if jitter:
         mag *= 1.0 + jitter * (2.0 * rng.uniform(n) - 1.0)
     return Particles(pos.astype(np.float32), (tang * mag).astype(np.float32),
