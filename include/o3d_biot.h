@@ -226,6 +226,77 @@ int o3d_tune_p2p(const o3d_points* src, o3d_points* targ, const o3d_opts* opts, 
 int o3d_tune_save(const char* path);
 int o3d_tune_load(const char* path);
 
+/* ---- serving: device-resident simulations stepped by continuous batching ------
+ * (no counterpart in the reference; each request is one inviscid Ralston RK2
+ * convection step with vortex stretching, Convection.h:349-425, of a session whose
+ * particle state stays in HBM between steps).
+ *
+ * A server owns one HBM arena of opts.hbm_budget bytes, one host thread and one
+ * stream.  Session state costs 32 B/particle (position, strength, radius,
+ * elongation); a step request needs 120 B/particle more while it runs (interim
+ * state, both stages' velocity and gradient).  Requests wait in arrival order
+ * and are admitted as soon as a slot and arena space are free, in any tick; each
+ * tick evaluates a budget of (request, target-chunk) items from every admitted
+ * request in one persistent launch, then applies the RK2 stage updates of the
+ * requests whose stage completed.  A step's result does not depend on what else
+ * was in the batch.  Steps of one session run in submission order. */
+typedef struct o3d_server o3d_server;
+
+typedef struct o3d_server_opts {
+  size_t hbm_budget;          /* arena bytes (0 = 8 GiB)                               */
+  int32_t max_active;         /* admitted requests at once (0 = 64)                    */
+  int32_t tile;               /* kernel tile / targets per thread (0 = defaults)       */
+  int32_t tpt;
+  int32_t use_graph;          /* replay each tick from a captured HIP graph            */
+  double tick_interactions;   /* pair-evaluation budget of one tick (0 = 4e10)         */
+} o3d_server_opts;
+
+enum o3d_request_state { O3D_RQ_QUEUED = 0, O3D_RQ_RUNNING = 1, O3D_RQ_DONE = 2, O3D_RQ_FAILED = 3,
+                         O3D_RQ_CANCELLED = 4 };
+
+typedef struct o3d_request_info {
+  int32_t state;              /* enum o3d_request_state                                */
+  int32_t status;             /* o3d_status of the step (O3D_OK when done)             */
+  double submit_s;            /* server clock, seconds                                 */
+  double admit_s;
+  double done_s;
+  double latency_s;           /* done - submit                                         */
+  double interactions;        /* 2 N^2 for a step                                      */
+} o3d_request_info;
+
+typedef struct o3d_server_stats {
+  int64_t submitted, completed, failed, cancelled, queued, running;
+  double p50_s, p99_s, mean_s;      /* latency of completed steps                       */
+  double interactions;              /* pair evaluations of completed steps              */
+  double wall_s;                    /* first submit -> last completion                  */
+  double ginter_per_s;              /* interactions / wall_s / 1e9                      */
+  size_t hbm_used, hbm_budget, hbm_peak;
+  int64_t ticks;
+} o3d_server_stats;
+
+int o3d_server_create(o3d_server** s, const o3d_server_opts* opts);
+int o3d_server_destroy(o3d_server* s);   /* waits for running steps, cancels queued ones */
+/* host arrays in (x, s, r of *p); core fixed per session; O3D_EADMIT if the state
+ * does not fit the arena, O3D_EINVAL on non-finite input */
+int o3d_server_open_session(o3d_server* s, const o3d_points* p, int32_t core, int64_t* session);
+int o3d_server_close_session(o3d_server* s, int64_t session);
+/* copy a session's current state to host arrays (waits for its queued steps);
+ * elong may be NULL */
+int o3d_server_read_session(o3d_server* s, int64_t session, float* const x[3], float* const str[3],
+                            float* elong);
+/* queue one step: dt, precision (FP32 or compensated), freestream fs[3] (NULL = 0),
+ * timeout_s (<= 0: none; a request still queued at its deadline is cancelled) */
+int o3d_server_submit(o3d_server* s, int64_t session, double dt, int32_t precision, const double* fs,
+                      double timeout_s, int64_t* request);
+/* wait up to timeout_s (< 0: forever) for a request to end; O3D_EBUSY if it has not */
+int o3d_server_wait(o3d_server* s, int64_t request, double timeout_s, o3d_request_info* info);
+int o3d_server_cancel(o3d_server* s, int64_t request);
+int o3d_server_get_stats(o3d_server* s, o3d_server_stats* st);
+int o3d_server_reset_stats(o3d_server* s);
+/* test hook: the next `count` admitted requests fail with O3D_EINJECTED instead of
+ * running, as if their kernels had failed (kind 1); kind 0 clears */
+int o3d_inject_fault(int32_t kind, int32_t count);
+
 /* ---- device plumbing (no counterpart in the reference: device-resident use) -- */
 int o3d_dev_malloc(void** ptr, size_t bytes);
 int o3d_dev_free(void* ptr);

@@ -53,38 +53,58 @@ __device__ __forceinline__ float horner_h(float q) {
   return fmaf(p, q, 0.5f);
 }
 
-// K = (1 - exp(-q))/rho^3,  B = 3 (c e^-q - K)/rho^2  with c = 1/cs, q = rho^3 c;
-// q < 0.001: K = c, B = -1.5 rho c^2 (CoreFunc.h:124-126, 172-174).
-__device__ __forceinline__ void exp_slow(float d2m, float ir, float ir2, float ood3, float cs,
-                                         bool far, float& K, float& B) {
+// Every core returns K and Bp = -B/3, where u += K (a x d) and the gradient's
+// dyad part is B d (x) (a x d); the factor -3 is applied once per target.
+//
+// Compact exponential, far branch: q = rho^3/cs > 16  <=>  rho^6 > 256 cs^2, where
+// K = 1/rho^3 and B = -3K/rho^2 (CoreFunc.h:121,169).  Elsewhere (CoreFunc.h:124-130,
+// 172-178), with c = 1/cs and e = exp(-q):
+//   q < 0.001:  K = c,                 B = -1.5 rho c^2
+//   q < 1:      K = c f(q),            B = -3 c^2 rho h(q)      (series, no cancellation)
+//   otherwise:  K = (1 - e)/rho^3,     B = 3 (c e - K)/rho^2
+__device__ __forceinline__ void exp_far(float d2, float& K, float& Bp) {
+  const float ir = frsq(d2);
+  const float ir2 = ir * ir;
+  K = ir2 * ir;
+  Bp = K * ir2;
+}
+__device__ __forceinline__ void exp_any(float d2, float cs, float& K, float& Bp) {
+  const bool far = d2 * d2 * d2 > 256.0f * (cs * cs);
+  float kf, bf;
+  exp_far(d2, kf, bf);                           // the far values, as on the fast path
+  const float d2m = fmaxf(d2, 1.17549435e-38f);
+  const float ir = frsq(d2m);
+  const float ir2 = ir * ir;
+  const float ood3 = ir2 * ir;
   const float c = frcp(cs);
   const float dist = d2m * ir;
   const float q = (d2m * dist) * c;
   const float e = fexp2(q * -1.44269504088896341f);
   const float kg = fmaf(-ood3, e, ood3);
-  const float bg = 3.0f * fmaf(c, e, -kg) * ir2;
+  const float bg = fmaf(-c, e, kg) * ir2;
   const float ks = c * horner_f(q);
-  const float bs = (-3.0f * c) * (c * dist) * horner_h(q);
+  const float bs = c * (c * dist) * horner_h(q);
   const bool small = q < 1.0f;
   float kn = small ? ks : kg;
   float bn = small ? bs : bg;
   const bool near = q < 0.001f;
   kn = near ? c : kn;
-  bn = near ? (-1.5f * dist) * c * c : bn;
-  K = far ? K : kn;
-  B = far ? B : bn;
+  bn = near ? (0.5f * dist) * c * c : bn;
+  K = far ? kf : kn;
+  Bp = far ? bf : bn;
 }
 
-// K and B of one pair from rho^2 (d2) and the pair's core parameter st
-// (sum of the two particles' core_term).
-template <int CORE, bool GRAD>
-__device__ __forceinline__ void kb(float d2, float st, float& K, float& B) {
+// K and Bp of one pair from rho^2 (d2) and the pair's core parameter st (sum of the
+// two particles' core_term).  thr (EXP only): rho^2 above which every pair of this
+// target with the current source tile is in the far branch.
+template <int CORE>
+__device__ __forceinline__ void kb(float d2, float st, float thr, float& K, float& Bp) {
   if (CORE == CORE_RM) {                       // CoreFunc.h:65-71
     const float r2 = d2 + st;
     const float ir = frsq(r2);
     const float ir2 = ir * ir;
     K = ir2 * ir;
-    B = K * (ir2 * -3.0f);
+    Bp = K * ir2;
   } else if (CORE == CORE_WL) {                // CoreFunc.h:265-274
     const float dd = d2 + st;
     const float top = fmaf(2.5f, st, d2);
@@ -92,35 +112,32 @@ __device__ __forceinline__ void kb(float d2, float st, float& K, float& B) {
     const float ir2 = ir * ir;
     const float dn5 = ir2 * ir2 * ir;
     K = top * dn5;
-    B = dn5 * fmaf(-5.0f, top * ir2, 2.0f);
+    Bp = dn5 * fmaf((float)(5.0 / 3.0), top * ir2, (float)(-2.0 / 3.0));
   } else if (CORE == CORE_V2) {                // CoreFunc.h:317-328
     const float den = fmaf(d2, d2, st);
     const float rs = frsq(den);
     K = rs * fsqrt(rs);
-    B = K * (rs * -3.0f);
+    Bp = K * rs;
   } else {                                     // CoreFunc.h:225-236
-    const float d2m = fmaxf(d2, 1.17549435e-38f);
-    const float ir = frsq(d2m);
-    const float ir2 = ir * ir;
-    const float ood3 = ir2 * ir;
-    K = ood3;
-    B = ood3 * (ir2 * -3.0f);
-    const bool far = ood3 * st < 0.0625f;      // rho^3/(ss^3+st^3) > 16
-    if (__builtin_expect(wave_any(!far), 0)) exp_slow(d2m, ir, ir2, ood3, st, far, K, B);
+    if (__builtin_expect(wave_any(!(d2 > thr)), 0)) exp_any(d2, st, K, Bp);
+    else exp_far(d2, K, Bp);
   }
 }
 
-// acc: [0..2] velocity, [3..11] sum_j B_j d_j (x) (alpha_j x d_j) in ug order,
-//      [12..14] W = sum_j K_j alpha_j
+// acc: [0..2] velocity; [3..10] D(j,i) = sum_j Bp d_j (a x d)_i for
+// (j,i) = 00 01 02 10 11 12 20 21 (D22 = -D00 - D11: d . (a x d) = 0);
+// [11..13] W = sum_j K_j a_j
+constexpr int NACC = 14;
+
 template <int CORE, bool GRAD>
-__device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt, const float4 P,
+__device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt, float thr, const float4 P,
                                          const float4 A, float* acc) {
   const float dx = tx - P.x;
   const float dy = ty - P.y;
   const float dz = tz - P.z;
   const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-  float K, B;
-  kb<CORE, GRAD>(d2, P.w + tt, K, B);
+  float K, Bp;
+  kb<CORE>(d2, P.w + tt, thr, K, Bp);
   const float cx = fmaf(dz, A.y, -dy * A.z);
   const float cy = fmaf(dx, A.z, -dz * A.x);
   const float cz = fmaf(dy, A.x, -dx * A.y);
@@ -128,7 +145,7 @@ __device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt,
   acc[1] = fmaf(K, cy, acc[1]);
   acc[2] = fmaf(K, cz, acc[2]);
   if (GRAD) {
-    const float bx = B * cx, by = B * cy, bz = B * cz;
+    const float bx = Bp * cx, by = Bp * cy, bz = Bp * cz;
     acc[3] = fmaf(dx, bx, acc[3]);
     acc[4] = fmaf(dx, by, acc[4]);
     acc[5] = fmaf(dx, bz, acc[5]);
@@ -137,57 +154,81 @@ __device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt,
     acc[8] = fmaf(dy, bz, acc[8]);
     acc[9] = fmaf(dz, bx, acc[9]);
     acc[10] = fmaf(dz, by, acc[10]);
-    acc[11] = fmaf(dz, bz, acc[11]);
-    acc[12] = fmaf(K, A.x, acc[12]);
-    acc[13] = fmaf(K, A.y, acc[13]);
-    acc[14] = fmaf(K, A.z, acc[14]);
+    acc[11] = fmaf(K, A.x, acc[11]);
+    acc[12] = fmaf(K, A.y, acc[12]);
+    acc[13] = fmaf(K, A.z, acc[13]);
   }
 }
 
-// skew(W) added to the dyad part gives the gradient (Kernels.h:183-192):
-// g1 += Wz, g2 -= Wy, g3 -= Wz, g5 += Wx, g6 += Wy, g7 -= Wx
+// gradient in ug order from the accumulators (Kernels.h:183-192):
+// g = -3 D + skew(W), skew: g1 += Wz, g2 -= Wy, g3 -= Wz, g5 += Wx, g6 += Wy, g7 -= Wx
+template <class T>
+__device__ __forceinline__ void grad_from(const T* a, T* g) {
+  g[0] = T(-3) * a[3];
+  g[1] = T(-3) * a[4] + a[13];
+  g[2] = T(-3) * a[5] - a[12];
+  g[3] = T(-3) * a[6] - a[13];
+  g[4] = T(-3) * a[7];
+  g[5] = T(-3) * a[8] + a[11];
+  g[6] = T(-3) * a[9] + a[12];
+  g[7] = T(-3) * a[10] - a[11];
+  g[8] = T(3) * (a[3] + a[7]);
+}
+
 __device__ __forceinline__ void fold_tile_comp(float* acc, float* hi, float* lo, double* gd, bool grad) {
   two_sum(hi[0], lo[0], acc[0]);
   two_sum(hi[1], lo[1], acc[1]);
   two_sum(hi[2], lo[2], acc[2]);
   acc[0] = acc[1] = acc[2] = 0.0f;
   if (grad) {
-    const double wx = acc[12], wy = acc[13], wz = acc[14];
-    gd[0] += (double)acc[3];
-    gd[1] += (double)acc[4] + wz;
-    gd[2] += (double)acc[5] - wy;
-    gd[3] += (double)acc[6] - wz;
-    gd[4] += (double)acc[7];
-    gd[5] += (double)acc[8] + wx;
-    gd[6] += (double)acc[9] + wy;
-    gd[7] += (double)acc[10] - wx;
-    gd[8] += (double)acc[11];
+    double a[NACC];
 #pragma unroll
-    for (int c = 3; c < 15; ++c) acc[c] = 0.0f;
+    for (int c = 3; c < NACC; ++c) a[c] = (double)acc[c];
+    double g[9];
+    grad_from<double>(a, g);
+#pragma unroll
+    for (int c = 0; c < 9; ++c) gd[c] += g[c];
+#pragma unroll
+    for (int c = 3; c < NACC; ++c) acc[c] = 0.0f;
   }
 }
 
-// stage sources [s0, s0+tile) into LDS as {x,y,z,term} {ax,ay,az,0}
+// far-branch threshold of the exponential core for one target and one source
+// tile: rho^2 > (256 cs_max^2)^(1/3) (1 + 2^-16) implies rho^6 > 256 cs^2 for every
+// source of the tile (cs <= cs_max); the margin covers the log2/exp2 error.
+__device__ __forceinline__ float exp_threshold(float csmax) {
+  const float l = __builtin_amdgcn_logf(256.0f * (csmax * csmax));
+  return fexp2(l * (float)(1.0 / 3.0)) * 1.0000153f;
+}
+
+// stage sources [s0, s0+tile) into LDS as {x,y,z,term} {ax,ay,az,0}; *tmax
+// (LDS) receives the largest core term of the tile (float bits compare as ints)
 template <int CORE>
-__device__ __forceinline__ void stage_tile(const SrcView& s, int64_t s0, int cnt, int tile, float4* sh) {
+__device__ __forceinline__ void stage_tile(const SrcView& s, int64_t s0, int cnt, int tile, float4* sh,
+                                           unsigned int* tmax) {
+  unsigned int mx = 0u;
   for (int j = threadIdx.x; j < tile; j += BLOCK) {
     float4 P = make_float4(0.f, 0.f, 0.f, 1.f), A = make_float4(0.f, 0.f, 0.f, 0.f);
     if (j < cnt) {
       const int64_t g = s0 + j;
       P = make_float4(s.x[g], s.y[g], s.z[g], core_term<CORE>(s.r[g]));
       A = make_float4(s.ax[g], s.ay[g], s.az[g], 0.f);
+      mx = max(mx, __float_as_uint(fmaxf(P.w, 0.0f)));
     }
     sh[2 * j] = P;
     sh[2 * j + 1] = A;
   }
+  if (CORE == CORE_EXP) atomicMax(tmax, mx);
 }
 
 // One chunk of BLOCK*TPT targets [t_base, t_base + BLOCK*TPT) against sources
-// [0, ns) of (s, t).  Shared by the single-collection and the batched kernels so
-// that a target's result does not depend on which launch computed it.
-template <int CORE, bool THICK, bool GRAD, int PREC, int TPT>
+// [0, ns) of (s, t).  Shared by the single-collection, batched and serving kernels
+// so that a target's result does not depend on which launch computed it.
+// ACC: add into the outputs (the reference's semantics); false: overwrite them.
+template <int CORE, bool THICK, bool GRAD, int PREC, int TPT, bool ACC = true>
 __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, int64_t ns, int64_t nt,
                                           int64_t t_base, int tile, bool self_excl, float4* sh) {
+  __shared__ unsigned int s_tmax;
   const int tid = threadIdx.x;
   float tx[TPT], ty[TPT], tz[TPT], tt[TPT];
   int64_t ti[TPT];
@@ -199,13 +240,13 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
     tx[k] = t.x[ic]; ty[k] = t.y[ic]; tz[k] = t.z[ic];
     tt[k] = THICK ? core_term<CORE>(t.r[ic]) : 0.0f;
   }
-  float acc[TPT][15];
+  float acc[TPT][NACC];
   float hi[TPT][3], lo[TPT][3];
   double gd[TPT][9];
 #pragma unroll
   for (int k = 0; k < TPT; ++k) {
 #pragma unroll
-    for (int c = 0; c < 15; ++c) acc[k][c] = 0.0f;
+    for (int c = 0; c < NACC; ++c) acc[k][c] = 0.0f;
 #pragma unroll
     for (int c = 0; c < 3; ++c) { hi[k][c] = 0.0f; lo[k][c] = 0.0f; }
 #pragma unroll
@@ -215,13 +256,18 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
   for (int64_t s0 = 0; s0 < ns; s0 += tile) {
     const int cnt = (int)((ns - s0) < tile ? (ns - s0) : tile);
     __syncthreads();
-    stage_tile<CORE>(s, s0, cnt, tile, sh);
+    if (CORE == CORE_EXP && tid == 0) s_tmax = 0u;
     __syncthreads();
+    stage_tile<CORE>(s, s0, cnt, tile, sh, &s_tmax);
+    __syncthreads();
+    float thr[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) thr[k] = CORE == CORE_EXP ? exp_threshold(__uint_as_float(s_tmax) + tt[k]) : 0.0f;
     for (int j = 0; j < cnt; ++j) {
       const float4 P = sh[2 * j];
       const float4 A = sh[2 * j + 1];
 #pragma unroll
-      for (int k = 0; k < TPT; ++k) pair_acc<CORE, GRAD>(tx[k], ty[k], tz[k], tt[k], P, A, acc[k]);
+      for (int k = 0; k < TPT; ++k) pair_acc<CORE, GRAD>(tx[k], ty[k], tz[k], tt[k], thr[k], P, A, acc[k]);
     }
     if (PREC == PREC_COMP) {
 #pragma unroll
@@ -237,8 +283,8 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
     ks[k] = 0.0f; sax[k] = say[k] = saz[k] = 0.0f;
     if (GRAD && self_excl) {
       const int64_t ic = ti[k] < nt ? ti[k] : nt - 1;
-      float B;
-      kb<CORE, GRAD>(0.0f, core_term<CORE>(s.r[ic]) + tt[k], ks[k], B);
+      float Bp;
+      kb<CORE>(0.0f, core_term<CORE>(s.r[ic]) + tt[k], 0.0f, ks[k], Bp);
       sax[k] = s.ax[ic]; say[k] = s.ay[ic]; saz[k] = s.az[ic];
     }
   }
@@ -248,38 +294,60 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
     const int64_t i = ti[k];
     if (i >= nt) continue;
     if (PREC == PREC_FP32) {
-      t.u[i] += acc[k][0];
-      t.v[i] += acc[k][1];
-      t.w[i] += acc[k][2];
+      float r[12];
+      r[0] = acc[k][0];
+      r[1] = acc[k][1];
+      r[2] = acc[k][2];
       if (GRAD) {
-        const float wx = fmaf(-ks[k], sax[k], acc[k][12]);
-        const float wy = fmaf(-ks[k], say[k], acc[k][13]);
-        const float wz = fmaf(-ks[k], saz[k], acc[k][14]);
-        t.g[0][i] += acc[k][3];
-        t.g[1][i] += acc[k][4] + wz;
-        t.g[2][i] += acc[k][5] - wy;
-        t.g[3][i] += acc[k][6] - wz;
-        t.g[4][i] += acc[k][7];
-        t.g[5][i] += acc[k][8] + wx;
-        t.g[6][i] += acc[k][9] + wy;
-        t.g[7][i] += acc[k][10] - wx;
-        t.g[8][i] += acc[k][11];
+        acc[k][11] = fmaf(-ks[k], sax[k], acc[k][11]);
+        acc[k][12] = fmaf(-ks[k], say[k], acc[k][12]);
+        acc[k][13] = fmaf(-ks[k], saz[k], acc[k][13]);
+        grad_from<float>(acc[k], r + 3);
+      }
+      if (ACC) {
+        t.u[i] += r[0]; t.v[i] += r[1]; t.w[i] += r[2];
+        if (GRAD) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) t.g[c][i] += r[3 + c];
+        }
+      } else {
+        t.u[i] = r[0]; t.v[i] = r[1]; t.w[i] = r[2];
+        if (GRAD) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) t.g[c][i] = r[3 + c];
+        }
       }
     } else {
-      t.u[i] = (float)((double)t.u[i] + (double)hi[k][0] + (double)lo[k][0]);
-      t.v[i] = (float)((double)t.v[i] + (double)hi[k][1] + (double)lo[k][1]);
-      t.w[i] = (float)((double)t.w[i] + (double)hi[k][2] + (double)lo[k][2]);
+      double r[12];
+      r[0] = (double)hi[k][0] + (double)lo[k][0];
+      r[1] = (double)hi[k][1] + (double)lo[k][1];
+      r[2] = (double)hi[k][2] + (double)lo[k][2];
       if (GRAD) {
         const double kx = (double)ks[k] * sax[k], ky = (double)ks[k] * say[k], kz = (double)ks[k] * saz[k];
-        t.g[0][i] = (float)((double)t.g[0][i] + gd[k][0]);
-        t.g[1][i] = (float)((double)t.g[1][i] + (gd[k][1] - kz));
-        t.g[2][i] = (float)((double)t.g[2][i] + (gd[k][2] + ky));
-        t.g[3][i] = (float)((double)t.g[3][i] + (gd[k][3] + kz));
-        t.g[4][i] = (float)((double)t.g[4][i] + gd[k][4]);
-        t.g[5][i] = (float)((double)t.g[5][i] + (gd[k][5] - kx));
-        t.g[6][i] = (float)((double)t.g[6][i] + (gd[k][6] - ky));
-        t.g[7][i] = (float)((double)t.g[7][i] + (gd[k][7] + kx));
-        t.g[8][i] = (float)((double)t.g[8][i] + gd[k][8]);
+        r[3] = gd[k][0];
+        r[4] = gd[k][1] - kz;
+        r[5] = gd[k][2] + ky;
+        r[6] = gd[k][3] + kz;
+        r[7] = gd[k][4];
+        r[8] = gd[k][5] - kx;
+        r[9] = gd[k][6] - ky;
+        r[10] = gd[k][7] + kx;
+        r[11] = gd[k][8];
+      }
+      if (ACC) {
+        t.u[i] = (float)((double)t.u[i] + r[0]);
+        t.v[i] = (float)((double)t.v[i] + r[1]);
+        t.w[i] = (float)((double)t.w[i] + r[2]);
+        if (GRAD) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) t.g[c][i] = (float)((double)t.g[c][i] + r[3 + c]);
+        }
+      } else {
+        t.u[i] = (float)r[0]; t.v[i] = (float)r[1]; t.w[i] = (float)r[2];
+        if (GRAD) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) t.g[c][i] = (float)r[3 + c];
+        }
       }
     }
   }
