@@ -44,6 +44,12 @@ def parse():
     ap.add_argument("--tpt", type=int, default=0)
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run the known-traffic copy kernel on this many MiB (counter calibration)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 256-instance batched workload")
+    ap.add_argument("--no-serve", action="store_true", help="skip the served RK2 workload")
+    ap.add_argument("--no-panels", action="store_true", help="skip the icosphere panel workload")
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--sweep", action="store_true",
+                    help="time every (precision, tile, tpt) on the N=1M workload, print them and exit")
     return ap.parse_args()
 
 
@@ -99,6 +105,25 @@ def main():
         wall = time.perf_counter() - t
         kms = np.array([a.elapsed_ms(b) for a, b in evs])
         return wall, kms
+
+    if args.sweep:
+        res = []
+        for prec in ("comp", "fp32"):
+            for tile in (128, 256, 512, 1024):
+                for tpt in (1, 2, 4):
+                    o = _lib.make_opts("exp", True, prec, 0, tile, tpt)
+                    e0, e1 = _lib.Event(), _lib.Event()
+                    _lib.check(L.o3d_points_affect_points_dev(C.byref(src), C.byref(tgt), C.byref(o), None))
+                    e0.record()
+                    _lib.check(L.o3d_points_affect_points_dev(C.byref(src), C.byref(tgt), C.byref(o), None))
+                    e1.record()
+                    _lib.synchronize()
+                    ms = e0.elapsed_ms(e1)
+                    res.append({"precision": prec, "tile": tile, "tpt": tpt, "ms": round(ms, 2),
+                                "ginter_s": round(float(N) * N / ms / 1e6, 1)})
+                    log(f"sweep: {res[-1]}")
+        print(json.dumps({"sweep": res}))
+        return
 
     if args.copy_probe_mb:
         nb = args.copy_probe_mb << 20
@@ -175,6 +200,16 @@ def main():
                              f"{dt:.1f} s"}
             log(f"bench: cpu baseline {cpu}")
 
+    from omega3d_amd import workloads
+    batched = None if args.no_batch else workloads.batched(log=log)
+    served = None if args.no_serve else workloads.served(duration=args.serve_seconds, log=log)
+    panels = None if args.no_panels else workloads.panels_run(log=log)
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "p2p_exp_n1m_traffic.json")
+    if os.path.exists(tfile):
+        traffic = json.load(open(tfile)).get("traffic_bytes")
+
     out = {
         "metric": "G interactions/s (vel+grad) at N=1M vs plain FP32; p50/p99 latency, mixed N",
         "value": round(value, 3),
@@ -193,15 +228,20 @@ def main():
                    "n": N, "core": "exp", "precision": "compensated", "self_term": "included"},
         "plain_fp32": {"value": round(plain_value, 3), "unit": "G interactions/s",
                        "kernel_ms": round(float(pkms.mean()), 3)},
-        "latency_ms": {"p50": round(float(np.percentile(kms, 50)), 3),
-                       "p99": round(float(np.percentile(kms, 99)), 3),
-                       "what": "per-step kernel time of this workload"},
+        "latency_ms": ({"p50": round(served["p50_s"] * 1e3, 3), "p99": round(served["p99_s"] * 1e3, 3),
+                        "what": "served RK2 step latency, mixed N (see served)"} if served else
+                       {"p50": round(float(np.percentile(kms, 50)), 3),
+                        "p99": round(float(np.percentile(kms, 99)), 3),
+                        "what": "per-step kernel time of this workload"}),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                      "kernel": KERNEL_NAME, "kernel_ms": round(float(kms.mean()), 3),
                      "flops_per_launch": flops},
         "errors_vs_fp64": errors,
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
+        "panels": panels,
         "device": dev["name"],
     }
     print(json.dumps(out), flush=True)

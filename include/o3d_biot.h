@@ -248,6 +248,8 @@ typedef struct o3d_server_opts {
   int32_t tile;               /* kernel tile / targets per thread (0 = defaults)       */
   int32_t tpt;
   int32_t use_graph;          /* replay each tick from a captured HIP graph            */
+  int32_t adaptive_tick;      /* raise the budget to (workgroup slots x the largest
+                                 active chunk) so that large sessions fill the GPU      */
   double tick_interactions;   /* pair-evaluation budget of one tick (0 = 4e10)         */
 } o3d_server_opts;
 

@@ -129,6 +129,65 @@ __device__ __forceinline__ void kb(float d2, float st, float thr, float& K, floa
 // [11..13] W = sum_j K_j a_j
 constexpr int NACC = 14;
 
+// accumulate one pair given its separation d = t - s and K, Bp
+template <bool GRAD>
+__device__ __forceinline__ void acc_pair(float dx, float dy, float dz, float K, float Bp, const float4 A, float* acc) {
+  const float cx = fmaf(dz, A.y, -dy * A.z);
+  const float cy = fmaf(dx, A.z, -dz * A.x);
+  const float cz = fmaf(dy, A.x, -dx * A.y);
+  acc[0] = fmaf(K, cx, acc[0]);
+  acc[1] = fmaf(K, cy, acc[1]);
+  acc[2] = fmaf(K, cz, acc[2]);
+  if (GRAD) {
+    const float bx = Bp * cx, by = Bp * cy, bz = Bp * cz;
+    acc[3] = fmaf(dx, bx, acc[3]);
+    acc[4] = fmaf(dx, by, acc[4]);
+    acc[5] = fmaf(dx, bz, acc[5]);
+    acc[6] = fmaf(dy, bx, acc[6]);
+    acc[7] = fmaf(dy, by, acc[7]);
+    acc[8] = fmaf(dy, bz, acc[8]);
+    acc[9] = fmaf(dz, bx, acc[9]);
+    acc[10] = fmaf(dz, by, acc[10]);
+    acc[11] = fmaf(K, A.x, acc[11]);
+    acc[12] = fmaf(K, A.y, acc[12]);
+    acc[13] = fmaf(K, A.z, acc[13]);
+  }
+}
+
+// one source against the TPT targets of this lane.  For the exponential core the
+// far-branch test of all TPT pairs is combined into a single wave-uniform branch,
+// so that the common case is one basic block holding all TPT pair evaluations.
+template <int CORE, bool GRAD, int TPT>
+__device__ __forceinline__ void source_acc(const float* tx, const float* ty, const float* tz, const float* tt,
+                                           const float* thr, const float4 P, const float4 A, float (*acc)[NACC]) {
+  float dx[TPT], dy[TPT], dz[TPT], d2[TPT];
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) {
+    dx[k] = tx[k] - P.x;
+    dy[k] = ty[k] - P.y;
+    dz[k] = tz[k] - P.z;
+    d2[k] = fmaf(dz[k], dz[k], fmaf(dy[k], dy[k], dx[k] * dx[k]));
+  }
+  float K[TPT], Bp[TPT];
+  if (CORE == CORE_EXP) {
+    bool allfar = true;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) allfar = allfar && (d2[k] > thr[k]);
+    if (__builtin_expect(wave_any(!allfar), 0)) {
+#pragma unroll
+      for (int k = 0; k < TPT; ++k) exp_any(d2[k], P.w + tt[k], K[k], Bp[k]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < TPT; ++k) exp_far(d2[k], K[k], Bp[k]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) kb<CORE>(d2[k], P.w + tt[k], thr[k], K[k], Bp[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) acc_pair<GRAD>(dx[k], dy[k], dz[k], K[k], Bp[k], A, acc[k]);
+}
+
 template <int CORE, bool GRAD>
 __device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt, float thr, const float4 P,
                                          const float4 A, float* acc) {
@@ -266,8 +325,7 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
     for (int j = 0; j < cnt; ++j) {
       const float4 P = sh[2 * j];
       const float4 A = sh[2 * j + 1];
-#pragma unroll
-      for (int k = 0; k < TPT; ++k) pair_acc<CORE, GRAD>(tx[k], ty[k], tz[k], tt[k], thr[k], P, A, acc[k]);
+      source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
     }
     if (PREC == PREC_COMP) {
 #pragma unroll

@@ -549,7 +549,15 @@ int plan_tick(o3d_server* S, bool* ran) {
     if (!r.finished_planning) act.push_back(&r);
   }
   if (act.empty()) return O3D_OK;
+  // A chunk of a large session keeps one workgroup busy for a long time, so the
+  // tick must hold enough such chunks to occupy every workgroup slot: the budget
+  // is at least slots x the largest active chunk.
   double budget = S->opts.tick_interactions;
+  if (S->opts.adaptive_tick) {
+    double biggest = 0;
+    for (Request* r : act) biggest = std::max(biggest, chunk_cost(S, S->sessions.at(r->sid), 0));
+    budget = std::max(budget, biggest * (double)S->grid);
+  }
   struct Take { Request* r; int64_t from, to; };
   std::vector<Take> take;
   auto remaining = [&](Request* r) {
@@ -607,6 +615,10 @@ int plan_tick(o3d_server* S, bool* ran) {
     S->h_inst[b][kv.second] = eval_inst(ses, *kv.first);
     S->h_minst[b][kv.second] = move_inst(ses, *kv.first);
   }
+  // longest items first: with the queue drained greedily this is LPT scheduling
+  std::stable_sort(take.begin(), take.end(), [&](const Take& x, const Take& y) {
+    return S->sessions.at(x.r->sid).n > S->sessions.at(y.r->sid).n;
+  });
   size_t k = 0;
   for (auto& tk : take)
     for (int64_t c = tk.from; c < tk.to; ++c) S->h_items[b][k++] = ServeItem{slot[tk.r], (int32_t)c};

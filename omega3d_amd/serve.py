@@ -20,7 +20,7 @@ from ._lib import CORES, PREC, check, lib
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int32), ("tile", C.c_int32), ("tpt", C.c_int32),
-                ("use_graph", C.c_int32), ("tick_interactions", C.c_double)]
+                ("use_graph", C.c_int32), ("adaptive_tick", C.c_int32), ("tick_interactions", C.c_double)]
 
 
 class RequestInfo(C.Structure):
@@ -62,8 +62,9 @@ def inject_fault(count, kind=1):
 
 
 class Server:
-    def __init__(self, hbm_budget=8 << 30, max_active=64, tile=0, tpt=0, use_graph=True, tick_interactions=0.0):
-        o = ServerOpts(hbm_budget, max_active, tile, tpt, int(use_graph), tick_interactions)
+    def __init__(self, hbm_budget=8 << 30, max_active=64, tile=0, tpt=0, use_graph=True, tick_interactions=0.0,
+                 adaptive_tick=True):
+        o = ServerOpts(hbm_budget, max_active, tile, tpt, int(use_graph), int(adaptive_tick), tick_interactions)
         h = C.c_void_p()
         check(lib.o3d_server_create(C.byref(h), C.byref(o)))
         self.h = h.value

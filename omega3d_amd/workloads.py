@@ -1,0 +1,221 @@
+"""Measurement drivers for the batched, served and panel workloads (used by bench.py).
+
+  batched(k=256)   many independent rings, N log-uniform on [2,048, 262,144], RM core,
+                   R ~ U[0.5, 2], sigma ~ U[0.01, 0.05], compensated FP32: the
+                   persistent batched launch (autotuned tile / targets per thread),
+                   its HIP-graph replay, and one ordinary launch per instance.
+  served(...)      Poisson stream of Ralston RK2 step requests on device-resident
+                   sessions, N log-uniform on [1,024, 1,048,576], single rings (RM,
+                   sigma 0.05) or ring pairs (EXP, sigma 0.02), plain and compensated
+                   requests mixed; admission by HBM footprint; p50/p99 step latency
+                   and aggregate interactions/s.
+  panels()         20,480-panel icosphere -> 524,288 shell targets, recursive
+                   centroid rule, compensated FP32.
+
+    python -m omega3d_amd.workloads batched|served|panels
+"""
+import ctypes as C
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import _lib, scenes
+from .metrics import field_errors
+
+
+def _fp64_self(core, p, sel):
+    """FP64 kernel: targets p[:, sel] against all of p (double device arrays)."""
+    L = _lib.lib
+    d64 = lambda a: _lib.DeviceArray.from_host(np.ascontiguousarray(a, np.float64))
+    sp, ss, sr = d64(p.pos), d64(p.str), d64(p.rad)
+    tp, tr = d64(p.pos[:, sel]), d64(p.rad[sel])
+    nt = len(sel)
+    v = _lib.DeviceArray.from_host(np.zeros((3, nt)))
+    g = _lib.DeviceArray.from_host(np.zeros((9, nt)))
+    s, t = _lib.Points64(), _lib.Points64()
+    s.n, t.n = p.n, nt
+    for d in range(3):
+        s.x[d], s.s[d], t.x[d], t.u[d] = sp.row(d), ss.row(d), tp.row(d), v.row(d)
+    s.r, t.r = sr.ptr, tr.ptr
+    for k in range(9):
+        t.ug[k] = g.row(k)
+    o = _lib.make_opts(core, True, "fp64")
+    _lib.check(L.o3d_points_affect_points64_dev(C.byref(s), C.byref(t), C.byref(o), None))
+    _lib.synchronize()
+    return v.to_host(), g.to_host()
+
+
+def batched(k=256, lo=2048, hi=262144, reps=2, tune=True, log=print):
+    from .batch import Batch, stream_create, stream_destroy
+    t0 = time.perf_counter()
+    inst = scenes.batch_instances(k, lo, hi)
+    n = np.array([p.n for p in inst])
+    pairs = float((n.astype(np.float64) ** 2).sum())
+    log(f"batched: {k} rings, {n.sum()} particles, {pairs:.4g} pairs, generated in {time.perf_counter() - t0:.1f}s")
+    b = Batch(inst, core="rm", precision="comp")
+    s = stream_create()
+    try:
+        ev0, ev1 = _lib.Event(), _lib.Event()
+        tuned = None
+        if tune:
+            bt, bp, ms = b.tune(stream=s, reps=1)
+            tuned = {"tile": bt, "tpt": bp, "ms": round(ms, 3)}
+            log(f"batched: autotuner picked tile {bt}, tpt {bp} ({ms:.1f} ms)")
+
+        def timed(fn):
+            fn()
+            _lib.check(_lib.lib.o3d_stream_synchronize(s))
+            ev0.record(s)
+            for _ in range(reps):
+                fn()
+            ev1.record(s)
+            _lib.check(_lib.lib.o3d_stream_synchronize(s))
+            return ev0.elapsed_ms(ev1) / reps
+
+        ms_batch = timed(lambda: b.run(s))
+        res = b.results()
+        b.capture(s)
+        ms_graph = timed(lambda: b.replay(s))
+        ms_sep = timed(lambda: b.run_separate(s))
+        # accuracy on the largest and the smallest instances, all their targets
+        errs = {}
+        for i in (int(np.argmax(n)), int(np.argmin(n))):
+            p = inst[i]
+            sel = np.arange(p.n) if p.n <= 32768 else np.linspace(0, p.n - 1, 32768).astype(np.int64)
+            vr, gr = _fp64_self("rm", p, sel)
+            v, g = res[i]
+            errs[f"instance_{i}_n{p.n}"] = field_errors(v[:, sel], g[:, sel], vr, gr)
+        cfg = b.config()
+    finally:
+        b.close()
+        stream_destroy(s)
+    gi = lambda ms: pairs / (ms / 1e3) / 1e9
+    out = {"workload": f"{k} independent rings, N log-uniform [{lo}, {hi}], RM core, compensated FP32",
+           "pairs": pairs, "particles": int(n.sum()), "config": cfg, "autotune": tuned,
+           "persistent_ms": round(ms_batch, 3), "persistent_ginter_s": round(gi(ms_batch), 2),
+           "graph_ms": round(ms_graph, 3), "graph_ginter_s": round(gi(ms_graph), 2),
+           "per_instance_ms": round(ms_sep, 3), "per_instance_ginter_s": round(gi(ms_sep), 2),
+           "speedup_vs_per_instance": round(ms_sep / ms_graph, 3), "errors_vs_fp64": errs}
+    log("batched: " + json.dumps({kk: vv for kk, vv in out.items() if kk != "errors_vs_fp64"}))
+    return out
+
+
+def served(duration=20.0, sessions=16, lo=1024, hi=1048576, load=0.5, rate=None, seed=scenes.SEED_BASE + 4,
+           log=print):
+    from .serve import Server
+    rng = scenes.Rng(seed, 11)
+    counts = scenes.log_uniform_counts(sessions, lo, hi, seed)
+    kinds = rng.uniform(sessions) < 0.5
+    t0 = time.perf_counter()
+    srv = Server(hbm_budget=16 << 30, max_active=64)
+    try:
+        sids, ns = [], []
+        for i in range(sessions):
+            if kinds[i]:
+                half = max(1, int(counts[i]) // 2)
+                p = scenes.colliding_rings(n_each=half, seed=seed + 100 + i)
+                core = "exp"
+            else:
+                p = scenes.ring(int(counts[i]), sigma=0.05, seed=seed, stream=100 + i)
+                core = "rm"
+            sids.append(srv.open_session(p, core))
+            ns.append(p.n)
+        ns = np.array(ns, np.float64)
+        log(f"served: {sessions} sessions, {int(ns.sum())} particles resident, set up in "
+            f"{time.perf_counter() - t0:.1f}s")
+        # warm-up: one step of the smallest session
+        srv.wait(srv.submit(sids[int(np.argmin(ns))], 1e-3, "comp"))
+        srv.reset_stats()
+        mean_pairs = float(np.mean(2.0 * ns * ns))
+        if rate is None:
+            # offered load relative to ~1.1e12 pairs/s (compensated FP32, mixed cores)
+            rate = load * 1.1e12 / mean_pairs
+        arr = scenes.Rng(seed, 12)
+        n_req = int(duration * rate) + 1
+        gaps = -np.log1p(-arr.uniform(n_req)) / rate
+        pick = (arr.uniform(n_req) * sessions).astype(int)
+        prec = np.where(arr.uniform(n_req) < 0.5, "comp", "fp32")
+        rids = []
+        t_start = time.perf_counter()
+        t_next = t_start
+        for j in range(n_req):
+            t_next += gaps[j]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            rids.append(srv.submit(sids[pick[j]], 1e-3, str(prec[j])))
+        for r in rids:
+            srv.wait(r)
+        st = srv.stats()
+        wall = time.perf_counter() - t_start
+    finally:
+        srv.close()
+    out = {"workload": f"Poisson RK2 step requests on {sessions} resident sessions, N log-uniform [{lo}, {hi}], "
+                       f"rings (RM) and ring pairs (EXP), plain and compensated mixed",
+           "rate_per_s": round(rate, 3), "requests": len(rids), "completed": st["completed"],
+           "failed": st["failed"], "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
+           "mean_s": round(st["mean_s"], 4), "ginter_per_s": round(st["ginter_per_s"], 2),
+           "wall_s": round(wall, 2), "ticks": st["ticks"], "hbm_peak_bytes": st["hbm_peak"]}
+    log("served: " + json.dumps(out))
+    return out
+
+
+def panels_run(n_targets=524288, reps=2, check_targets=4096, log=print):
+    """The icosphere workload on the GPU: timing, and the error of a target subset
+    against the FP64 panel kernel."""
+    L = _lib.lib
+    pan = scenes.icosphere_panels()
+    tp = scenes.shell_points(n_targets)
+    D = _lib.DeviceArray.from_host
+    dn, di, dts, dar = D(pan.nodes), D(pan.idx), D(pan.ts), D(pan.area)
+    dtp = D(tp)
+    dv, dg = _lib.DeviceArray((3, n_targets)), _lib.DeviceArray((9, n_targets))
+    p = _lib.Panels()
+    p.np, p.nn = pan.np, pan.nodes.shape[1]
+    for d in range(3):
+        p.x[d], p.ts[d] = dn.row(d), dts.row(d)
+    p.idx, p.area, p.ss = di.ptr, dar.ptr, None
+    t = _lib.Points()
+    t.n = n_targets
+    for d in range(3):
+        t.x[d], t.u[d] = dtp.row(d), dv.row(d)
+    for k in range(9):
+        t.ug[k] = dg.row(k)
+    o = _lib.make_opts("rm", True, "comp")
+    e0, e1 = _lib.Event(), _lib.Event()
+    dv.zero(); dg.zero()
+    _lib.check(L.o3d_panels_affect_points_dev(C.byref(p), C.byref(t), C.byref(o), None))
+    _lib.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dv.zero(); dg.zero()
+        _lib.check(L.o3d_panels_affect_points_dev(C.byref(p), C.byref(t), C.byref(o), None))
+    e1.record()
+    _lib.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    pairs = float(pan.np) * n_targets
+    vel, grad = dv.to_host(), dg.to_host()
+    from .influence import pan as pan_host
+    sel = np.linspace(0, n_targets - 1, check_targets).astype(np.int64)
+    vr = np.zeros((3, len(sel)), np.float32)
+    gr = np.zeros((9, len(sel)), np.float32)
+    pan_host("rm", pan.nodes, pan.idx, pan.ts, pan.area, None, tp[:, sel], True, "fp64", vel=vr, grad=gr)
+    out = {"workload": f"{pan.np}-panel icosphere -> {n_targets} targets in 1.02<r<1.5, recursive centroid rule "
+                       f"(maxlev 3), compensated FP32", "ms": round(ms, 3),
+           "g_panel_target_pairs_per_s": round(pairs / (ms / 1e3) / 1e9, 3),
+           "errors_vs_fp64": dict(field_errors(vel[:, sel], grad[:, sel], vr, gr), targets=len(sel))}
+    log("panels: " + json.dumps(out))
+    return out
+
+
+if __name__ == "__main__":
+    _lib.check(_lib.lib.o3d_init(0))
+    what = sys.argv[1] if len(sys.argv) > 1 else "batched"
+    if what == "batched":
+        print(json.dumps(batched()))
+    elif what == "served":
+        print(json.dumps(served()))
+    elif what == "panels":
+        print(json.dumps(panels_run()))

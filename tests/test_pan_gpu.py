@@ -8,7 +8,8 @@ a target's result by the quadrature error (~1e-3), far above these bars:
   * FP32 / compensated against the FP64 oracle: within the reference float path's
     own distance from FP64 (which takes its tree decisions in double and has
     singular near-field leaves): <= max(2e-6, 3 x |f32d - f64|)
-  * the FP64 panel kernel against the CPU FP64 oracle:                max rel <= 1e-12
+  * the FP64 panel kernel against the CPU FP64 oracle: max rel <= 1.2e-7 (the host
+    call accumulates the FP64 result into the caller's float arrays: half-ulp floor)
 """
 import os
 
@@ -52,9 +53,9 @@ def test_panels_vs_reference_golden(infl, core):
                     assert rel_errors(gr, g["grad_f64_" + key])[0] <= _band(g["grad_f32d_" + key],
                                                                             g["grad_f64_" + key])
             v, gr = infl.pan(core, g["nodes"], g["idx"], g["ts"], g["area"], ss, g["tpos"], grads, "fp64")
-            assert rel_errors(v, g["vel_f64_" + key])[0] <= 1e-12
+            assert rel_errors(v, g["vel_f64_" + key])[0] <= 1.2e-7
             if grads:
-                assert rel_errors(gr, g["grad_f64_" + key])[0] <= 1e-12
+                assert rel_errors(gr, g["grad_f64_" + key])[0] <= 1.2e-7
 
 
 @pytest.mark.parametrize("maxlev", [0, 1, 2, 3, 4])

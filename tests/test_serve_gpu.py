@@ -2,8 +2,10 @@
 
 Checks, through the C ABI:
   * one step against the oracle's restatement of advect_2nd_ralston (oracle/rk2.py):
-    position and strength increments within 2e-5 normwise of the reference's float
-    semantics and of the FP64 restatement;
+    positions and strengths within 4 float ulps (plus 2e-5 of the increment) of the
+    reference's float semantics, increments within 1e-4 normwise of the FP64
+    restatement (float storage of positions ~1 puts a 6e-8 absolute floor under
+    increments of ~1e-3);
   * the same session stepped alone and in a mix of concurrent requests ends in
     bit-identical state (fixed tile, per-target summation order);
   * admission by arena footprint, cancellation, deadlines, injected failures that
@@ -34,7 +36,7 @@ def _ring(n, seed, core_sigma=0.05):
 def test_step_matches_oracle(serve, core, prec):
     p = _ring(3000, 11)
     dt, fs = 0.02, (0.05, 0.0, -0.02)
-    srv = serve.Server(hbm_budget=256 << 20, tick_interactions=2e6)   # several ticks per stage
+    srv = serve.Server(hbm_budget=256 << 20, tick_interactions=2e6, adaptive_tick=False)  # several ticks/stage
     try:
         sid = srv.open_session(p, core)
         info = srv.wait(srv.submit(sid, dt, prec, fs))
@@ -47,11 +49,18 @@ def test_step_matches_oracle(serve, core, prec):
     dx, dx64 = x - p.pos, x64 - p.pos.astype(np.float64)
     ds, ds64 = s - p.str, s64 - p.str.astype(np.float64)
     print(core, prec, "dx vs f64", rel_errors(dx, dx64), "ds vs f64", rel_errors(ds, ds64))
-    assert rel_errors(dx, x32 - p.pos)[0] <= 2e-5
-    assert rel_errors(dx, dx64)[0] <= 2e-5
-    assert rel_errors(ds, s32 - p.str)[0] <= 2e-5
-    assert rel_errors(ds, ds64)[0] <= 2e-5
+    _close_float(x, x32, x32 - p.pos)
+    _close_float(s, s32, s32 - p.str)
+    assert rel_errors(dx, dx64)[0] <= 1e-4
+    assert rel_errors(ds, ds64)[0] <= 1e-4
     assert rel_errors((e - 1)[None, :], (e64 - 1)[None, :])[0] <= 1e-4
+
+
+def _close_float(a, b, inc):
+    """a within 4 ulps of b plus 2e-5 of the increment, elementwise"""
+    tol = 4 * np.spacing(np.abs(b)) + 2e-5 * np.abs(inc).max()
+    bad = np.abs(a.astype(np.float64) - b) > tol
+    assert not bad.any(), (np.abs(a - b).max(), tol.max())
 
 
 def test_two_steps_in_order(serve):
@@ -68,8 +77,8 @@ def test_two_steps_in_order(serve):
         srv.close()
     x1, s1, e1 = rk2.rk2_step("rm", p.pos, p.str, p.rad, np.ones(p.n), 0.01, (0, 0, 0), "f64")
     x2, s2, e2 = rk2.rk2_step("rm", x1, s1, p.rad, e1, 0.01, (0, 0, 0), "f64")
-    assert rel_errors(x - p.pos, x2 - p.pos)[0] <= 2e-5
-    assert rel_errors(s - p.str, s2 - p.str)[0] <= 2e-5
+    assert rel_errors(x - p.pos, x2 - p.pos)[0] <= 1e-4
+    assert rel_errors(s - p.str, s2 - p.str)[0] <= 1e-4
 
 
 def test_alone_vs_mix_bitwise(serve):
@@ -83,7 +92,7 @@ def test_alone_vs_mix_bitwise(serve):
         alone = srv.read_session(sid)
     finally:
         srv.close()
-    srv = serve.Server(hbm_budget=512 << 20, tile=256, tpt=1, tick_interactions=5e6)
+    srv = serve.Server(hbm_budget=512 << 20, tile=256, tpt=1, tick_interactions=5e6, adaptive_tick=False)
     try:
         others = []
         for k in range(6):
