@@ -309,3 +309,74 @@ def test_vcstyle_baseline_close_to_exact():
         c, gc = oracle.p2p_vcstyle(core, p.pos, p.str, p.rad, p.pos, p.rad, t0=100, t1=200)
         np.testing.assert_array_equal(c[:, 100:200], a[:, 100:200])
         assert np.abs(c[:, :100]).max() == 0
+
+
+# ---- the RK2 step restatement (oracle/rk2.py) ------------------------------------------
+
+@pytest.mark.parametrize("core", CORES)
+def test_rk2_corotating_pair(core):
+    """Two equal particles with strength a z_hat: by symmetry u_1 = -u_0 and
+    u_0 = K(|r|) a z_hat x r / (4 pi) with r = x_0 - x_1, so one step is Ralston's
+    method (stage at 2/3 h, weights 1/4 and 3/4) on this two-body ODE:
+        r1 = r0 + (4/3) h u_0(r0),   x_0 <- x_0 + h (u_0(r0)/4 + 3 u_0(r1)/4).
+    K comes from the pinned pair oracle.  The strength is not stretched
+    ((a . grad) u = 0 for a || z_hat; the self pair adds only K(0) skew(a)) and
+    the elongation stays 1.  A wrong stage point or weight fails this."""
+    from oracle import rk2
+    a, rho, sig, h = 0.8, 0.3, 0.1, 0.05
+    half = rho / 2
+    pos = np.array([[half, -half], [0.0, 0.0], [0.0, 0.0]])
+    st = np.array([[0.0, 0.0], [0.0, 0.0], [a, a]])
+    rad = np.array([sig, sig])
+
+    def u0(r):
+        """velocity of particle 0 for relative vector r (finalized: / 4 pi)"""
+        p2 = np.stack([r / 2, -r / 2], axis=1)
+        v, _ = oracle.p2p(core, p2, st, rad, p2, rad, False, "f64")
+        return v[:, 0] * 0.25 / np.pi
+
+    r0 = pos[:, 0] - pos[:, 1]
+    k1 = u0(r0)
+    r1 = r0 + (4.0 / 3.0) * h * k1
+    k2 = u0(r1)
+    want0 = pos[:, 0] + h * (0.25 * k1 + 0.75 * k2)
+    x, s, e = rk2.rk2_step(core, pos, st, rad, np.ones(2), h, (0.0, 0.0, 0.0), "f64")
+    np.testing.assert_allclose(x[:, 0], want0, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(x[:, 1], -want0, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(s, st, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(e, 1.0, rtol=0, atol=1e-14)
+    x32, s32, e32 = rk2.rk2_step(core, pos.astype(np.float32), st.astype(np.float32), rad.astype(np.float32),
+                                 np.ones(2, np.float32), h, (0.0, 0.0, 0.0), "f32d")
+    np.testing.assert_allclose(x32[:, 0], want0, rtol=0, atol=2e-7)
+
+
+@pytest.mark.parametrize("core", CORES)
+def test_rk2_stretching_sign(core):
+    """Stretching: a weak particle with alpha along z on the axis of a ring changes
+    its strength and elongation with the sign of dw/dz there, as (alpha . grad) u
+    prescribes (Points.h:376-423)."""
+    from oracle import rk2
+    ring = scenes.ring(400, R=1.0, a=0.01, sigma=0.05, seed=4)
+    probe_pos = np.array([[0.0], [0.0], [0.5]], np.float32)
+    pos = np.concatenate([ring.pos, probe_pos], axis=1)
+    st = np.concatenate([ring.str, np.array([[0.0], [0.0], [1e-6]], np.float32)], axis=1)
+    rad = np.concatenate([ring.rad, np.array([0.05], np.float32)])
+    x, s, e = rk2.rk2_step(core, pos, st, rad, np.ones(pos.shape[1]), 0.05, (0, 0, 0), "f64")
+    # on the axis of a ring the axial velocity decays away from the ring plane, so
+    # an axial filament there is stretched or compressed by the sign of dw/dz
+    vv, gg = oracle.p2p(core, pos, st, rad, probe_pos, rad[-1:], True, "f64")
+    dwdz = gg[8, 0]
+    grow = s[2, -1] - 1e-6
+    assert np.sign(grow) == np.sign(dwdz) and abs(grow) > 0
+    assert np.sign(e[-1] - 1.0) == np.sign(dwdz)
+
+
+def test_rk2_freestream_only():
+    """With zero strengths the step moves every particle by dt * U_inf exactly."""
+    from oracle import rk2
+    p = scenes.ring(50, seed=3)
+    st = np.zeros_like(p.str)
+    fs = (0.3, -0.1, 0.05)
+    x, s, e = rk2.rk2_step("rm", p.pos, st, p.rad, np.ones(p.n), 0.2, fs, "f64")
+    np.testing.assert_allclose(x, p.pos.astype(np.float64) + 0.2 * np.array(fs)[:, None], rtol=0, atol=1e-15)
+    assert np.all(s == 0) and np.all(e == 1)
