@@ -6,11 +6,16 @@ velocity gradient from every particle, inputs resident in HBM):
   tilted 15 deg about x, circulation +-1, |alpha| jittered +-5%), core radius 0.02,
   compact-exponential core, FP32 pairs with compensated accumulation.
 
-Prints ONE JSON line with G interactions/s (value), the plain-FP32 rate beside it,
-the roofline of the dominant kernel (FP32 VALU, repo flop count 68 per
-interaction + 12 per target), max/RMS relative error against the FP64 kernel on a
-target subset, step-latency percentiles and the CPU baseline (the oracle's
-Vc-style loop on this host's cores, on a bounded target sample).
+The evaluation uses the library's default method, which at this size is the
+factored one (Morton order; well-separated target-block / source-tile pairs
+summed through a per-source moment table on the matrix cores; the rest pair by
+pair).  Prints ONE JSON line with G interactions/s (value), the plain-FP32 rate
+beside it, the direct (pair-by-pair) method's rate, the roofline of the dominant
+kernel (the far-field matrix-core kernel: the reference's 68 flops per
+interaction times the pairs it takes, over its HIP-event time), max/RMS relative
+error against the FP64 kernel on a target subset, step-latency percentiles and the
+CPU baseline (the oracle's Vc-style loop on this host's cores, on a bounded
+target sample).
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-error]
 """
@@ -26,8 +31,8 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector peak
-KERNEL_NAME = "k_p2p<EXP,thick,grad,compensated>"
+FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 peak, vector and matrix alike
+FLOPS_PAIR = 68.0          # reference flop count of one EXP vel+grad interaction (Kernels.h)
 
 
 def parse():
@@ -42,6 +47,7 @@ def parse():
     ap.add_argument("--error-targets", type=int, default=16384)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--tpt", type=int, default=0)
+    ap.add_argument("--method", default="auto", choices=["auto", "direct", "factored"])
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run the known-traffic copy kernel on this many MiB (counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-instance batched workload")
@@ -88,8 +94,8 @@ def main():
     for k in range(9):
         tgt.ug[k] = dgrad.row(k)
 
-    def run(prec, steps, warmup):
-        o = _lib.make_opts("exp", True, prec, 0, args.tile, args.tpt)
+    def run(prec, steps, warmup, method=args.method):
+        o = _lib.make_opts("exp", True, prec, 0, args.tile, args.tpt, method=method)
         evs = [(_lib.Event(), _lib.Event()) for _ in range(steps)]
         for _ in range(warmup):
             dvel.zero(); dgrad.zero()
@@ -111,7 +117,7 @@ def main():
         for prec in ("comp", "fp32"):
             for tile in (128, 256, 512, 1024):
                 for tpt in (1, 2, 4):
-                    o = _lib.make_opts("exp", True, prec, 0, tile, tpt)
+                    o = _lib.make_opts("exp", True, prec, 0, tile, tpt, method="direct")
                     e0, e1 = _lib.Event(), _lib.Event()
                     _lib.check(L.o3d_points_affect_points_dev(C.byref(src), C.byref(tgt), C.byref(o), None))
                     e0.record()
@@ -141,15 +147,38 @@ def main():
     inter = float(N) * float(N)
     value = inter * args.steps / wall / 1e9
     kavg = kms.mean() / 1e3
-    flops = N * (12.0 + 68.0 * N)
-    achieved = flops / kavg / 1e12
-    log(f"bench: comp {value:.1f} G int/s, kernel avg {kms.mean():.1f} ms, {achieved:.2f} TFLOP/s")
+    flops = N * (12.0 + FLOPS_PAIR * N)
+    call_tflops = flops / kavg / 1e12
+    log(f"bench: comp {value:.1f} G int/s, call avg {kms.mean():.1f} ms, {call_tflops:.2f} TFLOP/s")
+
+    # stage times of the factored path, HIP events on the evaluation's stream
+    factored = args.method == "factored" or (args.method == "auto" and args.tile == 0 and args.tpt == 0)
+    prof = None
+    if factored:
+        o = _lib.make_opts("exp", True, "comp", 0, args.tile, 0, method="factored")
+        recs = []
+        for _ in range(max(1, min(args.steps, 3))):
+            dvel.zero(); dgrad.zero()
+            pr = _lib.FactoredProfile()
+            _lib.check(L.o3d_factored_profile_dev(C.byref(src), C.byref(tgt), C.byref(o), C.byref(pr)))
+            recs.append({k: getattr(pr, k) for k, _ in _lib.FactoredProfile._fields_})
+        prof = {k: float(np.mean([r[k] for r in recs])) for k in recs[0]}
+        prof["steps"] = len(recs)
+        log(f"bench: factored stages {prof}")
 
     psteps = max(1, min(args.steps, 3))
     pwall, pkms = run("fp32", psteps, 1)
     vel_p, grad_p = dvel.to_host(), dgrad.to_host()
     plain_value = inter * psteps / pwall / 1e9
     log(f"bench: plain fp32 {plain_value:.1f} G int/s, kernel avg {pkms.mean():.1f} ms")
+
+    direct = None
+    if factored:
+        dwall, dkms = run("comp", 1, 0, method="direct")
+        direct = {"value": round(inter / dwall / 1e9, 3), "unit": "G interactions/s",
+                  "kernel_ms": round(float(dkms.mean()), 3), "kernel": "k_p2p<EXP,thick,grad,compensated>",
+                  "tflops": round(flops / (dkms.mean() / 1e3) / 1e12, 3)}
+        log(f"bench: direct method {direct}")
 
     errors = None
     if not args.no_error:
@@ -206,9 +235,25 @@ def main():
     panels = None if args.no_panels else workloads.panels_run(log=log)
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "p2p_exp_n1m_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", "ff_far_exp_n1m_traffic.json" if factored else "p2p_exp_n1m_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("traffic_bytes")
+    if factored:
+        far_flops = FLOPS_PAIR * prof["far_pairs"]
+        achieved = far_flops / (prof["far_ms"] / 1e3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                "kernel": "k_ff_far<EXP,thick,compensated>", "kernel_ms": round(prof["far_ms"], 3),
+                "flops_per_launch": far_flops, "far_pair_fraction": round(prof["far_tile_fraction"], 5),
+                "stages_ms": {k: round(prof[k], 3) for k in ("prep_ms", "far_ms", "near_ms", "total_ms")},
+                "whole_call": {"achieved": round(call_tflops, 3), "frac": round(call_tflops / FP32_PEAK_TFLOPS, 4),
+                               "what": "reference flops of all N^2 pairs over the whole evaluation (sort, far, "
+                                       "near)"}}
+    else:
+        roof = {"bound": "valu", "achieved": round(call_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(call_tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                "kernel": "k_p2p<EXP,thick,grad,compensated>", "kernel_ms": round(float(kms.mean()), 3),
+                "flops_per_launch": flops}
 
     out = {
         "metric": "G interactions/s (vel+grad) at N=1M vs plain FP32; p50/p99 latency, mixed N",
@@ -225,7 +270,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "two colliding vortex rings, N=1,048,576 (524,288 each), compact-exponential core, "
                                "sigma 0.02, velocity + gradient at every particle, compensated FP32",
-                   "n": N, "core": "exp", "precision": "compensated", "self_term": "included"},
+                   "n": N, "core": "exp", "precision": "compensated", "self_term": "included",
+                   "method": "factored" if factored else "direct"},
         "plain_fp32": {"value": round(plain_value, 3), "unit": "G interactions/s",
                        "kernel_ms": round(float(pkms.mean()), 3)},
         "latency_ms": ({"p50": round(served["p50_s"] * 1e3, 3), "p99": round(served["p99_s"] * 1e3, 3),
@@ -233,10 +279,8 @@ def main():
                        {"p50": round(float(np.percentile(kms, 50)), 3),
                         "p99": round(float(np.percentile(kms, 99)), 3),
                         "what": "per-step kernel time of this workload"}),
-        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                     "kernel": KERNEL_NAME, "kernel_ms": round(float(kms.mean()), 3),
-                     "flops_per_launch": flops},
+        "roofline": roof,
+        "direct_method": direct,
         "errors_vs_fp64": errors,
         "cpu_baseline": cpu,
         "batched": batched,

@@ -53,6 +53,19 @@ enum o3d_precision {
 
 enum o3d_self { O3D_SELF_INCLUDE = 0, O3D_SELF_EXCLUDE = 1 };
 
+/* How the point->point sums are organised.  Both give the same direct sums over
+ * every pair; they differ in rounding only.
+ *   DIRECT:   one pair at a time on the vector ALUs, sources in index order.
+ *   FACTORED: particles in Morton order; a source tile well separated from a
+ *             target block is summed through an exact factorisation
+ *             K = Bp (|t'|^2 - 2 t'.s' + |s'|^2 + cores) as a product of the pair
+ *             weights with a 29-column per-source moment table on the matrix
+ *             cores; other pairs as in DIRECT.  RM and EXP cores, velocity and
+ *             gradient, FP32 or compensated precision; tile 64, 128 or 256.
+ *   AUTO:     FACTORED where it applies and both counts are >= 65536 with tile
+ *             and tpt left at 0, DIRECT otherwise. */
+enum o3d_method { O3D_METHOD_AUTO = 0, O3D_METHOD_DIRECT = 1, O3D_METHOD_FACTORED = 2 };
+
 enum o3d_status {
   O3D_OK = 0,
   O3D_EINVAL = 1,          /* bad argument (null pointer, bad enum, size mismatch)  */
@@ -103,7 +116,7 @@ typedef struct o3d_opts {
   int32_t tpt;         /* targets per thread (1, 2, 4); 0 = tuned                    */
   int32_t maxlev;      /* panels: recursion depth (reference RECURSIVE_LEVELS = 3);
                           negative selects the default                               */
-  int32_t reserved;
+  int32_t method;      /* enum o3d_method (point->point only)                        */
   void* stream;        /* hipStream_t for the *_dev calls; NULL = null stream        */
 } o3d_opts;
 
@@ -142,6 +155,24 @@ int o3d_points_affect_points_dev(const o3d_points* src, o3d_points* targ,
                                  const o3d_opts* opts, o3d_stats* stats);
 int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ,
                                  const o3d_opts* opts, o3d_stats* stats);
+
+/* Diagnostics of the factored method (no counterpart in the reference): runs one
+ * o3d_points_affect_points_dev evaluation with method FACTORED on device arrays,
+ * synchronises, and reports the time of its stages (preparation: box, Morton keys,
+ * sort, gathers, bounds, classification; far field on the matrix cores; near field
+ * and output) and the number of (target, source) pairs taken by the far field.
+ * Results are accumulated as by the plain call. */
+typedef struct o3d_factored_profile {
+  double prep_ms;
+  double far_ms;
+  double near_ms;
+  double total_ms;
+  double far_pairs;
+  double pairs;
+  double far_tile_fraction;  /* far_pairs / pairs */
+} o3d_factored_profile;
+int o3d_factored_profile_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* opts,
+                             o3d_factored_profile* out);
 
 /* Double-precision collections (the reference's Points<double>): always run the
  * FP64 kernel, whatever opts->precision says.  Host and device variants. */

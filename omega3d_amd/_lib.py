@@ -41,7 +41,7 @@ class Panels(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("core", C.c_int32), ("results", C.c_int32), ("precision", C.c_int32),
                 ("self_term", C.c_int32), ("tile", C.c_int32), ("tpt", C.c_int32),
-                ("maxlev", C.c_int32), ("reserved", C.c_int32), ("stream", C.c_void_p)]
+                ("maxlev", C.c_int32), ("method", C.c_int32), ("stream", C.c_void_p)]
 
 
 class Stats(C.Structure):
@@ -60,7 +60,7 @@ _int_fns = ["o3d_init", "o3d_abi_version", "o3d_points_affect_points", "o3d_poin
             "o3d_dev_malloc", "o3d_dev_free", "o3d_memcpy_htod", "o3d_memcpy_dtoh", "o3d_memset_dev",
             "o3d_device_synchronize", "o3d_stream_create", "o3d_stream_destroy", "o3d_stream_synchronize",
             "o3d_event_create", "o3d_event_destroy", "o3d_event_record", "o3d_event_elapsed_ms",
-            "o3d_device_info", "o3d_copy_probe"]
+            "o3d_device_info", "o3d_copy_probe", "o3d_factored_profile_dev"]
 for _n in _int_fns:
     if hasattr(lib, _n):
         getattr(lib, _n).restype = C.c_int
@@ -94,7 +94,16 @@ def last_error():
     return lib.o3d_last_error().decode(errors="replace")
 
 
-def make_opts(core="wl", grads=True, precision="comp", self_term=0, tile=0, tpt=0, maxlev=-1, stream=None):
+METHODS = {"auto": 0, "direct": 1, "factored": 2}
+
+
+class FactoredProfile(C.Structure):
+    _fields_ = [("prep_ms", C.c_double), ("far_ms", C.c_double), ("near_ms", C.c_double), ("total_ms", C.c_double),
+                ("far_pairs", C.c_double), ("pairs", C.c_double), ("far_tile_fraction", C.c_double)]
+
+
+def make_opts(core="wl", grads=True, precision="comp", self_term=0, tile=0, tpt=0, maxlev=-1, stream=None,
+              method="auto"):
     o = Opts()
     lib.o3d_default_opts(C.byref(o))
     o.core = CORES[core] if isinstance(core, str) else int(core)
@@ -104,6 +113,7 @@ def make_opts(core="wl", grads=True, precision="comp", self_term=0, tile=0, tpt=
     o.tile = int(tile)
     o.tpt = int(tpt)
     o.maxlev = int(maxlev)
+    o.method = METHODS[method] if isinstance(method, str) else int(method)
     o.stream = stream
     return o
 

@@ -1,0 +1,603 @@
+// ff_kernels.hpp -- factored far field for the point->point sums with velocity
+// gradient, Rosenhead-Moore and compact-exponential cores (CoreFunc.h:65-71,
+// 225-236), on the gfx950 matrix cores.
+//
+// The sums are the same direct sums over every pair (Kernels.h:155-193); only
+// their arithmetic is reorganised for pairs of a target block and a source tile
+// that are well separated.  Relative to the block centre c, with t' = t - c,
+// s' = s - c, d = t' - s', m = alpha x s', and
+//   RM:          r2 = |d|^2 + ss^2 + st^2,   K = r2^-3/2,  Bp = r2^-5/2
+//   EXP (q>16):  r2 = |d|^2,                 K = r2^-3/2,  Bp = r2^-5/2
+// every pair has K = Bp r2 = Bp (a - 2 t'.s' + b) with a = |t'|^2 + st^2 and
+// b = |s'|^2 + ss^2 (no core terms for EXP).  Hence, summing over the tile,
+//   W   = sum K alpha     = a P - 2 t'_l R_l + U
+//   M   = sum K m         = a Q - 2 t'_l S_l + V
+//   u   = sum K alpha x d = W x t' - M
+//   D_ji = sum Bp d_j (alpha x d)_i = t'_j (P x t')_i - t'_j Q_i - (R_j x t')_i + S_ji
+// with 29 moment sums, all of the form sum_k Bp_k f(source k):
+//   P = sum Bp alpha (3), Q = sum Bp m (3), R_lk = sum Bp s'_l alpha_k (9),
+//   S_lk = sum Bp s'_l m_k (8: s'.m = 0 so S_22 = -S_00 - S_11),
+//   U = sum Bp b alpha (3), V = sum Bp b m (3).
+// That is a product of the pair matrix Bp (sources x targets) with a per-source
+// moment table (29 columns padded to 32): one v_mfma_f32_32x32x2f32 per two
+// sources and 32 targets, leaving about ten VALU operations per pair (d, r2,
+// one v_rsq_f32 and three products) instead of about thirty-three.
+//
+// Separation: a tile is "far" for a block when the gap between their bounding
+// spheres is at least 2 block radii (then every term above is at most (1+2/2)^2
+// = 4 times the pair's own contribution, so FP32 sums keep their accuracy) and,
+// for EXP, when that gap guarantees q = rho^3/(ss^3+st^3) > 16 for every pair,
+// i.e. the reference's far branch (CoreFunc.h:121,169).  Other tiles go through
+// the direct per-pair code in original coordinates, so near pairs keep exact
+// differences t - s.
+//
+// Particles are put in Morton order first (keys on a 10-bit grid per axis) so
+// that blocks and tiles are compact; the decision per (block, tile) is computed
+// once into a bit mask read by both kernels.
+#pragma once
+#include "p2p_kernels.hpp"
+
+namespace o3d {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int FF_TB = 256;          // targets per workgroup: 4 waves x 64
+constexpr int FF_LDM = 33;          // LDS row stride of the moment table (floats)
+constexpr float FF_KAPPA = 2.0f;    // minimum gap, in block radii
+constexpr int FF_FOLD = 16;         // near field, compensated: sources per FP32 partial
+constexpr int FF_M0 = 16;           // near field: fewest tiles per work item
+
+struct FFArgs {
+  const float4* sp;    // sorted sources {x, y, z, core_term}
+  const float4* sa;    // sorted sources {ax, ay, az, 0}
+  const float4* tq;    // sorted targets {x, y, z, core_term or 0 for point targets}
+  const int* tperm;    // sorted target -> original target index
+  const float4* tsph;  // per source tile: bounding sphere {cx, cy, cz, r}
+  const float* tcs;    // per source tile: largest core term
+  const float4* bsph;  // per target block: bounding sphere
+  const float* bcs;    // per target block: largest core term
+  const unsigned* mask;  // far bit per (block, tile), mwords words per block
+  double* farv;        // 12 x nt, sorted target order: far-field velocity and gradient
+  const int* nn;       // near tiles per block
+  const int* items;    // near-field work items per block
+  const int* ioff;     // exclusive scan of items
+  const int* mfold;    // near tiles per item
+  unsigned* ctr;       // work-queue counter
+  double* part;        // per item: 12 x 256 partial sums
+  SrcView s;           // original sources (self-pair exclusion)
+  TgtView t;           // original targets (outputs, accumulated)
+  int64_t ns, nt;
+  int32_t tile;        // sources per tile (multiple of 32, <= 512)
+  int32_t mwords;
+  int32_t ntiles;
+  int32_t self_excl;
+};
+
+// ---- order-preserving float <-> uint (for atomic min / max of floats) ---------
+__device__ __forceinline__ unsigned fenc(float f) {
+  const unsigned b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float fdec(unsigned e) {
+  return __uint_as_float((e & 0x80000000u) ? (e & 0x7fffffffu) : ~e);
+}
+
+__global__ void k_ff_bbox_init(unsigned* bb) {
+  if (threadIdx.x < 3) bb[threadIdx.x] = 0xffffffffu;
+  else if (threadIdx.x < 6) bb[threadIdx.x] = 0u;
+}
+
+// bounding box of one or two point sets (x, y, z SoA) into bb[0..5] (encoded)
+__global__ __launch_bounds__(256) void k_ff_bbox(const float* x, const float* y, const float* z, int64_t n,
+                                                 unsigned* bb) {
+  float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
+  float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float v[3] = {x[i], y[i], z[i]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) { lo[d] = fminf(lo[d], v[d]); hi[d] = fmaxf(hi[d], v[d]); }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = fminf(lo[d], __shfl_xor(lo[d], off));
+      hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off));
+    }
+  }
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&bb[d], fenc(lo[d]));
+      atomicMax(&bb[3 + d], fenc(hi[d]));
+    }
+  }
+}
+
+__device__ __forceinline__ unsigned spread10(unsigned v) {
+  v &= 1023u;
+  v = (v | (v << 16)) & 0x030000FFu;
+  v = (v | (v << 8)) & 0x0300F00Fu;
+  v = (v | (v << 4)) & 0x030C30C3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+// 30-bit Morton keys on the common bounding box, and the identity permutation
+__global__ __launch_bounds__(256) void k_ff_keys(const float* x, const float* y, const float* z, int64_t n,
+                                                 const unsigned* bb, unsigned* keys, int* idx) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float lx = fdec(bb[0]), ly = fdec(bb[1]), lz = fdec(bb[2]);
+  const float ext = fmaxf(fmaxf(fdec(bb[3]) - lx, fdec(bb[4]) - ly), fdec(bb[5]) - lz);
+  const float sc = ext > 0.0f ? 1023.0f / ext : 0.0f;
+  const float q[3] = {(x[i] - lx) * sc, (y[i] - ly) * sc, (z[i] - lz) * sc};
+  unsigned k = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float c = fminf(fmaxf(q[d], 0.0f), 1023.0f);   // NaN -> 0
+    k |= spread10((unsigned)c) << d;
+  }
+  keys[i] = k;
+  idx[i] = (int)i;
+}
+
+template <int CORE>
+__global__ __launch_bounds__(256) void k_ff_gather_src(SrcView s, const int* perm, int64_t n, float4* sp,
+                                                       float4* sa) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  sp[i] = make_float4(s.x[j], s.y[j], s.z[j], core_term<CORE>(s.r[j]));
+  sa[i] = make_float4(s.ax[j], s.ay[j], s.az[j], 0.0f);
+}
+
+template <int CORE, bool THICK>
+__global__ __launch_bounds__(256) void k_ff_gather_tgt(TgtView t, const int* perm, int64_t n, float4* tq) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int j = perm[i];
+  tq[i] = make_float4(t.x[j], t.y[j], t.z[j], THICK ? core_term<CORE>(t.r[j]) : 0.0f);
+}
+
+// workgroup-wide reductions over 256 threads (4 waves), result in every thread
+__device__ __forceinline__ float wg_max(float v, float* red) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+}
+__device__ __forceinline__ float wg_min(float v, float* red) { return -wg_max(-v, red); }
+
+// bounding sphere (bbox centre, largest distance) and largest core term of each
+// group of `group` consecutive sorted points; one workgroup per group
+__global__ __launch_bounds__(256) void k_ff_bounds(const float4* p, int64_t n, int group, float4* sph, float* cs) {
+  __shared__ float red[4];
+  const int64_t b0 = (int64_t)blockIdx.x * group;
+  const int64_t b1 = (b0 + group) < n ? (b0 + group) : n;
+  float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
+  float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
+  float cmax = 0.0f;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const float4 q = p[i];
+    lo[0] = fminf(lo[0], q.x); hi[0] = fmaxf(hi[0], q.x);
+    lo[1] = fminf(lo[1], q.y); hi[1] = fmaxf(hi[1], q.y);
+    lo[2] = fminf(lo[2], q.z); hi[2] = fmaxf(hi[2], q.z);
+    cmax = fmaxf(cmax, q.w);
+  }
+  float c[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const float l = wg_min(lo[d], red);
+    const float h = wg_max(hi[d], red);
+    c[d] = 0.5f * l + 0.5f * h;
+  }
+  cmax = wg_max(cmax, red);
+  float r2 = 0.0f;
+  for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+    const float4 q = p[i];
+    const float dx = q.x - c[0], dy = q.y - c[1], dz = q.z - c[2];
+    r2 = fmaxf(r2, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
+  }
+  r2 = wg_max(r2, red);
+  if (threadIdx.x == 0) {
+    // margin for the rounding of the distances above
+    sph[blockIdx.x] = make_float4(c[0], c[1], c[2], __fsqrt_rn(r2) * 1.00001f + 1e-30f);
+    cs[blockIdx.x] = cmax;
+  }
+}
+
+// far bit of every (target block, source tile): one thread per 32 tiles of a block
+template <int CORE>
+__global__ __launch_bounds__(256) void k_ff_classify(const float4* bsph, const float* bcs, int64_t nb,
+                                                     const float4* tsph, const float* tcs, int ntiles, int mwords,
+                                                     unsigned* mask) {
+  const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nb * mwords) return;
+  const int64_t b = g / mwords;
+  const int w = (int)(g - b * mwords);
+  const float4 cb = bsph[b];
+  const float csb = bcs[b];
+  unsigned bits = 0u;
+  for (int k = 0; k < 32; ++k) {
+    const int ti = w * 32 + k;
+    if (ti >= ntiles) break;
+    const float4 ct = tsph[ti];
+    const float dx = cb.x - ct.x, dy = cb.y - ct.y, dz = cb.z - ct.z;
+    const float gap = __fsqrt_rn(fmaf(dz, dz, fmaf(dy, dy, dx * dx))) - cb.w - ct.w;
+    bool far = gap > 0.0f && gap > FF_KAPPA * cb.w;
+    if (CORE == CORE_EXP) far = far && gap * gap > exp_threshold(csb + tcs[ti]);
+    if (far) bits |= 1u << k;
+  }
+  mask[g] = bits;
+}
+
+// pair weight Bp = r2^-5/2 of a far pair (t' and s' relative to the block centre)
+template <int CORE>
+__device__ __forceinline__ float bp_far(float tx, float ty, float tz, float tt, const float4 P) {
+  const float dx = tx - P.x, dy = ty - P.y, dz = tz - P.z;
+  const float base = CORE == CORE_RM ? tt + P.w : 0.0f;
+  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, base)));
+  const float ir = frsq(r2);
+  const float ir2 = ir * ir;
+  return ir2 * ir2 * ir;
+}
+
+// velocity and gradient (ug order, g = -3 D + skew(W)) of one target from its 32
+// moment sums c[] and its offset t from the block centre; al = |t|^2 + st^2 (RM)
+__device__ __forceinline__ void ff_epilogue(const double* c, double tx, double ty, double tz, double al,
+                                            double* out) {
+  const double P[3] = {c[0], c[1], c[2]};
+  const double Q[3] = {c[3], c[4], c[5]};
+  double R[3][3], S[3][3];
+#pragma unroll
+  for (int l = 0; l < 3; ++l)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) R[l][k] = c[6 + 3 * l + k];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) S[e / 3][e % 3] = c[15 + e];
+  S[2][2] = -S[0][0] - S[1][1];
+  const double U[3] = {c[23], c[24], c[25]};
+  const double V[3] = {c[26], c[27], c[28]};
+  const double t[3] = {tx, ty, tz};
+  double W[3], M[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    W[k] = al * P[k] - 2.0 * (t[0] * R[0][k] + t[1] * R[1][k] + t[2] * R[2][k]) + U[k];
+    M[k] = al * Q[k] - 2.0 * (t[0] * S[0][k] + t[1] * S[1][k] + t[2] * S[2][k]) + V[k];
+  }
+  out[0] = W[1] * t[2] - W[2] * t[1] - M[0];
+  out[1] = W[2] * t[0] - W[0] * t[2] - M[1];
+  out[2] = W[0] * t[1] - W[1] * t[0] - M[2];
+  const double Px[3] = {P[1] * t[2] - P[2] * t[1], P[2] * t[0] - P[0] * t[2], P[0] * t[1] - P[1] * t[0]};
+  double a[NACC];   // a[3 + 3j + i] = D_ji for 3j + i < 8 (D_22 = -D_00 - D_11)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const double Rx[3] = {R[j][1] * t[2] - R[j][2] * t[1], R[j][2] * t[0] - R[j][0] * t[2],
+                          R[j][0] * t[1] - R[j][1] * t[0]};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      if (3 * j + i < 8) a[3 + 3 * j + i] = t[j] * Px[i] - t[j] * Q[i] - Rx[i] + S[j][i];
+  }
+  a[11] = W[0]; a[12] = W[1]; a[13] = W[2];
+  grad_from<double>(a, out + 3);
+}
+
+// Far field: one workgroup per 256 sorted targets; lane l of wave w owns target
+// w*64 + l at the end, and during the sums computes the pair weights of targets
+// w*64 + (l & 31) and w*64 + 32 + (l & 31) with source k0 + (l >> 5).
+//
+// v_mfma_f32_32x32x2f32 operand layout (A: 32 x 2, B: 2 x 32, D: 32 x 32):
+//   A[i][k] in lane i + 32k, B[k][j] in lane j + 32k,
+//   D[i][j] in lane j + 32 ((i / 4) % 2), register (i % 4) + 4 (i / 8).
+// A = moment table transposed (column i of source k), B = Bp (source k, target j),
+// so D[i][j] = sum over the two sources of moment i x Bp for target j.
+template <int CORE, bool THICK, int PREC>
+__global__ __launch_bounds__(256) void k_ff_far(FFArgs a) {
+  extern __shared__ float4 sh[];
+  const int T = a.tile;
+  float4* pos = sh;
+  float* mom = reinterpret_cast<float*>(sh + T);
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t blk = blockIdx.x;
+  const int64_t base = blk * FF_TB + w * 64;
+  const float4 cb = a.bsph[blk];
+
+  float tx[2], ty[2], tz[2], tt[2];
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    int64_t i = base + 32 * s + (lane & 31);
+    i = i < a.nt ? i : a.nt - 1;
+    const float4 q = a.tq[i];
+    tx[s] = q.x - cb.x; ty[s] = q.y - cb.y; tz[s] = q.z - cb.z;
+    tt[s] = CORE == CORE_RM ? q.w : 0.0f;
+  }
+  f32x16 acc0, acc1;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) { acc0[r] = 0.0f; acc1[r] = 0.0f; }
+  double raw0[16], raw1[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) { raw0[r] = 0.0; raw1[r] = 0.0; }
+
+  const unsigned* mrow = a.mask + blk * a.mwords;
+  for (int ti = 0; ti < a.ntiles; ++ti) {
+    if (!((mrow[ti >> 5] >> (ti & 31)) & 1u)) continue;
+    const int64_t s0 = (int64_t)ti * T;
+    const int cnt = (int)((a.ns - s0) < T ? (a.ns - s0) : T);
+    __syncthreads();
+    for (int j = tid; j < T; j += FF_TB) {
+      float* row = mom + j * FF_LDM;
+      if (j < cnt) {
+        const float4 P = a.sp[s0 + j];
+        const float4 A = a.sa[s0 + j];
+        const float sx = P.x - cb.x, sy = P.y - cb.y, sz = P.z - cb.z;
+        const float cw = CORE == CORE_RM ? P.w : 0.0f;
+        const float m0 = fmaf(A.y, sz, -A.z * sy);
+        const float m1 = fmaf(A.z, sx, -A.x * sz);
+        const float m2 = fmaf(A.x, sy, -A.y * sx);
+        const float b = fmaf(sz, sz, fmaf(sy, sy, fmaf(sx, sx, cw)));
+        pos[j] = make_float4(sx, sy, sz, cw);
+        const float sv[3] = {sx, sy, sz}, av[3] = {A.x, A.y, A.z}, mv[3] = {m0, m1, m2};
+        row[0] = A.x; row[1] = A.y; row[2] = A.z;
+        row[3] = m0; row[4] = m1; row[5] = m2;
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+#pragma unroll
+          for (int k = 0; k < 3; ++k) row[6 + 3 * l + k] = sv[l] * av[k];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) row[15 + e] = sv[e / 3] * mv[e % 3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { row[23 + k] = b * av[k]; row[26 + k] = b * mv[k]; }
+        row[29] = 0.0f; row[30] = 0.0f; row[31] = 0.0f;
+      } else {
+        // padding: far away with zero moments, so that Bp * moment = 0
+        pos[j] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+#pragma unroll
+        for (int c = 0; c < 32; ++c) row[c] = 0.0f;
+      }
+    }
+    __syncthreads();
+    const int half = lane >> 5, col = lane & 31;
+#pragma unroll 4
+    for (int k = 0; k < T; k += 2) {
+      const int ks = k + half;
+      const float4 P = pos[ks];
+      const float mv = mom[ks * FF_LDM + col];
+      const float b0 = bp_far<CORE>(tx[0], ty[0], tz[0], tt[0], P);
+      const float b1 = bp_far<CORE>(tx[1], ty[1], tz[1], tt[1], P);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(mv, b0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(mv, b1, acc1, 0, 0, 0);
+    }
+    if (PREC == PREC_COMP) {
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        raw0[r] += (double)acc0[r]; acc0[r] = 0.0f;
+        raw1[r] += (double)acc1[r]; acc1[r] = 0.0f;
+      }
+    }
+  }
+  if (PREC != PREC_COMP) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) { raw0[r] = (double)acc0[r]; raw1[r] = (double)acc1[r]; }
+  }
+
+  // lane l < 32 owns target w*64 + l (set 0), lane l >= 32 target w*64 + l (set 1);
+  // each holds the rows of its half (l >> 5) and gets the other half from lane l ^ 32
+  const int h = lane >> 5;
+  double c[32];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const double keep = h ? raw1[r] : raw0[r];
+    const double give = h ? raw0[r] : raw1[r];
+    const double got = __shfl_xor(give, 32);
+    const int i0 = 8 * (r / 4) + (r % 4);       // row held by half 0
+    c[i0] = h ? got : keep;
+    c[i0 + 4] = h ? keep : got;
+  }
+  const int64_t i = base + lane;
+  if (i < a.nt) {
+    const double dx = h ? tx[1] : tx[0], dy = h ? ty[1] : ty[0], dz = h ? tz[1] : tz[0];
+    const double al = dx * dx + dy * dy + dz * dz + (double)(h ? tt[1] : tt[0]);
+    double out[12];
+    ff_epilogue(c, dx, dy, dz, al, out);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) a.farv[(int64_t)k * a.nt + i] = out[k];
+  }
+}
+
+// ---- near field -------------------------------------------------------------
+// The near tiles of a block are split into items of at most M tiles (M chosen on
+// the device so that there are at most about 2 items per block), so that a block
+// whose targets straddle a jump of the Morton order, and which therefore has
+// nearly every tile near, is spread over many workgroups.  Items are taken from a
+// queue by a persistent kernel and write FP64 partial sums; the combine kernel
+// adds a block's partials in item order, so results do not depend on scheduling.
+
+__device__ __forceinline__ int near_tiles_of(const unsigned* mrow, int mwords, int ntiles) {
+  int far = 0;
+  for (int w = 0; w < mwords; ++w) far += __popc(mrow[w]);
+  return ntiles - far;
+}
+
+// nn[b] = number of near tiles of block b
+__global__ __launch_bounds__(256) void k_ff_plan_count(const unsigned* mask, int64_t nb, int mwords, int ntiles,
+                                                       int* nn) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nb) nn[b] = near_tiles_of(mask + b * mwords, mwords, ntiles);
+}
+
+// after the exclusive scan noff of nn: M = max(FF_M0, ceil(total / nb)) and the
+// item count of every block
+__global__ __launch_bounds__(256) void k_ff_plan_items(const int* nn, const int* noff, int64_t nb, int* items,
+                                                       int* mfold) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t total = (int64_t)noff[nb - 1] + nn[nb - 1];
+  int64_t m = (total + nb - 1) / nb;
+  m = m < FF_M0 ? FF_M0 : m;
+  if (b == 0) *mfold = (int)m;
+  if (b < nb) items[b] = (int)((nn[b] + m - 1) / m);
+}
+
+__global__ void k_ff_zero_u32(unsigned* p) {
+  if (threadIdx.x == 0) *p = 0u;
+}
+
+// per-target partial of one item: near tiles of rank [r0, r1) among block b's
+template <int CORE, int PREC>
+__device__ __forceinline__ void near_item(const FFArgs& a, int64_t b, int r0, int r1, float4* sh, double* r) {
+  const int T = a.tile;
+  const int tid = threadIdx.x;
+  const int64_t i = b * FF_TB + tid;
+  const int64_t ic = i < a.nt ? i : a.nt - 1;
+  const float4 q = a.tq[ic];
+  float tx[1] = {q.x}, ty[1] = {q.y}, tz[1] = {q.z}, tt[1] = {q.w};
+  float acc[1][NACC];
+  float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
+  double gd[9];
+#pragma unroll
+  for (int c = 0; c < NACC; ++c) acc[0][c] = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) gd[c] = 0.0;
+
+  const unsigned* mrow = a.mask + b * a.mwords;
+  int rank = 0;
+  for (int w = 0; w < a.mwords && rank < r1; ++w) {
+    const int base = w * 32;
+    const unsigned valid = (a.ntiles - base) >= 32 ? 0xffffffffu : ((1u << (a.ntiles - base)) - 1u);
+    unsigned near = ~mrow[w] & valid;
+    const int c = __popc(near);
+    if (rank + c <= r0) { rank += c; continue; }
+    while (near && rank < r1) {
+      const int ti = base + __builtin_ctz(near);
+      near &= near - 1u;
+      if (rank++ < r0) continue;
+      const int64_t s0 = (int64_t)ti * T;
+      const int cnt = (int)((a.ns - s0) < T ? (a.ns - s0) : T);
+      __syncthreads();
+      for (int j = tid; j < cnt; j += FF_TB) {
+        sh[2 * j] = a.sp[s0 + j];
+        sh[2 * j + 1] = a.sa[s0 + j];
+      }
+      __syncthreads();
+      float thr[1] = {CORE == CORE_EXP ? exp_threshold(a.tcs[ti] + tt[0]) : 0.0f};
+      if (PREC == PREC_COMP) {
+        // In Morton order a near tile holds the target's closest sources, whose large
+        // terms cancel; FP32 partials are therefore folded every FF_FOLD sources.
+        for (int j0 = 0; j0 < cnt; j0 += FF_FOLD) {
+          const int j1 = j0 + FF_FOLD < cnt ? j0 + FF_FOLD : cnt;
+          for (int j = j0; j < j1; ++j)
+            source_acc<CORE, true, 1>(tx, ty, tz, tt, thr, sh[2 * j], sh[2 * j + 1], acc);
+          fold_tile_comp(acc[0], hi, lo, gd, true);
+        }
+      } else {
+        for (int j = 0; j < cnt; ++j)
+          source_acc<CORE, true, 1>(tx, ty, tz, tt, thr, sh[2 * j], sh[2 * j + 1], acc);
+      }
+    }
+  }
+  if (PREC == PREC_COMP) {
+    r[0] = (double)hi[0] + (double)lo[0];
+    r[1] = (double)hi[1] + (double)lo[1];
+    r[2] = (double)hi[2] + (double)lo[2];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) r[3 + c] = gd[c];
+  } else {
+    float g[9];
+    grad_from<float>(acc[0], g);
+    r[0] = acc[0][0]; r[1] = acc[0][1]; r[2] = acc[0][2];
+#pragma unroll
+    for (int c = 0; c < 9; ++c) r[3 + c] = g[c];
+  }
+}
+
+// persistent: every workgroup takes items from the queue until none is left
+template <int CORE, int PREC>
+__global__ __launch_bounds__(256) void k_ff_near(FFArgs a) {
+  extern __shared__ float4 sh[];
+  __shared__ int s_item;
+  const int64_t nb = (a.nt + FF_TB - 1) / FF_TB;
+  const int total = a.ioff[nb - 1] + a.items[nb - 1];
+  const int m = *a.mfold;
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_item = (int)atomicAdd(a.ctr, 1u);
+    __syncthreads();
+    const int item = s_item;
+    if (item >= total) break;
+    // block b: the last with ioff[b] <= item (blocks without items share the offset
+    // of the next block, so the last of a run of equal offsets owns the items)
+    int64_t lo = 0, hi = nb;
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) / 2;
+      if (a.ioff[mid] <= item) lo = mid; else hi = mid;
+    }
+    const int64_t b = lo;
+    const int k = item - a.ioff[b];
+    const int r0 = k * m;
+    const int nnb = a.nn[b];
+    const int r1 = r0 + m < nnb ? r0 + m : nnb;
+    double r[12];
+    near_item<CORE, PREC>(a, b, r0, r1, sh, r);
+    double* slot = a.part + (int64_t)item * 12 * FF_TB;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) slot[c * FF_TB + threadIdx.x] = r[c];
+  }
+}
+
+// one workgroup per block: near partials in item order, far part, self-pair
+// exclusion, then accumulate into the caller's arrays at the original index
+template <int CORE, int PREC>
+__global__ __launch_bounds__(256) void k_ff_combine(FFArgs a) {
+  const int64_t b = blockIdx.x;
+  const int64_t i = b * FF_TB + threadIdx.x;
+  if (i >= a.nt) return;
+  double r[12];
+#pragma unroll
+  for (int c = 0; c < 12; ++c) r[c] = a.farv[(int64_t)c * a.nt + i];
+  const int n_it = a.items[b], off = a.ioff[b];
+  for (int k = 0; k < n_it; ++k) {
+    const double* slot = a.part + (int64_t)(off + k) * 12 * FF_TB;
+#pragma unroll
+    for (int c = 0; c < 12; ++c) r[c] += slot[c * FF_TB + threadIdx.x];
+  }
+  const int io = a.tperm[i];
+  if (a.self_excl) {
+    // the self pair entered only W (velocity and dyad vanish at d = 0)
+    float ks, Bp;
+    kb<CORE>(0.0f, core_term<CORE>(a.s.r[io]) + a.tq[i].w, 0.0f, ks, Bp);
+    const double kx = (double)ks * a.s.ax[io], ky = (double)ks * a.s.ay[io], kz = (double)ks * a.s.az[io];
+    r[4] -= kz; r[5] += ky; r[6] += kz; r[8] -= kx; r[9] -= ky; r[10] += kx;
+  }
+  a.t.u[io] = (float)((double)a.t.u[io] + r[0]);
+  a.t.v[io] = (float)((double)a.t.v[io] + r[1]);
+  a.t.w[io] = (float)((double)a.t.w[io] + r[2]);
+#pragma unroll
+  for (int c = 0; c < 9; ++c) a.t.g[c][io] = (float)((double)a.t.g[c][io] + r[3 + c]);
+}
+
+// number of (target, source) pairs the far-field kernel takes: one thread per block
+__global__ __launch_bounds__(256) void k_ff_count(const unsigned* mask, int64_t nb, int mwords, int ntiles, int T,
+                                                  int64_t ns, int64_t nt, unsigned long long* out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int64_t bt = (nt - b * FF_TB) < FF_TB ? (nt - b * FF_TB) : FF_TB;
+  unsigned long long sum = 0;
+  for (int w = 0; w < mwords; ++w) {
+    unsigned bits = mask[b * mwords + w];
+    while (bits) {
+      const int ti = w * 32 + __builtin_ctz(bits);
+      bits &= bits - 1u;
+      const int64_t cnt = (ns - (int64_t)ti * T) < T ? (ns - (int64_t)ti * T) : T;
+      if (ti < ntiles) sum += (unsigned long long)(cnt * bt);
+    }
+  }
+  atomicAdd(out, sum);
+}
+
+__global__ void k_ff_zero_u64(unsigned long long* p) {
+  if (threadIdx.x == 0) *p = 0ull;
+}
+
+}  // namespace o3d

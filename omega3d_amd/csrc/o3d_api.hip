@@ -108,15 +108,15 @@ static int pool_get(int slot, size_t bytes, void** out) {
 // Targets per thread: enough workgroups to cover every CU several times, more
 // register reuse of each staged source when the target count allows it.
 void choose_config(const Tuning& req, int64_t nt, int64_t ns, bool grad, int prec, int* tile, int* tpt) {
-  int tl = req.tile > 0 ? req.tile : 256;
+  // measured at N = 1M (velocity + gradient, both FP32 modes): tile 512 with one
+  // target per lane is fastest; more targets per lane cost occupancy
+  int tl = req.tile > 0 ? req.tile : (ns >= 65536 ? 512 : 256);
   int tp = req.tpt;
   if (tp <= 0) {
     const int64_t cus = num_cus();
     tp = 1;
-    if (prec != PREC_FP64 && nt >= (int64_t)BLOCK * 2 * cus * 4) tp = 2;
-    if (!grad && nt >= (int64_t)BLOCK * 4 * cus * 4) tp = 4;
+    if (!grad && prec != PREC_FP64 && nt >= (int64_t)BLOCK * 4 * cus * 4) tp = 4;
   }
-  (void)ns;
   *tile = tl;
   *tpt = tp;
 }
@@ -151,6 +151,10 @@ int check_opts(const o3d_opts* o) {
   if (o->tile != 0 && (o->tile < 64 || o->tile > 1024 || o->tile % 64 != 0))
     return fail(O3D_EINVAL, "tile %d: must be 0 or a multiple of 64 in [64, 1024]", o->tile);
   if (o->tpt != 0 && o->tpt != 1 && o->tpt != 2 && o->tpt != 4) return fail(O3D_EINVAL, "tpt %d: must be 0, 1, 2 or 4", o->tpt);
+  if (o->method < O3D_METHOD_AUTO || o->method > O3D_METHOD_FACTORED)
+    return fail(O3D_EINVAL, "unknown method %d", o->method);
+  if (o->method == O3D_METHOD_FACTORED && o->precision == PREC_FP64)
+    return fail(O3D_EINVAL, "the factored method computes in FP32 (plain or compensated), not FP64");
   return O3D_OK;
 }
 
@@ -197,6 +201,10 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   const bool thick = targ->r != nullptr;
   if (src->n == 0 || targ->n == 0) return O3D_OK;
   if (o->precision == PREC_FP64) return fail(O3D_EINVAL, "internal: FP64 routed to p2p_dev");
+  if (o->method == O3D_METHOD_FACTORED) return p2p_factored(src, targ, o, st);
+  if (o->method == O3D_METHOD_AUTO && !tun && o->tile == 0 && o->tpt == 0 && ff_supported(o->core, grad, o->precision) &&
+      src->n >= 65536 && targ->n >= 65536 && src->n <= INT32_MAX && targ->n <= INT32_MAX)
+    return p2p_factored(src, targ, o, st);
   P2PArgs a{};
   a.s = SrcView{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
   a.t.x = targ->x[0]; a.t.y = targ->x[1]; a.t.z = targ->x[2]; a.t.r = targ->r;
@@ -407,6 +415,21 @@ int o3d_points_affect_points_dev(const o3d_points* src, o3d_points* targ, const 
   if (rc != O3D_OK) return rc;
   fill_stats(stats, 0.0, flops_of(o->core, grad, targ->r != nullptr, targ->n, src->n), (double)targ->n * (double)src->n);
   return O3D_OK;
+}
+
+int o3d_factored_profile_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, o3d_factored_profile* out) {
+  clear_error();
+  int rc;
+  if (!out) return fail(O3D_EINVAL, "out is NULL");
+  *out = o3d_factored_profile{};
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  const bool grad = o->results == O3D_VELANDGRAD;
+  if ((rc = check_points(src, true, false, grad, "src")) != O3D_OK) return rc;
+  if ((rc = check_points(targ, false, true, grad, "targ")) != O3D_OK) return rc;
+  if ((rc = check_self(src, targ, o)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  if (src->n == 0 || targ->n == 0) return O3D_OK;
+  return p2p_factored(src, targ, o, (hipStream_t)o->stream, out);
 }
 
 int o3d_points_affect_points64(const o3d_points64* src, o3d_points64* targ, const o3d_opts* o, o3d_stats* stats) {

@@ -27,6 +27,9 @@ hipError_t dispatch_p2p(int core, const P2PArgs& a, bool thick, bool grad, int p
 double flops_of(int core, bool grad, bool thick, int64_t nt, int64_t ns);
 int check_opts(const o3d_opts* o);
 void fill_stats(o3d_stats* stats, double secs, double flops, double inter);
+bool ff_supported(int core, bool grad, int prec);
+int p2p_factored(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st,
+                 o3d_factored_profile* prof = nullptr);
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);
 
 }  // namespace o3d

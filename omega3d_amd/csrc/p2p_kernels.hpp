@@ -82,11 +82,14 @@ __device__ __forceinline__ void exp_any(float d2, float cs, float& K, float& Bp)
   const float e = fexp2(q * -1.44269504088896341f);
   const float kg = fmaf(-ood3, e, ood3);
   const float bg = fmaf(-c, e, kg) * ir2;
-  const float ks = c * horner_f(q);
-  const float bs = c * (c * dist) * horner_h(q);
   const bool small = q < 1.0f;
-  float kn = small ? ks : kg;
-  float bn = small ? bs : bg;
+  float kn = kg, bn = bg;
+  if (wave_any(small)) {          // the series only where some lane needs them
+    const float ks = c * horner_f(q);
+    const float bs = c * (c * dist) * horner_h(q);
+    kn = small ? ks : kg;
+    bn = small ? bs : bg;
+  }
   const bool near = q < 0.001f;
   kn = near ? c : kn;
   bn = near ? (0.5f * dist) * c * c : bn;

@@ -126,7 +126,7 @@ def _c64(a):
 
 
 def p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, precision="comp", self_term=0,
-        tile=0, tpt=0, vel=None, grad=None, stats=False):
+        tile=0, tpt=0, vel=None, grad=None, stats=False, method="auto"):
     """Host-array point->point sum on the GPU.  Accumulates into vel/grad (new zero
     arrays when not given) and returns them (and the o3d_stats when stats=True)."""
     f = _c64 if precision == "fp64" else _c32
@@ -135,7 +135,7 @@ def p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, precision="comp", s
     nt = tpos.shape[1]
     vel = np.zeros((3, nt), dt) if vel is None else vel
     grad = (np.zeros((9, nt), dt) if grad is None else grad) if grads else None
-    o = make_opts(core, grads, precision, self_term, tile, tpt)
+    o = make_opts(core, grads, precision, self_term, tile, tpt, method=method)
     st = _lib.Stats()
     if dt == np.float64:
         s = _pts_struct(spos, sstr, srad, None, None, _lib.Points64)

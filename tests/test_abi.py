@@ -49,7 +49,7 @@ def test_default_opts():
     from omega3d_amd import _lib
     o = _lib.Opts()
     _lib.lib.o3d_default_opts(C.byref(o))
-    assert (o.core, o.results, o.precision, o.self_term, o.tile, o.tpt, o.maxlev) == (2, 2, 1, 0, 0, 0, -1)
+    assert (o.core, o.results, o.precision, o.self_term, o.tile, o.tpt, o.maxlev, o.method) == (2, 2, 1, 0, 0, 0, -1, 0)
     assert _lib.lib.o3d_abi_version() == 1
 
 

@@ -1,0 +1,159 @@
+"""GPU parity of the factored method (Morton order, far tiles through the matrix
+cores, near tiles direct) against the oracle, through the C ABI.
+
+Bars (normwise, omega3d_amd.metrics), the same as for the direct kernels:
+  * compensated FP32 against the FP64 oracle: max rel <= 1e-6 on ring workloads,
+    <= 2e-6 on random clouds;
+  * plain FP32: no worse than twice the direct plain-FP32 kernel's own error plus 1e-6
+    (both sum in FP32 per target; the orders differ);
+  * the factored and direct methods agree to 2e-6 on every case.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle
+from omega3d_amd import scenes
+from omega3d_amd.metrics import rel_errors, self_term_gradient
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def infl():
+    from omega3d_amd import _lib, influence
+    assert _lib.lib.o3d_init(-1) == 0, _lib.last_error()
+    return influence
+
+
+def _sel(n, k, seed=3):
+    return np.sort(np.random.default_rng(seed).choice(n, size=min(k, n), replace=False))
+
+
+@pytest.mark.parametrize("core", ["rm", "exp"])
+@pytest.mark.parametrize("prec", ["comp", "fp32"])
+def test_factored_ring_matches_oracle(infl, core, prec):
+    """Two colliding rings, N = 131,072 (the full two-ring geometry at 1/8 size): every
+    target evaluated, 2,048 of them checked against the FP64 oracle."""
+    p = scenes.colliding_rings(n_each=65536)
+    print(f"case: {core} {prec} N={p.n}", flush=True)
+    v, g = infl.p2p(core, p.pos, p.str, p.rad, p.pos, p.rad, True, prec, method="factored")
+    vd, gd = infl.p2p(core, p.pos, p.str, p.rad, p.pos, p.rad, True, prec, method="direct")
+    sel = _sel(p.n, 2048)
+    ov, og = oracle.p2p(core, p.pos, p.str, p.rad, p.pos[:, sel], p.rad[sel], True, "f64")
+    ev, eg = rel_errors(v[:, sel], ov), rel_errors(g[:, sel], og)
+    edv, edg = rel_errors(vd[:, sel], ov), rel_errors(gd[:, sel], og)
+    print(f"factored vs fp64 vel {ev} grad {eg}; direct vel {edv} grad {edg}")
+    if prec == "comp":
+        assert ev[0] <= 1e-6 and eg[0] <= 1e-6
+        assert rel_errors(v, vd)[0] <= 2e-6 and rel_errors(g, gd)[0] <= 2e-6
+    else:
+        assert ev[0] <= 2 * edv[0] + 1e-6 and eg[0] <= 2 * edg[0] + 1e-6
+    # the far-field path ran: the summation order differs from the direct kernel
+    assert not np.array_equal(g, gd)
+
+
+def test_factored_full_two_rings_subset(infl):
+    """The full two-ring workload (N = 1,048,576, EXP, sigma 0.02, compensated) by the
+    default method (factored at this size): 512 targets against the FP64 oracle."""
+    c = scenes.colliding_rings()
+    v, g = infl.p2p("exp", c.pos, c.str, c.rad, c.pos, c.rad, True, "comp")
+    sel = np.arange(0, c.n, c.n // 512)[:512]
+    ov, og = oracle.p2p("exp", c.pos, c.str, c.rad, c.pos[:, sel], c.rad[sel], True, "f64")
+    ev, eg = rel_errors(v[:, sel], ov), rel_errors(g[:, sel], og)
+    print("c1 factored comp vs fp64", ev, eg)
+    assert ev[0] <= 1e-6 and eg[0] <= 1e-6
+
+
+@pytest.mark.parametrize("core", ["rm", "exp"])
+@pytest.mark.parametrize("thick", [True, False])
+def test_factored_separate_targets(infl, core, thick):
+    """Targets a different collection from the sources (sorted separately on the
+    common box), counts not multiples of the block or tile, point targets."""
+    c = scenes.colliding_rings(n_each=40000)
+    ns = c.n - 77
+    sp, ss, sr = c.pos[:, :ns], c.str[:, :ns], c.rad[:ns]
+    tp = scenes.ring(70001, R=1.2, a=0.3, seed=9).pos
+    tp[2] += 0.5
+    tr = np.full(tp.shape[1], 0.03, np.float32) if thick else None
+    print(f"case: {core} thick={thick} ns={ns} nt={tp.shape[1]}", flush=True)
+    v, g = infl.p2p(core, sp, ss, sr, tp, tr, True, "comp", method="factored")
+    vd, gd = infl.p2p(core, sp, ss, sr, tp, tr, True, "comp", method="direct")
+    sel = _sel(tp.shape[1], 1024)
+    ov, og = oracle.p2p(core, sp, ss, sr, tp[:, sel], None if tr is None else tr[sel], True, "f64")
+    assert rel_errors(v[:, sel], ov)[0] <= 1e-6 and rel_errors(g[:, sel], og)[0] <= 1e-6
+    assert rel_errors(v, vd)[0] <= 2e-6 and rel_errors(g, gd)[0] <= 2e-6
+
+
+@pytest.mark.parametrize("core", ["rm", "exp"])
+def test_factored_self_excluded(infl, core):
+    p = scenes.colliding_rings(n_each=50000)
+    v, g = infl.p2p(core, p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", self_term=1, method="factored")
+    sel = _sel(p.n, 1024)
+    ov, og = oracle.p2p(core, p.pos, p.str, p.rad, p.pos[:, sel], p.rad[sel], True, "f64")
+    og = og - self_term_gradient(core, p.str[:, sel], p.rad[sel])
+    assert rel_errors(v[:, sel], ov)[0] <= 1e-6 and rel_errors(g[:, sel], og)[0] <= 1e-6
+
+
+@pytest.mark.parametrize("n", [1, 2, 63, 257, 1000, 5000])
+def test_factored_small_and_ragged(infl, n):
+    p = scenes.ring(n, sigma=0.02, seed=n)
+    for core in ("rm", "exp"):
+        for tile in (64, 128, 256):
+            v, g = infl.p2p(core, p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", tile=tile, method="factored")
+            ov, og = oracle.p2p(core, p.pos, p.str, p.rad, p.pos, p.rad, True, "f64")
+            assert rel_errors(v, ov)[0] <= 2e-6 and rel_errors(g, og)[0] <= 2e-6, (core, tile)
+
+
+def test_factored_accumulates(infl):
+    p = scenes.colliding_rings(n_each=40000)
+    v0 = np.random.default_rng(1).normal(size=(3, p.n)).astype(np.float32)
+    g0 = np.random.default_rng(2).normal(size=(9, p.n)).astype(np.float32)
+    v, g = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", vel=v0.copy(), grad=g0.copy(),
+                    method="factored")
+    vz, gz = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", method="factored")
+    np.testing.assert_allclose(v, v0 + vz, rtol=1e-6, atol=1e-6 * np.abs(vz).max())
+    np.testing.assert_allclose(g, g0 + gz, rtol=1e-6, atol=1e-6 * np.abs(gz).max())
+
+
+def test_factored_repeatable_and_device_entry(infl):
+    """Same bits on repeated calls, through the host and the device entry points."""
+    from omega3d_amd import _lib
+    p = scenes.colliding_rings(n_each=40000)
+    a = infl.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", method="factored")
+    b = infl.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", method="factored")
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+    D = _lib.DeviceArray.from_host
+    dp, ds, dr = D(p.pos), D(p.str), D(p.rad)
+    dv, dg = _lib.DeviceArray((3, p.n)), _lib.DeviceArray((9, p.n))
+    dv.zero(); dg.zero()
+    s, t = _lib.Points(), _lib.Points()
+    s.n = t.n = p.n
+    for d in range(3):
+        s.x[d] = t.x[d] = dp.row(d)
+        s.s[d] = ds.row(d)
+        t.u[d] = dv.row(d)
+    s.r = t.r = dr.ptr
+    for k in range(9):
+        t.ug[k] = dg.row(k)
+    o = _lib.make_opts("exp", True, "comp", method="factored")
+    _lib.check(_lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(t), C.byref(o), None))
+    _lib.synchronize()
+    # one collection on the device (sorted once) vs two host copies (sorted twice):
+    # the same permutation, so the same bits
+    np.testing.assert_array_equal(dv.to_host(), a[0])
+    np.testing.assert_array_equal(dg.to_host(), a[1])
+
+
+def test_factored_rejects_unsupported(infl):
+    from omega3d_amd import _lib
+    p = scenes.ring(500)
+    for kw in ({"core": "wl"}, {"core": "v2"}, {"precision": "fp64"}, {"tile": 512}, {"grads": False}):
+        args = dict(core="rm", precision="comp", tile=0, grads=True)
+        args.update(kw)
+        with pytest.raises(_lib.O3DError) as ei:
+            infl.p2p(args["core"], p.pos, p.str, p.rad, p.pos, p.rad, args["grads"], args["precision"],
+                     tile=args["tile"], method="factored")
+        assert ei.value.code == 1, kw
