@@ -118,6 +118,7 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
     float4* tq = cv.take<float4>(nt);
     float4* tsph = cv.take<float4>(ntiles);
     float* tcs = cv.take<float>(ntiles);
+    float* tam = cv.take<float>(ntiles);
     float4* bsph = cv.take<float4>(nb);
     float* bcs = cv.take<float>(nb);
     unsigned* mask = cv.take<unsigned>((size_t)nb * mwords);
@@ -166,8 +167,9 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
     }
     hipLaunchKernelGGL((k_ff_gather_src<CORE>), dim3(grid_of(ns, 256)), dim3(256), 0, st, sv, is_out, ns, sp, sa);
     hipLaunchKernelGGL((k_ff_gather_tgt<CORE, THICK>), dim3(grid_of(nt, 256)), dim3(256), 0, st, tv, tperm, nt, tq);
-    hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)ntiles), dim3(256), 0, st, sp, ns, T, tsph, tcs);
-    hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)nb), dim3(256), 0, st, tq, nt, FF_TB, bsph, bcs);
+    hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)ntiles), dim3(256), 0, st, sp, sa, ns, T, tsph, tcs, tam);
+    hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)nb), dim3(256), 0, st, tq, (const float4*)nullptr, nt, FF_TB, bsph,
+                       bcs, (float*)nullptr);
     hipLaunchKernelGGL((k_ff_classify<CORE>), dim3(grid_of(nb * mwords, 256)), dim3(256), 0, st, bsph, bcs, nb, tsph,
                        tcs, ntiles, mwords, mask);
     hipLaunchKernelGGL(k_ff_plan_count, dim3(grid_of(nb, 256)), dim3(256), 0, st, mask, nb, mwords, ntiles, nn);
@@ -183,16 +185,16 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
 
     FFArgs a{};
     a.sp = sp; a.sa = sa; a.tq = tq; a.tperm = tperm;
-    a.tsph = tsph; a.tcs = tcs; a.bsph = bsph; a.bcs = bcs;
+    a.tsph = tsph; a.tcs = tcs; a.tam = tam; a.bsph = bsph; a.bcs = bcs;
     a.mask = mask; a.farv = farv;
     a.nn = nn; a.items = items; a.ioff = ioff; a.mfold = mfold; a.ctr = ctr; a.part = part;
     a.s = sv; a.t = tv;
     a.ns = ns; a.nt = nt;
     a.tile = T; a.mwords = mwords; a.ntiles = ntiles;
     a.self_excl = self_excl ? 1 : 0;
-    const size_t lds_far = (size_t)T * (sizeof(float4) + FF_LDM * sizeof(float));
+    const size_t lds_far = (size_t)256 * sizeof(float4) + (size_t)2 * 32 * FF_TP * sizeof(_Float16);
     if (prof) HIPCHK(hipEventRecord(ev[1].e, st));
-    hipLaunchKernelGGL((k_ff_far<CORE, THICK, PREC>), dim3((unsigned)nb), dim3(FF_TB), lds_far, st, a);
+    hipLaunchKernelGGL((k_ff_far<CORE, THICK, PREC>), dim3((unsigned)nb), dim3(FF_FAR_THREADS), lds_far, st, a);
     HIPCHK(hipGetLastError());
     if (prof) HIPCHK(hipEventRecord(ev[2].e, st));
     const size_t lds_near = (size_t)T * 2 * sizeof(float4);

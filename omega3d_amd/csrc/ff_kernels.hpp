@@ -40,12 +40,14 @@
 namespace o3d {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int FF_TB = 256;          // targets per workgroup: 4 waves x 64
-constexpr int FF_LDM = 33;          // LDS row stride of the moment table (floats)
 constexpr float FF_KAPPA = 2.0f;    // minimum gap, in block radii
 constexpr int FF_FOLD = 16;         // near field, compensated: sources per FP32 partial
 constexpr int FF_M0 = 16;           // near field: fewest tiles per work item
+constexpr int FF_FAR_THREADS = 256; // far field: 4 waves of 64 targets per block
+constexpr int FF_TP = 264;          // far field: LDS row of the FP16 moment tables (halves, tile <= 256)
 
 struct FFArgs {
   const float4* sp;    // sorted sources {x, y, z, core_term}
@@ -54,6 +56,7 @@ struct FFArgs {
   const int* tperm;    // sorted target -> original target index
   const float4* tsph;  // per source tile: bounding sphere {cx, cy, cz, r}
   const float* tcs;    // per source tile: largest core term
+  const float* tam;    // per source tile: largest |alpha|
   const float4* bsph;  // per target block: bounding sphere
   const float* bcs;    // per target block: largest core term
   const unsigned* mask;  // far bit per (block, tile), mwords words per block
@@ -171,9 +174,11 @@ __device__ __forceinline__ float wg_max(float v, float* red) {
 }
 __device__ __forceinline__ float wg_min(float v, float* red) { return -wg_max(-v, red); }
 
-// bounding sphere (bbox centre, largest distance) and largest core term of each
-// group of `group` consecutive sorted points; one workgroup per group
-__global__ __launch_bounds__(256) void k_ff_bounds(const float4* p, int64_t n, int group, float4* sph, float* cs) {
+// bounding sphere (bbox centre, largest distance), largest core term and (when pa
+// is given) largest |alpha| of each group of `group` consecutive sorted points;
+// one workgroup per group
+__global__ __launch_bounds__(256) void k_ff_bounds(const float4* p, const float4* pa, int64_t n, int group,
+                                                   float4* sph, float* cs, float* amax) {
   __shared__ float red[4];
   const int64_t b0 = (int64_t)blockIdx.x * group;
   const int64_t b1 = (b0 + group) < n ? (b0 + group) : n;
@@ -202,10 +207,18 @@ __global__ __launch_bounds__(256) void k_ff_bounds(const float4* p, int64_t n, i
     r2 = fmaxf(r2, fmaf(dz, dz, fmaf(dy, dy, dx * dx)));
   }
   r2 = wg_max(r2, red);
+  float a2 = 0.0f;
+  if (pa)
+    for (int64_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+      const float4 q = pa[i];
+      a2 = fmaxf(a2, fmaf(q.z, q.z, fmaf(q.y, q.y, q.x * q.x)));
+    }
+  a2 = wg_max(a2, red);
   if (threadIdx.x == 0) {
     // margin for the rounding of the distances above
     sph[blockIdx.x] = make_float4(c[0], c[1], c[2], __fsqrt_rn(r2) * 1.00001f + 1e-30f);
     cs[blockIdx.x] = cmax;
+    if (amax) amax[blockIdx.x] = __fsqrt_rn(a2) * 1.00001f;
   }
 }
 
@@ -234,12 +247,12 @@ __global__ __launch_bounds__(256) void k_ff_classify(const float4* bsph, const f
   mask[g] = bits;
 }
 
-// pair weight Bp = r2^-5/2 of a far pair (t' and s' relative to the block centre)
-template <int CORE>
-__device__ __forceinline__ float bp_far(float tx, float ty, float tz, float tt, const float4 P) {
-  const float dx = tx - P.x, dy = ty - P.y, dz = tz - P.z;
-  const float base = CORE == CORE_RM ? tt + P.w : 0.0f;
-  const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, base)));
+// pair weight Bp = r2^-5/2 of a far pair.  r2 = a + b - 2 t'.s' with a = |t'|^2 + st^2
+// per target (m2t = -2 t') and b = |s'|^2 + ss^2 per source (P.w): for far pairs
+// |t'| <= |d| / 2, so the three terms are at most about 4 |d|^2 and r2 keeps a few
+// ulp of relative accuracy.
+__device__ __forceinline__ float bp_far(float m2tx, float m2ty, float m2tz, float al, const float4 P) {
+  const float r2 = fmaf(m2tz, P.z, fmaf(m2ty, P.y, fmaf(m2tx, P.x, al + P.w)));
   const float ir = frsq(r2);
   const float ir2 = ir * ir;
   return ir2 * ir2 * ir;
@@ -285,34 +298,73 @@ __device__ __forceinline__ void ff_epilogue(const double* c, double tx, double t
   grad_from<double>(a, out + 3);
 }
 
-// Far field: one workgroup per 256 sorted targets; lane l of wave w owns target
-// w*64 + l at the end, and during the sums computes the pair weights of targets
-// w*64 + (l & 31) and w*64 + 32 + (l & 31) with source k0 + (l >> 5).
+// moment column -> scale group: alpha (0), m and s' alpha (1), s' m (2), b alpha (3),
+// b m (4); padding columns 29-31 hold zeros
+__device__ __forceinline__ int ff_group(int col) {
+  return col < 3 ? 0 : col < 15 ? 1 : col < 23 ? 2 : col < 26 ? 3 : 4;
+}
+
+// E with x < 2^E (x > 0), in double so that bounds like gap^-5 cannot overflow
+__device__ __forceinline__ int ff_exp2_above(double x) {
+  int e;
+  (void)frexp(x, &e);
+  return e;
+}
+
+// FP16 hi/lo split of a scaled FP32 value (|x| < 2^15): hi keeps the top 11
+// significant bits (truncated, so it converts to FP16 exactly for |x| >= 2^-14),
+// lo = x - hi is exact in FP32 and rounds to FP16 with a relative error of 2^-11,
+// i.e. about 2^-22 of x.
+__device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
+  const float h = __uint_as_float(__float_as_uint(x) & 0xffffe000u);
+  hi = (_Float16)h;
+  lo = (_Float16)(x - h);
+}
+
+// Far field: one workgroup per 256 sorted targets (4 waves of 64); lane l of wave w
+// owns target w*64 + l at the end, and during the sums computes the pair weights
+// of targets w*64 + (l & 31) (set 0) and w*64 + 32 + (l & 31) (set 1) with sources
+// k0 + 8 (l >> 5) + e, e = 0..7.
 //
-// v_mfma_f32_32x32x2f32 operand layout (A: 32 x 2, B: 2 x 32, D: 32 x 32):
-//   A[i][k] in lane i + 32k, B[k][j] in lane j + 32k,
-//   D[i][j] in lane j + 32 ((i / 4) % 2), register (i % 4) + 4 (i / 8).
-// A = moment table transposed (column i of source k), B = Bp (source k, target j),
-// so D[i][j] = sum over the two sources of moment i x Bp for target j.
+// The products run on the FP16 matrix cores with FP32 accumulation, each factor
+// split into hi + lo halves and three products kept (hi hi + lo hi + hi lo): about
+// 22 significant bits per product, the accuracy of the FP32 products it replaces,
+// at 3/16 of their matrix cost.  Per tile, pair weights are scaled by 2^ew (from
+// Bp <= gap^-5) and each moment group by 2^eg (from the tile's largest |alpha| and
+// source distance) so that every factor stays below 2^14; the scales are removed
+// when the tile's FP32 sums are folded into FP64.
+//
+// v_mfma_f32_32x32x16_f16 operand layout (A: 32 x 16, B: 16 x 32, D: 32 x 32):
+//   A[i][k] in lane i + 32 (k / 8), element k % 8;  B[k][j] in lane j + 32 (k / 8),
+//   element k % 8;  D[i][j] in lane j + 32 ((i / 4) % 2), register (i % 4) + 4 (i / 8).
+// A = moment table transposed (column i of source k), B = Bp (source k, target j).
+// LDS holds the scaled moments as FP16 hi and lo tables [column][source], rows of
+// FF_TP = 264 halves (132 dwords) so that the 16-byte reads of 16 lanes cover all
+// 64 banks.
 template <int CORE, bool THICK, int PREC>
-__global__ __launch_bounds__(256) void k_ff_far(FFArgs a) {
+__global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
   extern __shared__ float4 sh[];
   const int T = a.tile;
+  constexpr int TP = FF_TP;
   float4* pos = sh;
-  float* mom = reinterpret_cast<float*>(sh + T);
+  _Float16* mh = reinterpret_cast<_Float16*>(sh + 256);
+  _Float16* ml = mh + 32 * TP;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int half = lane >> 5, col = lane & 31;
   const int64_t blk = blockIdx.x;
   const int64_t base = blk * FF_TB + w * 64;
   const float4 cb = a.bsph[blk];
 
-  float tx[2], ty[2], tz[2], tt[2];
+  float tx[2], ty[2], tz[2], tt[2], mx[2], my[2], mz[2], al[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
-    int64_t i = base + 32 * s + (lane & 31);
+    int64_t i = base + 32 * s + col;
     i = i < a.nt ? i : a.nt - 1;
     const float4 q = a.tq[i];
     tx[s] = q.x - cb.x; ty[s] = q.y - cb.y; tz[s] = q.z - cb.z;
     tt[s] = CORE == CORE_RM ? q.w : 0.0f;
+    mx[s] = -2.0f * tx[s]; my[s] = -2.0f * ty[s]; mz[s] = -2.0f * tz[s];
+    al[s] = fmaf(tz[s], tz[s], fmaf(ty[s], ty[s], fmaf(tx[s], tx[s], tt[s])));
   }
   f32x16 acc0, acc1;
 #pragma unroll
@@ -326,9 +378,31 @@ __global__ __launch_bounds__(256) void k_ff_far(FFArgs a) {
     if (!((mrow[ti >> 5] >> (ti & 31)) & 1u)) continue;
     const int64_t s0 = (int64_t)ti * T;
     const int cnt = (int)((a.ns - s0) < T ? (a.ns - s0) : T);
+    // scales of this (block, tile), the same in every lane
+    const float4 ct = a.tsph[ti];
+    const double ddx = (double)cb.x - ct.x, ddy = (double)cb.y - ct.y, ddz = (double)cb.z - ct.z;
+    const double dc = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+    const double gap = dc - cb.w - ct.w;                   // > 0 for a far tile
+    const double g2 = gap * gap;
+    const int ew = 14 - ff_exp2_above(1.0 / (g2 * g2 * gap));
+    const double rs = dc + ct.w;                           // |s'| bound
+    const double am = (double)a.tam[ti] + 1e-300;
+    const double bm = rs * rs + (CORE == CORE_RM ? (double)a.tcs[ti] : 0.0);
+    int eg[5];
+    eg[0] = 14 - ff_exp2_above(am);
+    eg[1] = 14 - ff_exp2_above(am * rs);
+    eg[2] = 14 - ff_exp2_above(am * rs * rs);
+    eg[3] = 14 - ff_exp2_above(am * bm);
+    eg[4] = 14 - ff_exp2_above(am * bm * rs);
+    float gs[5];
+#pragma unroll
+    for (int g = 0; g < 5; ++g) gs[g] = ldexpf(1.0f, eg[g]);
+    const float wsc = ldexpf(1.0f, ew);
+
     __syncthreads();
-    for (int j = tid; j < T; j += FF_TB) {
-      float* row = mom + j * FF_LDM;
+    for (int j = tid; j < T; j += FF_FAR_THREADS) {
+      // each column is scaled, split and stored as soon as it is formed
+      auto put = [&](int c, float x) { split16(x * gs[ff_group(c)], mh[c * TP + j], ml[c * TP + j]); };
       if (j < cnt) {
         const float4 P = a.sp[s0 + j];
         const float4 A = a.sa[s0 + j];
@@ -338,54 +412,62 @@ __global__ __launch_bounds__(256) void k_ff_far(FFArgs a) {
         const float m1 = fmaf(A.z, sx, -A.x * sz);
         const float m2 = fmaf(A.x, sy, -A.y * sx);
         const float b = fmaf(sz, sz, fmaf(sy, sy, fmaf(sx, sx, cw)));
-        pos[j] = make_float4(sx, sy, sz, cw);
+        pos[j] = make_float4(sx, sy, sz, b);
         const float sv[3] = {sx, sy, sz}, av[3] = {A.x, A.y, A.z}, mv[3] = {m0, m1, m2};
-        row[0] = A.x; row[1] = A.y; row[2] = A.z;
-        row[3] = m0; row[4] = m1; row[5] = m2;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { put(k, av[k]); put(3 + k, mv[k]); }
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-          for (int k = 0; k < 3; ++k) row[6 + 3 * l + k] = sv[l] * av[k];
+          for (int k = 0; k < 3; ++k) put(6 + 3 * l + k, sv[l] * av[k]);
 #pragma unroll
-        for (int e = 0; e < 8; ++e) row[15 + e] = sv[e / 3] * mv[e % 3];
+        for (int e = 0; e < 8; ++e) put(15 + e, sv[e / 3] * mv[e % 3]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { row[23 + k] = b * av[k]; row[26 + k] = b * mv[k]; }
-        row[29] = 0.0f; row[30] = 0.0f; row[31] = 0.0f;
+        for (int k = 0; k < 3; ++k) { put(23 + k, b * av[k]); put(26 + k, b * mv[k]); }
+#pragma unroll
+        for (int c = 29; c < 32; ++c) put(c, 0.0f);
       } else {
         // padding: far away with zero moments, so that Bp * moment = 0
-        pos[j] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
+        pos[j] = make_float4(1e18f, 1e18f, 1e18f, 3e36f);
 #pragma unroll
-        for (int c = 0; c < 32; ++c) row[c] = 0.0f;
+        for (int c = 0; c < 32; ++c) put(c, 0.0f);
       }
     }
     __syncthreads();
-    const int half = lane >> 5, col = lane & 31;
-#pragma unroll 4
-    for (int k = 0; k < T; k += 2) {
-      const int ks = k + half;
-      const float4 P = pos[ks];
-      const float mv = mom[ks * FF_LDM + col];
-      const float b0 = bp_far<CORE>(tx[0], ty[0], tz[0], tt[0], P);
-      const float b1 = bp_far<CORE>(tx[1], ty[1], tz[1], tt[1], P);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x2f32(mv, b0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x2f32(mv, b1, acc1, 0, 0, 0);
-    }
-    if (PREC == PREC_COMP) {
+    for (int k0 = 0; k0 < T; k0 += 16) {
+      const int kb = k0 + 8 * half;
+      const f16x8 Ah = *reinterpret_cast<const f16x8*>(mh + col * TP + kb);
+      const f16x8 Al = *reinterpret_cast<const f16x8*>(ml + col * TP + kb);
+      f16x8 Bh0, Bl0, Bh1, Bl1;
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        raw0[r] += (double)acc0[r]; acc0[r] = 0.0f;
-        raw1[r] += (double)acc1[r]; acc1[r] = 0.0f;
+      for (int e = 0; e < 8; ++e) {
+        const float4 P = pos[kb + e];
+        _Float16 h, l;
+        split16(bp_far(mx[0], my[0], mz[0], al[0], P) * wsc, h, l);
+        Bh0[e] = h; Bl0[e] = l;
+        split16(bp_far(mx[1], my[1], mz[1], al[1], P) * wsc, h, l);
+        Bh1[e] = h; Bl1[e] = l;
       }
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh1, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl1, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh0, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh1, acc1, 0, 0, 0);
     }
-  }
-  if (PREC != PREC_COMP) {
+    // fold, removing the scales: row of register r in this lane's half
 #pragma unroll
-    for (int r = 0; r < 16; ++r) { raw0[r] = (double)acc0[r]; raw1[r] = (double)acc1[r]; }
+    for (int r = 0; r < 16; ++r) {
+      const int row = 8 * (r / 4) + 4 * half + (r % 4);
+      const double un = ldexp(1.0, -(ew + eg[ff_group(row)]));
+      raw0[r] += (double)acc0[r] * un; acc0[r] = 0.0f;
+      raw1[r] += (double)acc1[r] * un; acc1[r] = 0.0f;
+    }
   }
 
   // lane l < 32 owns target w*64 + l (set 0), lane l >= 32 target w*64 + l (set 1);
   // each holds the rows of its half (l >> 5) and gets the other half from lane l ^ 32
-  const int h = lane >> 5;
+  const int h = half;
   double c[32];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
@@ -399,9 +481,9 @@ __global__ __launch_bounds__(256) void k_ff_far(FFArgs a) {
   const int64_t i = base + lane;
   if (i < a.nt) {
     const double dx = h ? tx[1] : tx[0], dy = h ? ty[1] : ty[0], dz = h ? tz[1] : tz[0];
-    const double al = dx * dx + dy * dy + dz * dz + (double)(h ? tt[1] : tt[0]);
+    const double a2 = dx * dx + dy * dy + dz * dz + (double)(h ? tt[1] : tt[0]);
     double out[12];
-    ff_epilogue(c, dx, dy, dz, al, out);
+    ff_epilogue(c, dx, dy, dz, a2, out);
 #pragma unroll
     for (int k = 0; k < 12; ++k) a.farv[(int64_t)k * a.nt + i] = out[k];
   }
