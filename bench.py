@@ -238,6 +238,10 @@ def main():
     tfile = os.path.join(ROOT, "profiles", "ff_far_exp_n1m_traffic.json" if factored else "p2p_exp_n1m_traffic.json")
     if os.path.exists(tfile):
         traffic = json.load(open(tfile)).get("traffic_bytes")
+    executed = None
+    cfile = os.path.join(ROOT, "profiles", "ff_far_exp_n1m_counters.json")
+    if factored and os.path.exists(cfile):
+        executed = json.load(open(cfile))
     if factored:
         far_flops = FLOPS_PAIR * prof["far_pairs"]
         achieved = far_flops / (prof["far_ms"] / 1e3) / 1e12
@@ -248,7 +252,11 @@ def main():
                 "stages_ms": {k: round(prof[k], 3) for k in ("prep_ms", "far_ms", "near_ms", "total_ms")},
                 "whole_call": {"achieved": round(call_tflops, 3), "frac": round(call_tflops / FP32_PEAK_TFLOPS, 4),
                                "what": "reference flops of all N^2 pairs over the whole evaluation (sort, far, "
-                                       "near)"}}
+                                       "near)"},
+                "note": "achieved counts the reference's 68 FP32 flops per far pair; the kernel executes an exact "
+                        "factorisation instead (about 12 VALU operations per pair plus FP16 hi/lo matrix products), "
+                        "so it can exceed the FP32 peak; 'executed' gives the utilisation of the units it runs on",
+                "executed": executed}
     else:
         roof = {"bound": "valu", "achieved": round(call_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(call_tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic,

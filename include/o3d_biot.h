@@ -149,11 +149,23 @@ int o3d_points_affect_points(const o3d_points* src, o3d_points* targ,
 int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ,
                              const o3d_opts* opts, o3d_stats* stats);
 
+/* points_affect_panels (Influence.h:1107-1221): the velocity of each target panel
+ * induced by vortex particles, the reference's boundary-element right-hand side
+ * input.  The panel kernel is run backwards (rkernel_2vs_0p with the panel carrying
+ * alpha_j / area_i and the particle as the evaluation point) and the sum is
+ * SUBTRACTED from vel[0..2][panel] (Influence.h:1208-1210).  Velocity only
+ * (opts->results must be O3D_VELONLY; the reference asserts otherwise, :1111);
+ * particle radii are not used; targ->ts and targ->ss are not read.  HOST arrays. */
+int o3d_points_affect_panels(const o3d_points* src, const o3d_panels* targ, float* const vel[3],
+                             const o3d_opts* opts, o3d_stats* stats);
+
 /* Same two calls on DEVICE pointers, enqueued on opts->stream, no host sync.
  * stats->seconds is not filled (the call returns before the kernels end). */
 int o3d_points_affect_points_dev(const o3d_points* src, o3d_points* targ,
                                  const o3d_opts* opts, o3d_stats* stats);
 int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ,
+                                 const o3d_opts* opts, o3d_stats* stats);
+int o3d_points_affect_panels_dev(const o3d_points* src, const o3d_panels* targ, float* const vel[3],
                                  const o3d_opts* opts, o3d_stats* stats);
 
 /* Diagnostics of the factored method (no counterpart in the reference): runs one

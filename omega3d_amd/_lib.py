@@ -60,7 +60,8 @@ _int_fns = ["o3d_init", "o3d_abi_version", "o3d_points_affect_points", "o3d_poin
             "o3d_dev_malloc", "o3d_dev_free", "o3d_memcpy_htod", "o3d_memcpy_dtoh", "o3d_memset_dev",
             "o3d_device_synchronize", "o3d_stream_create", "o3d_stream_destroy", "o3d_stream_synchronize",
             "o3d_event_create", "o3d_event_destroy", "o3d_event_record", "o3d_event_elapsed_ms",
-            "o3d_device_info", "o3d_copy_probe", "o3d_factored_profile_dev"]
+            "o3d_device_info", "o3d_copy_probe", "o3d_factored_profile_dev",
+            "o3d_points_affect_panels", "o3d_points_affect_panels_dev"]
 for _n in _int_fns:
     if hasattr(lib, _n):
         getattr(lib, _n).restype = C.c_int

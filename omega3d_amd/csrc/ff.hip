@@ -130,7 +130,7 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
     int* ioff = cv.take<int>(nb);
     int* mfold = cv.take<int>(1);
     unsigned* ctr = cv.take<unsigned>(1);
-    double* part = cv.take<double>((size_t)2 * nb * 12 * FF_TB);   // at most 2 items per block
+    double* part = cv.take<double>((size_t)3 * nb * 12 * FF_TB);   // at most 3 items per block
     if (pass == 0) {
       int rc = workspace(st, cv.off, &base);
       if (rc != O3D_OK) return rc;
@@ -198,7 +198,7 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
     HIPCHK(hipGetLastError());
     if (prof) HIPCHK(hipEventRecord(ev[2].e, st));
     const size_t lds_near = (size_t)T * 2 * sizeof(float4);
-    const unsigned gnear = (unsigned)std::min<int64_t>(2 * nb, near_grid<CORE, PREC>(lds_near));
+    const unsigned gnear = (unsigned)std::min<int64_t>(3 * nb, near_grid<CORE, PREC>(lds_near));
     hipLaunchKernelGGL((k_ff_near<CORE, PREC>), dim3(gnear), dim3(FF_TB), lds_near, st, a);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((k_ff_combine<CORE, PREC>), dim3((unsigned)nb), dim3(FF_TB), 0, st, a);

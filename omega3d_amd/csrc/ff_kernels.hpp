@@ -329,10 +329,11 @@ __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
 // The products run on the FP16 matrix cores with FP32 accumulation, each factor
 // split into hi + lo halves and three products kept (hi hi + lo hi + hi lo): about
 // 22 significant bits per product, the accuracy of the FP32 products it replaces,
-// at 3/16 of their matrix cost.  Per tile, pair weights are scaled by 2^ew (from
-// Bp <= gap^-5) and each moment group by 2^eg (from the tile's largest |alpha| and
-// source distance) so that every factor stays below 2^14; the scales are removed
-// when the tile's FP32 sums are folded into FP64.
+// at 3/16 of their matrix cost.  Per tile, pair weights are scaled by 2^(5 kw) (from
+// Bp <= gap^-5), applied for free as r2 -> r2 2^(-2 kw) through the staged source
+// terms and the targets' a, and each moment group by 2^eg (from the tile's largest
+// |alpha| and source distance), so that every factor stays below 2^14; the scales
+// are removed when the tile's FP32 sums are folded into FP64.
 //
 // v_mfma_f32_32x32x16_f16 operand layout (A: 32 x 16, B: 16 x 32, D: 32 x 32):
 //   A[i][k] in lane i + 32 (k / 8), element k % 8;  B[k][j] in lane j + 32 (k / 8),
@@ -384,7 +385,11 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
     const double dc = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
     const double gap = dc - cb.w - ct.w;                   // > 0 for a far tile
     const double g2 = gap * gap;
-    const int ew = 14 - ff_exp2_above(1.0 / (g2 * g2 * gap));
+    // weight scale 2^(5 kw) <= 2^14 / max Bp, applied as r2 -> r2 2^(-2 kw)
+    const int kw = (int)floor((14 - ff_exp2_above(1.0 / (g2 * g2 * gap))) / 5.0);
+    const int ew = 5 * kw;
+    const float rsc = ldexpf(1.0f, -2 * kw);
+    const float al0 = al[0] * rsc, al1 = al[1] * rsc;
     const double rs = dc + ct.w;                           // |s'| bound
     const double am = (double)a.tam[ti] + 1e-300;
     const double bm = rs * rs + (CORE == CORE_RM ? (double)a.tcs[ti] : 0.0);
@@ -397,7 +402,6 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
     float gs[5];
 #pragma unroll
     for (int g = 0; g < 5; ++g) gs[g] = ldexpf(1.0f, eg[g]);
-    const float wsc = ldexpf(1.0f, ew);
 
     __syncthreads();
     for (int j = tid; j < T; j += FF_FAR_THREADS) {
@@ -412,7 +416,7 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
         const float m1 = fmaf(A.z, sx, -A.x * sz);
         const float m2 = fmaf(A.x, sy, -A.y * sx);
         const float b = fmaf(sz, sz, fmaf(sy, sy, fmaf(sx, sx, cw)));
-        pos[j] = make_float4(sx, sy, sz, b);
+        pos[j] = make_float4(sx * rsc, sy * rsc, sz * rsc, b * rsc);   // exact: rsc is a power of 2
         const float sv[3] = {sx, sy, sz}, av[3] = {A.x, A.y, A.z}, mv[3] = {m0, m1, m2};
 #pragma unroll
         for (int k = 0; k < 3; ++k) { put(k, av[k]); put(3 + k, mv[k]); }
@@ -443,9 +447,9 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
       for (int e = 0; e < 8; ++e) {
         const float4 P = pos[kb + e];
         _Float16 h, l;
-        split16(bp_far(mx[0], my[0], mz[0], al[0], P) * wsc, h, l);
+        split16(bp_far(mx[0], my[0], mz[0], al0, P), h, l);
         Bh0[e] = h; Bl0[e] = l;
-        split16(bp_far(mx[1], my[1], mz[1], al[1], P) * wsc, h, l);
+        split16(bp_far(mx[1], my[1], mz[1], al1, P), h, l);
         Bh1[e] = h; Bl1[e] = l;
       }
       acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh0, acc0, 0, 0, 0);
@@ -491,7 +495,7 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
 
 // ---- near field -------------------------------------------------------------
 // The near tiles of a block are split into items of at most M tiles (M chosen on
-// the device so that there are at most about 2 items per block), so that a block
+// the device so that there are at most 3 items per block on average), so that a block
 // whose targets straddle a jump of the Morton order, and which therefore has
 // nearly every tile near, is spread over many workgroups.  Items are taken from a
 // queue by a persistent kernel and write FP64 partial sums; the combine kernel
@@ -510,13 +514,13 @@ __global__ __launch_bounds__(256) void k_ff_plan_count(const unsigned* mask, int
   if (b < nb) nn[b] = near_tiles_of(mask + b * mwords, mwords, ntiles);
 }
 
-// after the exclusive scan noff of nn: M = max(FF_M0, ceil(total / nb)) and the
-// item count of every block
+// after the exclusive scan noff of nn: M = max(FF_M0, ceil(total / (2 nb))) and the
+// item count of every block (at most 3 nb items in all)
 __global__ __launch_bounds__(256) void k_ff_plan_items(const int* nn, const int* noff, int64_t nb, int* items,
                                                        int* mfold) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t total = (int64_t)noff[nb - 1] + nn[nb - 1];
-  int64_t m = (total + nb - 1) / nb;
+  int64_t m = (total + 2 * nb - 1) / (2 * nb);
   m = m < FF_M0 ? FF_M0 : m;
   if (b == 0) *mfold = (int)m;
   if (b < nb) items[b] = (int)((nn[b] + m - 1) / m);

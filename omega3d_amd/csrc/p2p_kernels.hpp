@@ -70,12 +70,13 @@ __device__ __forceinline__ void exp_far(float d2, float& K, float& Bp) {
 }
 __device__ __forceinline__ void exp_any(float d2, float cs, float& K, float& Bp) {
   const bool far = d2 * d2 * d2 > 256.0f * (cs * cs);
-  float kf, bf;
-  exp_far(d2, kf, bf);                           // the far values, as on the fast path
+  // d2m differs from d2 only below FLT_MIN, where no pair is far: the far values
+  // below are bit-identical to exp_far(d2)
   const float d2m = fmaxf(d2, 1.17549435e-38f);
   const float ir = frsq(d2m);
   const float ir2 = ir * ir;
   const float ood3 = ir2 * ir;
+  const float kf = ood3, bf = ood3 * ir2;
   const float c = frcp(cs);
   const float dist = d2m * ir;
   const float q = (d2m * dist) * c;
@@ -132,7 +133,11 @@ __device__ __forceinline__ void kb(float d2, float st, float thr, float& K, floa
 // [11..13] W = sum_j K_j a_j
 constexpr int NACC = 14;
 
-// accumulate one pair given its separation d = t - s and K, Bp
+// accumulate one pair given its separation d = t - s and K, Bp.
+// VALU operations per pair with the gradient (far branch of EXP, or RM): d 3,
+// |d|^2 3, v_rsq 1, K and Bp 3, branch test 1 (EXP), c = alpha x d 6, velocity 3,
+// b = Bp c 3, dyad 8, W 3: 34; the LDS reads of the source are shared by the lane's
+// targets.
 template <bool GRAD>
 __device__ __forceinline__ void acc_pair(float dx, float dy, float dz, float K, float Bp, const float4 A, float* acc) {
   const float cx = fmaf(dz, A.y, -dy * A.z);

@@ -1,0 +1,250 @@
+// ptpan.hip -- particles -> panels: Omega3D's points_affect_panels
+// (Influence.h:1107-1221), the velocity of each panel induced by vortex particles,
+// as used for the boundary-element right-hand side.
+//
+// The reference runs the panel kernel backwards: for target panel i and particle
+// j it evaluates rkernel_2vs_0p with the panel carrying alpha_j / area_i (divided
+// in S, multiplied back by the area at level 0), the particle's position as the
+// "target", and SUBTRACTS the sum from the panel's velocity (Influence.h:1196-1210).
+// Velocity only: the reference asserts on gradient requests (:1111).
+//
+// GPU form: one lane per target panel (its corners, area, sqrt(area) and exact
+// centroid in registers), particles staged per tile in LDS and broadcast, the
+// same depth-first tree walk as panels_affect_points (pan_kernels.hpp), so the
+// subdivision decisions are the reference's.
+#include "o3d_api_internal.hpp"
+#include "pan_kernels.hpp"
+
+#include <chrono>
+#include <vector>
+
+namespace o3d {
+
+struct PtPanArgs {
+  const float* sx; const float* sy; const float* sz;    // particles
+  const float* ax; const float* ay; const float* az;
+  int64_t ns;
+  const float* nx; const float* ny; const float* nz;    // panel nodes
+  const uint32_t* idx;
+  const float* area;
+  int64_t np;
+  void* u; void* v; void* w;   // float (decremented in place) or double (FP64: zeroed scratch)
+  int32_t tile;
+  int32_t maxlev;
+};
+
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
+  using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
+  extern __shared__ float4 shs[];
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  const bool active = i < a.np;
+  const int64_t ic = active ? i : a.np - 1;
+  const uint32_t i0 = a.idx[3 * ic], i1 = a.idx[3 * ic + 1], i2 = a.idx[3 * ic + 2];
+  const float x0 = a.nx[i0], y0 = a.ny[i0], z0 = a.nz[i0];
+  const float x1 = a.nx[i1], y1 = a.ny[i1], z1 = a.nz[i1];
+  const float x2 = a.nx[i2], y2 = a.ny[i2], z2 = a.nz[i2];
+  const float ar = a.area[ic];
+  float cx, cy, cz;
+  centroid_exact(x0, y0, z0, x1, y1, z1, x2, y2, z2, cx, cy, cz);
+  // this lane's panel in the staged-panel layout of pan_kernels.hpp; the strength
+  // fields are filled per particle
+  float4 s[PAN_F4];
+  s[0] = make_float4(x0, y0, z0, x1);
+  s[1] = make_float4(y1, z1, x2, y2);
+  s[2] = make_float4(z2, 0.0f, 0.0f, 0.0f);
+  s[3] = make_float4(0.0f, ar, sqrtf(ar), 0.0f);
+  s[4] = make_float4(cx, cy, cz, 0.0f);
+  s[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  R acc[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) acc[c] = R(0);
+  float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
+
+  for (int64_t p0 = 0; p0 < a.ns; p0 += a.tile) {
+    const int cnt = (int)((a.ns - p0) < a.tile ? (a.ns - p0) : a.tile);
+    __syncthreads();
+    for (int j = threadIdx.x; j < cnt; j += BLOCK) {
+      const int64_t g = p0 + j;
+      shs[2 * j] = make_float4(a.sx[g], a.sy[g], a.sz[g], 0.0f);
+      shs[2 * j + 1] = make_float4(a.ax[g], a.ay[g], a.az[g], 0.0f);
+    }
+    __syncthreads();
+    for (int j = 0; j < cnt; ++j) {
+      const float4 P = shs[2 * j], A = shs[2 * j + 1];
+      if (PREC == PREC_FP64) {
+        s[5] = make_float4(A.x, A.y, A.z, 0.0f);   // the walk forms (alpha / area) * area in double
+      } else {
+        // alpha / area in float (Influence.h:1198), times the area at level 0
+        s[2].y = __fmul_rn(__fdiv_rn(A.x, ar), ar);
+        s[2].z = __fmul_rn(__fdiv_rn(A.y, ar), ar);
+        s[2].w = __fmul_rn(__fdiv_rn(A.z, ar), ar);
+      }
+      walk_panel<R, false>(P.x, P.y, P.z, s, a.maxlev, active, acc);
+    }
+    if (PREC == PREC_COMP) {
+      two_sum(hi[0], lo[0], (float)acc[0]);
+      two_sum(hi[1], lo[1], (float)acc[1]);
+      two_sum(hi[2], lo[2], (float)acc[2]);
+      acc[0] = acc[1] = acc[2] = R(0);
+    }
+  }
+  if (!active) return;
+  if (PREC == PREC_FP64) {
+    ((double*)a.u)[i] = -(double)acc[0];
+    ((double*)a.v)[i] = -(double)acc[1];
+    ((double*)a.w)[i] = -(double)acc[2];
+  } else if (PREC == PREC_FP32) {
+    ((float*)a.u)[i] -= (float)acc[0];
+    ((float*)a.v)[i] -= (float)acc[1];
+    ((float*)a.w)[i] -= (float)acc[2];
+  } else {
+    float* u = (float*)a.u; float* v = (float*)a.v; float* w = (float*)a.w;
+    u[i] = (float)((double)u[i] - ((double)hi[0] + (double)lo[0]));
+    v[i] = (float)((double)v[i] - ((double)hi[1] + (double)lo[1]));
+    w[i] = (float)((double)w[i] - ((double)hi[2] + (double)lo[2]));
+  }
+}
+
+static hipError_t launch_ptpan(const PtPanArgs& a, int prec, hipStream_t st) {
+  if (a.np <= 0 || a.ns <= 0) return hipSuccess;
+  const size_t lds = (size_t)a.tile * 2 * sizeof(float4);
+  const dim3 grid((unsigned)((a.np + BLOCK - 1) / BLOCK));
+  if (prec == PREC_FP32) hipLaunchKernelGGL((k_ptpan<PREC_FP32>), grid, dim3(BLOCK), lds, st, a);
+  else if (prec == PREC_COMP) hipLaunchKernelGGL((k_ptpan<PREC_COMP>), grid, dim3(BLOCK), lds, st, a);
+  else hipLaunchKernelGGL((k_ptpan<PREC_FP64>), grid, dim3(BLOCK), lds, st, a);
+  return hipGetLastError();
+}
+
+static int check_ptpan(const o3d_points* src, const o3d_panels* targ, float* const* vel, const o3d_opts* o) {
+  if (o->results != O3D_VELONLY)
+    return fail(O3D_EINVAL, "points_affect_panels computes velocity only (the reference asserts on gradients, "
+                            "Influence.h:1111)");
+  if (!src) return fail(O3D_EINVAL, "src is NULL");
+  if (src->n < 0) return fail(O3D_EINVAL, "src: negative count");
+  if (src->n > 0)
+    for (int d = 0; d < 3; ++d)
+      if (!src->x[d] || !src->s[d]) return fail(O3D_EINVAL, "src: position or strength component %d is NULL", d);
+  if (!targ) return fail(O3D_EINVAL, "targ panels are NULL");
+  if (targ->np < 0 || targ->nn < 0) return fail(O3D_EINVAL, "negative panel or node count");
+  if (targ->np > 0) {
+    if (targ->nn <= 0) return fail(O3D_EINVAL, "panels without nodes");
+    for (int d = 0; d < 3; ++d)
+      if (!targ->x[d]) return fail(O3D_EINVAL, "panel node component %d is NULL", d);
+    if (!targ->idx || !targ->area) return fail(O3D_EINVAL, "panel indices or areas are NULL");
+    if (!vel) return fail(O3D_EINVAL, "panel velocity arrays are NULL");
+    for (int d = 0; d < 3; ++d)
+      if (!vel[d]) return fail(O3D_EINVAL, "panel velocity component %d is NULL", d);
+  }
+  if (o->maxlev > 8) return fail(O3D_EINVAL, "maxlev %d: at most 8 levels", o->maxlev);
+  return O3D_OK;
+}
+
+static PtPanArgs ptpan_args(const o3d_points* src, const o3d_panels* targ, const o3d_opts* o) {
+  PtPanArgs a{};
+  a.sx = src->x[0]; a.sy = src->x[1]; a.sz = src->x[2];
+  a.ax = src->s[0]; a.ay = src->s[1]; a.az = src->s[2];
+  a.ns = src->n;
+  a.nx = targ->x[0]; a.ny = targ->x[1]; a.nz = targ->x[2];
+  a.idx = targ->idx; a.area = targ->area; a.np = targ->np;
+  a.tile = o->tile > 0 ? o->tile : 256;
+  a.maxlev = o->maxlev < 0 ? 3 : o->maxlev;   // RECURSIVE_LEVELS, Influence.h:23
+  return a;
+}
+
+static double ptpan_flops(int64_t np, int64_t ns) {
+  // one leaf kernel_0vs_0p per (panel, particle) pair plus 3 per panel (Influence.h:1212)
+  const int f = o3d_flops_per_interaction(CORE_RM, O3D_VELONLY, 0, 1);
+  return (double)np * (3.0 + (double)f * (double)ns);
+}
+
+}  // namespace o3d
+
+using namespace o3d;
+
+extern "C" {
+
+int o3d_points_affect_panels(const o3d_points* src, const o3d_panels* targ, float* const vel[3],
+                             const o3d_opts* o, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  if ((rc = check_ptpan(src, targ, vel, o)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t ns = src->n, np = targ->np, nn = targ->nn;
+  if (ns > 0 && np > 0) {
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    hipStream_t st = lib_stream();
+    const bool f64 = o->precision == PREC_FP64;
+    void *dp, *ds;
+    // panels: nodes 3*nn, area np (floats), idx 3*np (uint32), velocities 3*np (float or double)
+    const size_t vbytes = (f64 ? sizeof(double) : sizeof(float)) * 3 * np;
+    if ((rc = pool_buffer(2, sizeof(float) * (3 * nn + 4 * np) + vbytes + 256, &dp)) != O3D_OK) return rc;
+    if ((rc = pool_buffer(3, sizeof(float) * 6 * ns, &ds)) != O3D_OK) return rc;
+    float* f = (float*)dp;
+    o3d_panels d{};
+    d.np = np; d.nn = nn;
+    for (int k = 0; k < 3; ++k) d.x[k] = f + k * nn;
+    d.area = f + 3 * nn;
+    d.idx = (const uint32_t*)(f + 3 * nn + np);
+    char* vb = (char*)(f + 3 * nn + 4 * np);
+    vb += (256 - ((uintptr_t)vb & 255)) & 255;
+    for (int k = 0; k < 3; ++k)
+      HIPCHK(hipMemcpyAsync((void*)d.x[k], targ->x[k], sizeof(float) * nn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync((void*)d.area, targ->area, sizeof(float) * np, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync((void*)d.idx, targ->idx, sizeof(uint32_t) * 3 * np, hipMemcpyHostToDevice, st));
+    o3d_points dsrc{};
+    dsrc.n = ns;
+    float* sf = (float*)ds;
+    for (int k = 0; k < 3; ++k) {
+      dsrc.x[k] = sf + k * ns;
+      dsrc.s[k] = sf + (3 + k) * ns;
+      HIPCHK(hipMemcpyAsync((void*)dsrc.x[k], src->x[k], sizeof(float) * ns, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync((void*)dsrc.s[k], src->s[k], sizeof(float) * ns, hipMemcpyHostToDevice, st));
+    }
+    PtPanArgs a = ptpan_args(&dsrc, &d, o);
+    const size_t vsz = f64 ? sizeof(double) : sizeof(float);
+    a.u = vb; a.v = vb + vsz * np; a.w = vb + 2 * vsz * np;
+    if (!f64)
+      for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpyAsync(vb + vsz * np * k, vel[k], sizeof(float) * np, hipMemcpyHostToDevice, st));
+    hipError_t e = launch_ptpan(a, o->precision, st);
+    if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
+    if (f64) {
+      std::vector<double> h((size_t)3 * np);
+      HIPCHK(hipMemcpyAsync(h.data(), vb, vbytes, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      for (int k = 0; k < 3; ++k)
+        for (int64_t i = 0; i < np; ++i) vel[k][i] = (float)((double)vel[k][i] + h[k * np + i]);
+    } else {
+      for (int k = 0; k < 3; ++k)
+        HIPCHK(hipMemcpyAsync(vel[k], vb + vsz * np * k, sizeof(float) * np, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+    }
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, secs, ptpan_flops(np, ns), (double)np * (double)ns);
+  return O3D_OK;
+}
+
+int o3d_points_affect_panels_dev(const o3d_points* src, const o3d_panels* targ, float* const vel[3],
+                                 const o3d_opts* o, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  if ((rc = check_ptpan(src, targ, vel, o)) != O3D_OK) return rc;
+  if (o->precision == PREC_FP64)
+    return fail(O3D_EINVAL, "FP64 particles-on-panels on device float arrays are not offered; use the host call");
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  if (src->n > 0 && targ->np > 0) {
+    PtPanArgs a = ptpan_args(src, targ, o);
+    a.u = vel[0]; a.v = vel[1]; a.w = vel[2];
+    hipError_t e = launch_ptpan(a, o->precision, (hipStream_t)o->stream);
+    if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
+  }
+  fill_stats(stats, 0.0, ptpan_flops(targ->np, src->n), (double)targ->np * (double)src->n);
+  return O3D_OK;
+}
+
+}  // extern "C"

@@ -148,6 +148,31 @@ def p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, precision="comp", s
     return (vel, grad, st) if stats else (vel, grad)
 
 
+def ptpan(spos, sstr, nodes, idx, area, precision="comp", maxlev=-1, vel=None, stats=False):
+    """Host-array particles -> panels (points_affect_panels, Influence.h:1107-1221):
+    panel velocities, decremented by the particles' influence (new zero arrays when
+    vel is not given)."""
+    spos, sstr, nodes, area = _c32(spos), _c32(sstr), _c32(nodes), _c32(area)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    np_ = area.shape[0]
+    vel = np.zeros((3, np_), np.float32) if vel is None else vel
+    s = _lib.Points()
+    s.n = spos.shape[1]
+    for d in range(3):
+        s.x[d] = spos[d].ctypes.data
+        s.s[d] = sstr[d].ctypes.data
+    p = _lib.Panels()
+    p.np, p.nn = np_, nodes.shape[1]
+    for d in range(3):
+        p.x[d] = nodes[d].ctypes.data
+    p.idx, p.area = idx.ctypes.data, area.ctypes.data
+    vp = (C.c_void_p * 3)(*[vel[d].ctypes.data for d in range(3)])
+    o = make_opts("rm", False, precision, 0, 0, 0, maxlev)
+    st = _lib.Stats()
+    check(lib.o3d_points_affect_panels(C.byref(s), C.byref(p), vp, C.byref(o), C.byref(st)))
+    return (vel, st) if stats else vel
+
+
 def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, precision="comp", maxlev=-1,
         vel=None, grad=None, stats=False):
     """Host-array panel->point sum (recursive centroid rule) on the GPU."""

@@ -161,6 +161,27 @@ def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, mode="f32d", maxlev=RE
     return vel, grad
 
 
+def ptpan(core, spos, sstr, nodes, idx, area, mode="f32d", maxlev=RECURSIVE_LEVELS, nthreads=0):
+    """points_affect_panels restated (Influence.h:1190-1210): velocity of each panel,
+    decremented by the panel kernel run backwards from every particle."""
+    cid = _core_id(core)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    dt = np.float64 if mode == "f64" else np.float32
+    spos, sstr, nodes, area = _as(spos, dt), _as(sstr, dt), _as(nodes, dt), _as(area, dt)
+    np_ = area.shape[0]
+    vel = np.zeros((3, np_), dt)
+    args = [_i64(spos.shape[1]), _ptr(spos[0]), _ptr(spos[1]), _ptr(spos[2]),
+            _ptr(sstr[0]), _ptr(sstr[1]), _ptr(sstr[2]), _i64(np_), _ptr(nodes[0]), _ptr(nodes[1]),
+            _ptr(nodes[2]), _ptr(idx), _ptr(area), _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), int(nthreads)]
+    if mode == "f64":
+        rc = lib().orc_ptpan_d(cid, int(maxlev), *args)
+    else:
+        rc = lib().orc_ptpan_f(cid, {"f32f": 0, "f32d": 1}[mode], int(maxlev), *args)
+    if rc != 0:
+        raise RuntimeError(f"oracle ptpan failed rc={rc}")
+    return vel
+
+
 # ---- the reference's own kernels (oracle/_ref), same conventions -------------
 
 def ref_p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, mode="f32d"):
@@ -206,6 +227,27 @@ def ref_pan(core, nodes, idx, ts, area, ss, tpos, grads=True, mode="f32d", maxle
     if rc != 0:
         raise RuntimeError(f"ref pan failed rc={rc}")
     return vel, grad
+
+
+def ref_ptpan(core, spos, sstr, nodes, idx, area, mode="f32d", maxlev=RECURSIVE_LEVELS):
+    L = ref_lib(core)
+    if L is None:
+        raise FileNotFoundError(ref_path(core))
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    dt = np.float64 if mode == "f64" else np.float32
+    spos, sstr, nodes, area = _as(spos, dt), _as(sstr, dt), _as(nodes, dt), _as(area, dt)
+    np_ = area.shape[0]
+    vel = np.zeros((3, np_), dt)
+    args = [_i64(spos.shape[1]), _ptr(spos[0]), _ptr(spos[1]), _ptr(spos[2]),
+            _ptr(sstr[0]), _ptr(sstr[1]), _ptr(sstr[2]), _i64(np_), _ptr(nodes[0]), _ptr(nodes[1]),
+            _ptr(nodes[2]), _ptr(idx), _ptr(area), _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2])]
+    if mode == "f64":
+        rc = L.ref_ptpan_d(int(maxlev), *args)
+    else:
+        rc = L.ref_ptpan_f({"f32f": 0, "f32d": 1}[mode], int(maxlev), *args)
+    if rc != 0:
+        raise RuntimeError(f"ref ptpan failed rc={rc}")
+    return vel
 
 
 def ref_flops(core, which):

@@ -358,6 +358,38 @@ void pan_loop(const Pan<T>& p, const Tgt<T>& t, int maxlev, int nthreads) {
   }
 }
 
+// points_affect_panels (Influence.h:1190-1210, scalar branch): the same recursion
+// used backwards -- panel i carries particle j's strength divided by the panel's
+// area, the "target" is the particle, and the sum is subtracted from the panel's
+// velocity.  Velocity only (the reference asserts on gradients, :1111).
+template <int CORE, class S, class A, class T>
+void ptpan_loop(const Src<T>& s, const Pan<T>& p, T* u, T* v, T* w, int maxlev, int nthreads) {
+  const int64_t np = p.np, ns = s.n;
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int64_t i = 0; i < np; ++i) {
+    A acc[12] = {0};
+    const uint32_t a = p.idx[3*i], b = p.idx[3*i+1], c = p.idx[3*i+2];
+    const S ar = S(p.area[i]);
+    for (int64_t j = 0; j < ns; ++j)
+      rpanel<CORE, false, S, A>(S(p.x[a]), S(p.y[a]), S(p.z[a]), S(p.x[b]), S(p.y[b]), S(p.z[b]),
+                                S(p.x[c]), S(p.y[c]), S(p.z[c]),
+                                S(s.sx[j])/ar, S(s.sy[j])/ar, S(s.sz[j])/ar, S(0),
+                                S(s.x[j]), S(s.y[j]), S(s.z[j]), ar, 0, maxlev, acc);
+    u[i] -= acc[0]; v[i] -= acc[1]; w[i] -= acc[2];
+  }
+}
+
+template <class S, class A, class T>
+int ptpan_dispatch(int core, const Src<T>& s, const Pan<T>& p, T* u, T* v, T* w, int maxlev, int nth) {
+  switch (core) {
+    case C_RM:  ptpan_loop<C_RM, S, A>(s, p, u, v, w, maxlev, nth); return 0;
+    case C_WL:  ptpan_loop<C_WL, S, A>(s, p, u, v, w, maxlev, nth); return 0;
+    case C_EXP: ptpan_loop<C_EXP, S, A>(s, p, u, v, w, maxlev, nth); return 0;
+    case C_V2:  ptpan_loop<C_V2, S, A>(s, p, u, v, w, maxlev, nth); return 0;
+  }
+  return 1;
+}
+
 template <class S, class A, class T>
 int pan_dispatch(int core, int grads, const Pan<T>& p, const Tgt<T>& t, int maxlev, int nth) {
 #define O3D_PAN_CASE(CID) \
@@ -577,6 +609,30 @@ int orc_pan_d(int core, int grads, int maxlev, int64_t np, const double* x, cons
   Pan<double> p{np, x, y, z, idx, tsx, tsy, tsz, area, ss};
   Tgt<double> t{nt, tx, ty, tz, nullptr, tu, tv, tw, tug};
   return pan_dispatch<double, double>(core, grads, p, t, maxlev, clamp_threads(nthreads));
+}
+
+// points_affect_panels: particles (position, strength; radii unused) onto panels;
+// u/v/w per panel, decremented as the reference does.  mode 0: S=A=float,
+// 1: S=float, A=double.
+int orc_ptpan_f(int core, int mode, int maxlev, int64_t ns, const float* sx, const float* sy, const float* sz,
+                const float* ssx, const float* ssy, const float* ssz, int64_t np, const float* x, const float* y,
+                const float* z, const uint32_t* idx, const float* area, float* tu, float* tv, float* tw,
+                int nthreads) {
+  Src<float> s{ns, sx, sy, sz, nullptr, ssx, ssy, ssz};
+  Pan<float> p{np, x, y, z, idx, nullptr, nullptr, nullptr, area, nullptr};
+  const int nth = clamp_threads(nthreads);
+  if (mode == 0) return ptpan_dispatch<float, float>(core, s, p, tu, tv, tw, maxlev, nth);
+  if (mode == 1) return ptpan_dispatch<float, double>(core, s, p, tu, tv, tw, maxlev, nth);
+  return 3;
+}
+
+int orc_ptpan_d(int core, int maxlev, int64_t ns, const double* sx, const double* sy, const double* sz,
+                const double* ssx, const double* ssy, const double* ssz, int64_t np, const double* x,
+                const double* y, const double* z, const uint32_t* idx, const double* area, double* tu, double* tv,
+                double* tw, int nthreads) {
+  Src<double> s{ns, sx, sy, sz, nullptr, ssx, ssy, ssz};
+  Pan<double> p{np, x, y, z, idx, nullptr, nullptr, nullptr, area, nullptr};
+  return ptpan_dispatch<double, double>(core, s, p, tu, tv, tw, maxlev, clamp_threads(nthreads));
 }
 
 // 1 if the host has AVX2+FMA (the Vc-style loop needs them)

@@ -10,7 +10,9 @@
 // The loops below restate the scalar branches of points_affect_points
 // (Influence.h:405-440 thick targets, :280-307 point targets) and
 // panels_affect_points (Influence.h:729-775, 968-1006): zero accumulators of type A
-// per target, visit every source in index order, add into the caller's outputs.
+// per target, visit every source in index order, add into the caller's outputs;
+// and of points_affect_panels (Influence.h:1190-1210), which runs the panel kernel
+// backwards and subtracts from the panels' velocities.
 
 #include "Kernels.h"
 #include <cstdint>
@@ -76,9 +78,42 @@ void ref_pan_loop(int grads, int maxlev, int64_t np, const S* x, const S* y, con
   }
 }
 
+template <class S, class A>
+void ref_ptpan_loop(int maxlev, int64_t ns, const S* sx, const S* sy, const S* sz, const S* ssx, const S* ssy,
+                    const S* ssz, int64_t np, const S* x, const S* y, const S* z, const uint32_t* idx,
+                    const S* area, S* tu, S* tv, S* tw) {
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < np; ++i) {
+    A u = 0, v = 0, w = 0;
+    const size_t a = idx[3*i], b = idx[3*i+1], c = idx[3*i+2];
+    for (int64_t j = 0; j < ns; ++j)
+      (void) rkernel_2vs_0p<S, A>(x[a], y[a], z[a], x[b], y[b], z[b], x[c], y[c], z[c],
+                                  ssx[j]/area[i], ssy[j]/area[i], ssz[j]/area[i], S(0.0),
+                                  sx[j], sy[j], sz[j], area[i], 0, maxlev, &u, &v, &w);
+    tu[i] -= u; tv[i] -= v; tw[i] -= w;
+  }
+}
+
 }  // namespace
 
 extern "C" {
+
+// points_affect_panels; accum: 0 = A float, 1 = A double (S = float)
+int ref_ptpan_f(int accum, int maxlev, int64_t ns, const float* sx, const float* sy, const float* sz,
+                const float* ssx, const float* ssy, const float* ssz, int64_t np, const float* x, const float* y,
+                const float* z, const uint32_t* idx, const float* area, float* tu, float* tv, float* tw) {
+  if (accum) ref_ptpan_loop<float, double>(maxlev, ns, sx, sy, sz, ssx, ssy, ssz, np, x, y, z, idx, area, tu, tv, tw);
+  else       ref_ptpan_loop<float, float>(maxlev, ns, sx, sy, sz, ssx, ssy, ssz, np, x, y, z, idx, area, tu, tv, tw);
+  return 0;
+}
+
+int ref_ptpan_d(int maxlev, int64_t ns, const double* sx, const double* sy, const double* sz, const double* ssx,
+                const double* ssy, const double* ssz, int64_t np, const double* x, const double* y, const double* z,
+                const uint32_t* idx, const double* area, double* tu, double* tv, double* tw) {
+  ref_ptpan_loop<double, double>(maxlev, ns, sx, sy, sz, ssx, ssy, ssz, np, x, y, z, idx, area, tu, tv, tw);
+  return 0;
+}
+
 
 // accum: 0 = A float, 1 = A double (S = float)
 int ref_p2p_f(int grads, int accum, int64_t ns, const float* sx, const float* sy, const float* sz,

@@ -10,6 +10,9 @@ MathHelper.h).  The fixtures hold seeded inputs and the reference's outputs:
   pan_<core>.npz   panels_affect_points scalar loop (Influence.h:729-775) with
                    rkernel_2vs_0pg / rkernel_2vs_0p, maxlev = 3, covering every
                    recursion depth, with and without a source strength.
+  ptpan_<core>.npz points_affect_panels scalar loop (Influence.h:1190-1210): the
+                   panel kernel run backwards from particles onto the same panels,
+                   particles placed so that every recursion depth is used.
   flops.npz        the reference's flops_0v_0bg .. flops_0vs_0p for each core.
 
     python tests/golden/make_golden.py
@@ -93,6 +96,14 @@ def main():
         np.savez_compressed(os.path.join(HERE, f"pan_{core}.npz"), **out)
         flops[core] = np.array([oracle.ref_flops(core, k) for k in range(6)], np.int32)
     np.savez_compressed(os.path.join(HERE, "flops.npz"), **flops)
+    rng = np.random.default_rng(20241202)
+    for core in CORES:
+        nodes, idx, _, area, _, ppos = panel_inputs(rng)
+        pstr = (rng.normal(size=ppos.shape) * 0.05).astype(np.float32)
+        out = dict(nodes=nodes, idx=idx, area=area, spos=ppos, sstr=pstr)
+        for mode in MODES:
+            out["vel_" + mode] = oracle.ref_ptpan(core, ppos, pstr, nodes, idx, area, mode)
+        np.savez_compressed(os.path.join(HERE, f"ptpan_{core}.npz"), **out)
     print("golden fixtures written to", HERE)
 
 

@@ -380,3 +380,60 @@ def test_rk2_freestream_only():
     x, s, e = rk2.rk2_step("rm", p.pos, st, p.rad, np.ones(p.n), 0.2, fs, "f64")
     np.testing.assert_allclose(x, p.pos.astype(np.float64) + 0.2 * np.array(fs)[:, None], rtol=0, atol=1e-15)
     assert np.all(s == 0) and np.all(e == 1)
+
+
+# ---- points_affect_panels (Influence.h:1190-1210) ---------------------------------
+
+@pytest.mark.parametrize("core", CORES)
+def test_particles_on_panels_match_reference_golden_bitwise(core):
+    g = gold(f"ptpan_{core}.npz")
+    for mode in MODES:
+        v = oracle.ptpan(core, g["spos"], g["sstr"], g["nodes"], g["idx"], g["area"], mode)
+        np.testing.assert_array_equal(v, g["vel_" + mode], err_msg=mode)
+
+
+@pytest.mark.skipif(any(oracle.ref_lib(c) is None for c in CORES), reason="oracle/_ref not built")
+def test_particles_on_panels_match_live_reference():
+    pan = scenes.icosphere_panels(levels=2)
+    rng = np.random.default_rng(78)
+    d = rng.normal(size=(3, 90))
+    sp = (d / np.linalg.norm(d, axis=0) * rng.uniform(0.95, 2.0, 90)).astype(np.float32)
+    ss = (rng.normal(size=(3, 90)) * 0.02).astype(np.float32)
+    for core in CORES:
+        for mode in MODES:
+            a = oracle.ptpan(core, sp, ss, pan.nodes, pan.idx, pan.area, mode)
+            b = oracle.ref_ptpan(core, sp, ss, pan.nodes, pan.idx, pan.area, mode)
+            np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("core", CORES)
+def test_particles_on_panels_far_field_is_point_vortex_velocity(core):
+    """A particle more than 4 sqrt(area) from a panel's centroid moves the panel at the
+    point-vortex velocity it induces at the centroid: u = alpha x (c - p) / |c - p|^3
+    (the singular kernel; the reference's 1/(4 pi) comes later)."""
+    rng = np.random.default_rng(12)
+    nodes, idx, _, area, _ = _single_panel(rng)
+    c = nodes.mean(axis=1)
+    d = rng.normal(size=(3, 5))
+    p = c[:, None] + d / np.linalg.norm(d, axis=0) * (4.5 * np.sqrt(area[0]) + rng.uniform(0, 1, 5))
+    a = rng.normal(size=(3, 5))
+    for j in range(5):
+        v = oracle.ptpan(core, p[:, j:j + 1], a[:, j:j + 1], nodes, idx, area, "f64")
+        dd = c - p[:, j]
+        want = np.cross(a[:, j], dd) / np.linalg.norm(dd) ** 3
+        np.testing.assert_allclose(v[:, 0], want, rtol=1e-12)
+
+
+def test_particles_on_panels_is_panel_kernel_backwards():
+    """Particle j on panel i equals minus the panel (carrying alpha_j) on the point p_j,
+    operation for operation, at every recursion depth."""
+    rng = np.random.default_rng(13)
+    nodes, idx, _, area, _ = _single_panel(rng)
+    c = nodes.mean(axis=1)
+    pts = c[:, None] + rng.normal(size=(3, 40)) * np.geomspace(0.005, 2.0, 40)
+    for mode in MODES:
+        for j in range(40):
+            a = rng.normal(size=(3, 1))
+            v = oracle.ptpan("rm", pts[:, j:j + 1], a, nodes, idx, area, mode)
+            w, _ = oracle.pan("rm", nodes, idx, a, area, None, pts[:, j:j + 1], False, mode)
+            np.testing.assert_array_equal(v, -w)
