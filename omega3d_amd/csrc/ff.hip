@@ -192,7 +192,7 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
     a.ns = ns; a.nt = nt;
     a.tile = T; a.mwords = mwords; a.ntiles = ntiles;
     a.self_excl = self_excl ? 1 : 0;
-    const size_t lds_far = (size_t)256 * sizeof(float4) + (size_t)2 * 32 * FF_TP * sizeof(_Float16);
+    const size_t lds_far = (size_t)256 * FF_SL * sizeof(__bf16) + (size_t)2 * 32 * FF_TP * sizeof(_Float16);
     if (prof) HIPCHK(hipEventRecord(ev[1].e, st));
     hipLaunchKernelGGL((k_ff_far<CORE, THICK, PREC>), dim3((unsigned)nb), dim3(FF_FAR_THREADS), lds_far, st, a);
     HIPCHK(hipGetLastError());

@@ -41,6 +41,7 @@ namespace o3d {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int FF_TB = 256;          // targets per workgroup: 4 waves x 64
 constexpr float FF_KAPPA = 2.0f;    // minimum gap, in block radii
@@ -48,6 +49,7 @@ constexpr int FF_FOLD = 16;         // near field, compensated: sources per FP32
 constexpr int FF_M0 = 16;           // near field: fewest tiles per work item
 constexpr int FF_FAR_THREADS = 256; // far field: 4 waves of 64 targets per block
 constexpr int FF_TP = 264;          // far field: LDS row of the FP16 moment tables (halves, tile <= 256)
+constexpr int FF_SL = 40;           // far field: LDS row of a source's BF16 r2 slots (32 used, 80 bytes)
 
 struct FFArgs {
   const float4* sp;    // sorted sources {x, y, z, core_term}
@@ -247,12 +249,8 @@ __global__ __launch_bounds__(256) void k_ff_classify(const float4* bsph, const f
   mask[g] = bits;
 }
 
-// pair weight Bp = r2^-5/2 of a far pair.  r2 = a + b - 2 t'.s' with a = |t'|^2 + st^2
-// per target (m2t = -2 t') and b = |s'|^2 + ss^2 per source (P.w): for far pairs
-// |t'| <= |d| / 2, so the three terms are at most about 4 |d|^2 and r2 keeps a few
-// ulp of relative accuracy.
-__device__ __forceinline__ float bp_far(float m2tx, float m2ty, float m2tz, float al, const float4 P) {
-  const float r2 = fmaf(m2tz, P.z, fmaf(m2ty, P.y, fmaf(m2tx, P.x, al + P.w)));
+// pair weight Bp = r2^-5/2 from a (scaled) squared distance
+__device__ __forceinline__ float bp_of(float r2) {
   const float ir = frsq(r2);
   const float ir2 = ir * ir;
   return ir2 * ir2 * ir;
@@ -321,34 +319,66 @@ __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(x - h);
 }
 
+// BF16 three-way split x = hi + mid + lo (about 24 significant bits): hi and mid
+// take the leading bits, each remainder is exact in FP32
+__device__ __forceinline__ void split_bf3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
+  hi = (__bf16)x;
+  const float r = x - (float)hi;
+  mid = (__bf16)r;
+  lo = (__bf16)(r - (float)mid);
+}
+
+// squared distances on the matrix cores: r2 = sum_c S_c T_c over the five terms
+// S = (-2 s'x, -2 s'y, -2 s'z, b, 1) (per source) and T = (t'x, t'y, t'z, 1, a) (per
+// target), both pre-scaled by the tile's r2 scale, each split into BF16 hi/mid/lo
+// and multiplied as hi hi + hi mid + mid hi + hi lo + lo hi + mid mid (about 24 bits:
+// the accuracy of the FP32 expression it replaces).  Slot q < 30 of the K = 32
+// product holds term q / 5 of component q % 5; the parts taken per term:
+__device__ __forceinline__ int ff_src_part(int term) { return term == 2 || term == 5 ? 1 : (term == 4 ? 2 : 0); }
+__device__ __forceinline__ int ff_tgt_part(int term) { return term == 1 || term == 5 ? 1 : (term == 3 ? 2 : 0); }
+
+// source position of tile slot p in the FP16 moment tables: inside each group of 16
+// sources, slot 16g + 8h + e holds source 16g + 4h + (e % 4) + 8 (e / 4), so that the
+// squared distances a lane gets from the r2 product (rows 8(r/4) + 4h + r%4) are, in
+// register order, the sources of its contiguous 8 K-slots in the moment product.
+__device__ __forceinline__ int ff_perm(int j) {
+  const int g = j & ~15, s = j & 15;
+  return g + 8 * ((s >> 2) & 1) + (s & 3) + 4 * (s >> 3);
+}
+
 // Far field: one workgroup per 256 sorted targets (4 waves of 64); lane l of wave w
-// owns target w*64 + l at the end, and during the sums computes the pair weights
-// of targets w*64 + (l & 31) (set 0) and w*64 + 32 + (l & 31) (set 1) with sources
-// k0 + 8 (l >> 5) + e, e = 0..7.
+// owns target w*64 + l at the end, and during the sums works for targets
+// w*64 + (l & 31) (set 0) and w*64 + 32 + (l & 31) (set 1).  Per chunk of 32 sources
+// and set: two BF16 products give the 32 x 32 squared distances (lane l: target
+// l & 31, 16 sources), the lane turns its 16 into pair weights Bp (one v_rsq_f32,
+// three products) split into FP16 hi + lo, and six FP16 products (two 16-source
+// K blocks, three split terms each) add the weights times the 29 moment columns.
 //
-// The products run on the FP16 matrix cores with FP32 accumulation, each factor
-// split into hi + lo halves and three products kept (hi hi + lo hi + hi lo): about
-// 22 significant bits per product, the accuracy of the FP32 products it replaces,
-// at 3/16 of their matrix cost.  Per tile, pair weights are scaled by 2^(5 kw) (from
-// Bp <= gap^-5), applied for free as r2 -> r2 2^(-2 kw) through the staged source
-// terms and the targets' a, and each moment group by 2^eg (from the tile's largest
-// |alpha| and source distance), so that every factor stays below 2^14; the scales
-// are removed when the tile's FP32 sums are folded into FP64.
+// The moment products keep about 22 significant bits per product (hi hi + lo hi +
+// hi lo), the accuracy of the FP32 products they replace.  Per tile, pair weights
+// are scaled by 2^(5 kw) (from Bp <= gap^-5), applied as r2 -> r2 2^(-2 kw) through
+// the staged source terms and the targets' a, and each moment group by 2^eg (from
+// the tile's largest |alpha| and source distance), so that every FP16 factor stays
+// below 2^14; the scales are removed when the tile's FP32 sums are folded into FP64.
 //
-// v_mfma_f32_32x32x16_f16 operand layout (A: 32 x 16, B: 16 x 32, D: 32 x 32):
+// 32x32x16 FP16/BF16 matrix operand layout (A: 32 x 16, B: 16 x 32, D: 32 x 32):
 //   A[i][k] in lane i + 32 (k / 8), element k % 8;  B[k][j] in lane j + 32 (k / 8),
 //   element k % 8;  D[i][j] in lane j + 32 ((i / 4) % 2), register (i % 4) + 4 (i / 8).
-// A = moment table transposed (column i of source k), B = Bp (source k, target j).
-// LDS holds the scaled moments as FP16 hi and lo tables [column][source], rows of
-// FF_TP = 264 halves (132 dwords) so that the 16-byte reads of 16 lanes cover all
-// 64 banks.
+// r2 product: A = source slots (row = source), B = target slots, D = r2.
+// Moment product: A = moment table transposed (row = column), B = Bp (source, target).
+// LDS: r2 source slots [source][FF_SL] BF16 (rows of 80 bytes), and the scaled
+// moments as FP16 hi and lo tables [column][FF_TP], rows of 264 halves, so that the
+// 16-byte reads of 16 lanes cover all 64 banks.  Each tile's matrix sums are folded
+// at once through the FP64 epilogue into the target's 12 results, which keeps the
+// kernel within 256 registers (two waves per SIMD) with every matrix result in
+// ordinary VGPRs.
 template <int CORE, bool THICK, int PREC>
-__global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
+__global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
   extern __shared__ float4 sh[];
   const int T = a.tile;
   constexpr int TP = FF_TP;
-  float4* pos = sh;
-  _Float16* mh = reinterpret_cast<_Float16*>(sh + 256);
+  __bf16* sl = reinterpret_cast<__bf16*>(sh);
+  _Float16* mh = reinterpret_cast<_Float16*>(sl + 256 * FF_SL);
   _Float16* ml = mh + 32 * TP;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int half = lane >> 5, col = lane & 31;
@@ -356,7 +386,7 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
   const int64_t base = blk * FF_TB + w * 64;
   const float4 cb = a.bsph[blk];
 
-  float tx[2], ty[2], tz[2], tt[2], mx[2], my[2], mz[2], al[2];
+  float tx[2], ty[2], tz[2], tt[2], al[2];
 #pragma unroll
   for (int s = 0; s < 2; ++s) {
     int64_t i = base + 32 * s + col;
@@ -364,15 +394,17 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
     const float4 q = a.tq[i];
     tx[s] = q.x - cb.x; ty[s] = q.y - cb.y; tz[s] = q.z - cb.z;
     tt[s] = CORE == CORE_RM ? q.w : 0.0f;
-    mx[s] = -2.0f * tx[s]; my[s] = -2.0f * ty[s]; mz[s] = -2.0f * tz[s];
     al[s] = fmaf(tz[s], tz[s], fmaf(ty[s], ty[s], fmaf(tx[s], tx[s], tt[s])));
   }
   f32x16 acc0, acc1;
 #pragma unroll
   for (int r = 0; r < 16; ++r) { acc0[r] = 0.0f; acc1[r] = 0.0f; }
-  double raw0[16], raw1[16];
+  // lane l < 32 owns target w*64 + l (set 0), lane l >= 32 target w*64 + l (set 1)
+  const double ox = half ? tx[1] : tx[0], oy = half ? ty[1] : ty[0], oz = half ? tz[1] : tz[0];
+  const double oa = ox * ox + oy * oy + oz * oz + (double)(half ? tt[1] : tt[0]);
+  double sum[12];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) { raw0[r] = 0.0; raw1[r] = 0.0; }
+  for (int k = 0; k < 12; ++k) sum[k] = 0.0;
 
   const unsigned* mrow = a.mask + blk * a.mwords;
   for (int ti = 0; ti < a.ntiles; ++ti) {
@@ -389,7 +421,6 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
     const int kw = (int)floor((14 - ff_exp2_above(1.0 / (g2 * g2 * gap))) / 5.0);
     const int ew = 5 * kw;
     const float rsc = ldexpf(1.0f, -2 * kw);
-    const float al0 = al[0] * rsc, al1 = al[1] * rsc;
     const double rs = dc + ct.w;                           // |s'| bound
     const double am = (double)a.tam[ti] + 1e-300;
     const double bm = rs * rs + (CORE == CORE_RM ? (double)a.tcs[ti] : 0.0);
@@ -403,10 +434,33 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
 #pragma unroll
     for (int g = 0; g < 5; ++g) gs[g] = ldexpf(1.0f, eg[g]);
 
+    // this lane's target slots for both sets and both r2 products (K slots 8 half + e
+    // and 16 + 8 half + e)
+    bf16x8 tb[2][2];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const float tv[5] = {tx[s], ty[s], tz[s], 1.0f, al[s] * rsc};
+      __bf16 part[3][5];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) split_bf3(tv[c], part[0][c], part[1][c], part[2][c]);
+#pragma unroll
+      for (int m = 0; m < 2; ++m)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int q0 = 16 * m + e, q1 = 16 * m + 8 + e;     // slot for half 0 / half 1
+          const __bf16 v0 = q0 < 30 ? part[ff_tgt_part(q0 / 5)][q0 % 5] : (__bf16)0.0f;
+          const __bf16 v1 = q1 < 30 ? part[ff_tgt_part(q1 / 5)][q1 % 5] : (__bf16)0.0f;
+          tb[s][m][e] = half ? v1 : v0;
+        }
+    }
+
     __syncthreads();
     for (int j = tid; j < T; j += FF_FAR_THREADS) {
-      // each column is scaled, split and stored as soon as it is formed
-      auto put = [&](int c, float x) { split16(x * gs[ff_group(c)], mh[c * TP + j], ml[c * TP + j]); };
+      // each moment column is scaled, split and stored as soon as it is formed
+      const int pj = ff_perm(j);
+      auto put = [&](int c, float x) { split16(x * gs[ff_group(c)], mh[c * TP + pj], ml[c * TP + pj]); };
+      __bf16* srow = sl + j * FF_SL;
+      float sv5[5];
       if (j < cnt) {
         const float4 P = a.sp[s0 + j];
         const float4 A = a.sa[s0 + j];
@@ -416,7 +470,8 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
         const float m1 = fmaf(A.z, sx, -A.x * sz);
         const float m2 = fmaf(A.x, sy, -A.y * sx);
         const float b = fmaf(sz, sz, fmaf(sy, sy, fmaf(sx, sx, cw)));
-        pos[j] = make_float4(sx * rsc, sy * rsc, sz * rsc, b * rsc);   // exact: rsc is a power of 2
+        sv5[0] = -2.0f * sx * rsc; sv5[1] = -2.0f * sy * rsc; sv5[2] = -2.0f * sz * rsc;   // exact scalings
+        sv5[3] = b * rsc; sv5[4] = 1.0f;
         const float sv[3] = {sx, sy, sz}, av[3] = {A.x, A.y, A.z}, mv[3] = {m0, m1, m2};
 #pragma unroll
         for (int k = 0; k < 3; ++k) { put(k, av[k]); put(3 + k, mv[k]); }
@@ -431,65 +486,77 @@ __global__ __launch_bounds__(FF_FAR_THREADS) void k_ff_far(FFArgs a) {
 #pragma unroll
         for (int c = 29; c < 32; ++c) put(c, 0.0f);
       } else {
-        // padding: far away with zero moments, so that Bp * moment = 0
-        pos[j] = make_float4(1e18f, 1e18f, 1e18f, 3e36f);
+        // padding: zero moments and a large finite distance, so that Bp * moment = 0
+        sv5[0] = 0.0f; sv5[1] = 0.0f; sv5[2] = 0.0f; sv5[3] = 1e20f; sv5[4] = 1.0f;
 #pragma unroll
         for (int c = 0; c < 32; ++c) put(c, 0.0f);
       }
+      __bf16 part[3][5];
+#pragma unroll
+      for (int c = 0; c < 5; ++c) split_bf3(sv5[c], part[0][c], part[1][c], part[2][c]);
+#pragma unroll
+      for (int q = 0; q < 32; ++q) srow[q] = q < 30 ? part[ff_src_part(q / 5)][q % 5] : (__bf16)0.0f;
     }
     __syncthreads();
-    for (int k0 = 0; k0 < T; k0 += 16) {
-      const int kb = k0 + 8 * half;
-      const f16x8 Ah = *reinterpret_cast<const f16x8*>(mh + col * TP + kb);
-      const f16x8 Al = *reinterpret_cast<const f16x8*>(ml + col * TP + kb);
-      f16x8 Bh0, Bl0, Bh1, Bl1;
+    for (int k0 = 0; k0 < T; k0 += 32) {
+      // r2 of sources k0 + 0..31 with this lane's two targets
+      const __bf16* srow = sl + (k0 + col) * FF_SL + 8 * half;
+      const bf16x8 sa0 = *reinterpret_cast<const bf16x8*>(srow);
+      const bf16x8 sa1 = *reinterpret_cast<const bf16x8*>(srow + 16);
+      f32x16 r20 = {}, r21 = {};
+      r20 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sa0, tb[0][0], r20, 0, 0, 0);
+      r21 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sa0, tb[1][0], r21, 0, 0, 0);
+      r20 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sa1, tb[0][1], r20, 0, 0, 0);
+      r21 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(sa1, tb[1][1], r21, 0, 0, 0);
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const float4 P = pos[kb + e];
-        _Float16 h, l;
-        split16(bp_far(mx[0], my[0], mz[0], al0, P), h, l);
-        Bh0[e] = h; Bl0[e] = l;
-        split16(bp_far(mx[1], my[1], mz[1], al1, P), h, l);
-        Bh1[e] = h; Bl1[e] = l;
+      for (int kb = 0; kb < 2; ++kb) {
+        const int pbase = k0 + 16 * kb + 8 * half;
+        const f16x8 Ah = *reinterpret_cast<const f16x8*>(mh + col * TP + pbase);
+        const f16x8 Al = *reinterpret_cast<const f16x8*>(ml + col * TP + pbase);
+        f16x8 Bh0, Bl0, Bh1, Bl1;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          _Float16 hh, ll;
+          split16(bp_of(r20[8 * kb + e]), hh, ll);
+          Bh0[e] = hh; Bl0[e] = ll;
+          split16(bp_of(r21[8 * kb + e]), hh, ll);
+          Bh1[e] = hh; Bl1[e] = ll;
+        }
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh1, acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl1, acc1, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh0, acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh1, acc1, 0, 0, 0);
       }
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh1, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bl1, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh0, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh1, acc1, 0, 0, 0);
     }
-    // fold, removing the scales: row of register r in this lane's half
+    // fold this tile: one v_permlane32_swap per register exchanges the upper 32 lanes
+    // of acc0[r] with the lower 32 of acc1[r], after which every lane holds its own
+    // target's rows of half 0 in acc0[r] and of half 1 in acc1[r] (row of register r
+    // in half h: 8 (r / 4) + 4 h + r % 4); the scales are removed and the epilogue
+    // adds this tile's velocity and gradient in FP64
+    {
+      double c[32];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int row = 8 * (r / 4) + 4 * half + (r % 4);
-      const double un = ldexp(1.0, -(ew + eg[ff_group(row)]));
-      raw0[r] += (double)acc0[r] * un; acc0[r] = 0.0f;
-      raw1[r] += (double)acc1[r] * un; acc1[r] = 0.0f;
+      for (int r = 0; r < 16; ++r) {
+        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]),
+                                                         false, false);
+        const int i0 = 8 * (r / 4) + (r % 4);
+        c[i0] = (double)__uint_as_float(sw[0]) * ldexp(1.0, -(ew + eg[ff_group(i0)]));
+        c[i0 + 4] = (double)__uint_as_float(sw[1]) * ldexp(1.0, -(ew + eg[ff_group(i0 + 4)]));
+        acc0[r] = 0.0f;
+        acc1[r] = 0.0f;
+      }
+      double out[12];
+      ff_epilogue(c, ox, oy, oz, oa, out);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) sum[k] += out[k];
     }
-  }
-
-  // lane l < 32 owns target w*64 + l (set 0), lane l >= 32 target w*64 + l (set 1);
-  // each holds the rows of its half (l >> 5) and gets the other half from lane l ^ 32
-  const int h = half;
-  double c[32];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const double keep = h ? raw1[r] : raw0[r];
-    const double give = h ? raw0[r] : raw1[r];
-    const double got = __shfl_xor(give, 32);
-    const int i0 = 8 * (r / 4) + (r % 4);       // row held by half 0
-    c[i0] = h ? got : keep;
-    c[i0 + 4] = h ? keep : got;
   }
   const int64_t i = base + lane;
   if (i < a.nt) {
-    const double dx = h ? tx[1] : tx[0], dy = h ? ty[1] : ty[0], dz = h ? tz[1] : tz[0];
-    const double a2 = dx * dx + dy * dy + dz * dz + (double)(h ? tt[1] : tt[0]);
-    double out[12];
-    ff_epilogue(c, dx, dy, dz, a2, out);
 #pragma unroll
-    for (int k = 0; k < 12; ++k) a.farv[(int64_t)k * a.nt + i] = out[k];
+    for (int k = 0; k < 12; ++k) a.farv[(int64_t)k * a.nt + i] = sum[k];
   }
 }
 
