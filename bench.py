@@ -254,7 +254,7 @@ def main():
                                "what": "reference flops of all N^2 pairs over the whole evaluation (sort, far, "
                                        "near)"},
                 "note": "achieved counts the reference's 68 FP32 flops per far pair; the kernel executes an exact "
-                        "factorisation instead (about 12 VALU operations per pair plus FP16 hi/lo matrix products), "
+                        "factorisation instead (about 8 VALU operations per pair plus BF16/FP16 split matrix products), "
                         "so it can exceed the FP32 peak; 'executed' gives the utilisation of the units it runs on",
                 "executed": executed}
     else:
