@@ -238,7 +238,13 @@ int o3d_flops_per_interaction(int core, int results, int thick_target, int vorte
 typedef struct o3d_batch o3d_batch;
 
 /* cores / precs / self_terms: per-instance arrays (NULL = the opts value for all).
- * opts->tile / opts->tpt: 0 = defaults (see o3d_batch_tune). */
+ * opts->tile / opts->tpt: 0 = defaults (see o3d_batch_tune).
+ * opts->method: AUTO sends every instance of >= 65,536 particles with an RM or EXP
+ * core in FP32 or compensated precision to the factored far-field method; FACTORED
+ * sends all instances there (an instance it cannot take is an error); DIRECT keeps
+ * all in the persistent queue.  Factored instances run on up to 4 streams the
+ * batch creates here, each with scratch allocated here; they fork from the
+ * caller's stream after the zero fill and the caller's stream waits for them. */
 int o3d_batch_create(o3d_batch** b, int32_t ninst, const int64_t* off, const int64_t* n,
                      const int32_t* cores, const int32_t* precs, const int32_t* self_terms,
                      const o3d_opts* opts);
@@ -246,19 +252,26 @@ int o3d_batch_destroy(o3d_batch* b);
 /* Zero the instances' outputs and evaluate every instance, on `stream`.
  * all: the shared device arrays (x, s, r, u, ug with all->n = total particles). */
 int o3d_batch_run(o3d_batch* b, const o3d_points* all, void* stream);
-/* The same work as one ordinary launch per instance (the comparison baseline). */
+/* The same work as one ordinary call per instance, one after another on `stream`
+ * (the comparison baseline). */
 int o3d_batch_run_separate(o3d_batch* b, const o3d_points* all, void* stream);
-/* Capture o3d_batch_run into a HIP graph (zero-fill kernel, queue reset kernel,
- * persistent kernel) and replay it; the graph reads the queue from device memory. */
+/* Capture o3d_batch_run's queue part into a HIP graph (zero-fill kernel, queue
+ * reset kernel, persistent kernel) and replay it; the graph reads the queue from
+ * device memory.  A replay launches the graph, then enqueues the factored
+ * instances (if any) on the same stream with the arrays given at capture: their
+ * radix sorts record memory fills, which are kept out of graphs. */
 int o3d_batch_capture(o3d_batch* b, const o3d_points* all, void* stream);
 int o3d_batch_replay(o3d_batch* b, void* stream);
-/* Autotuner: time every (tile, tpt) candidate `reps` times on the real batch and
- * keep the fastest in b; reports it and its mean time.  NULL lists = the default
- * candidates tile {128, 256, 512} x tpt {1, 2, 4}. */
+/* Autotuner: time every (tile, tpt) candidate `reps` times on the real batch's
+ * queue part and keep the fastest in b; reports it and its mean time.  NULL lists =
+ * the default candidates tile {128, 256, 512} x tpt {1, 2, 4}.  The outputs hold
+ * the full evaluation afterwards. */
 int o3d_batch_tune(o3d_batch* b, const o3d_points* all, void* stream,
                    const int32_t* tiles, int32_t ntiles, const int32_t* tpts, int32_t ntpts,
                    int32_t reps, int32_t* best_tile, int32_t* best_tpt, float* best_ms);
 int o3d_batch_config(const o3d_batch* b, int32_t* tile, int32_t* tpt, int32_t* grid, int64_t* items);
+/* How many instances the factored method evaluates, and the scratch they share. */
+int o3d_batch_factored(const o3d_batch* b, int32_t* ninst, int64_t* scratch_bytes);
 
 /* Autotuner for single-collection calls: time the candidates on the given device
  * collection and record the winner for (core, results, precision, target kind,
@@ -293,6 +306,13 @@ typedef struct o3d_server_opts {
   int32_t use_graph;          /* replay each tick from a captured HIP graph            */
   int32_t adaptive_tick;      /* raise the budget to (workgroup slots x the largest
                                  active chunk) so that large sessions fill the GPU      */
+  int32_t method;             /* enum o3d_method.  AUTO (0): a step of a session of
+                                 >= 65,536 particles with the RM or EXP core evaluates
+                                 both stages by the factored far-field method on a
+                                 second stream, overlapping the ticks; FACTORED: every
+                                 RM/EXP session; DIRECT: ticks only.  The factored
+                                 scratch is sized for the largest such session and is
+                                 held outside the arena                                */
   double tick_interactions;   /* pair-evaluation budget of one tick (0 = 4e10)         */
 } o3d_server_opts;
 
@@ -317,6 +337,8 @@ typedef struct o3d_server_stats {
   double ginter_per_s;              /* interactions / wall_s / 1e9                      */
   size_t hbm_used, hbm_budget, hbm_peak;
   int64_t ticks;
+  int64_t factored_steps;           /* steps evaluated by the factored method           */
+  size_t factored_scratch;          /* bytes of its scratch (outside the arena)         */
 } o3d_server_stats;
 
 int o3d_server_create(o3d_server** s, const o3d_server_opts* opts);

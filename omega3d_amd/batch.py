@@ -23,7 +23,9 @@ lib.o3d_batch_tune.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C
                                C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_float)]
 lib.o3d_batch_config.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
                                  C.POINTER(C.c_int64)]
+lib.o3d_batch_factored.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)]
 for _n in ("o3d_batch_create", "o3d_batch_destroy", "o3d_batch_replay", "o3d_batch_tune", "o3d_batch_config",
+           "o3d_batch_factored",
            "o3d_tune_p2p", "o3d_tune_save", "o3d_tune_load"):
     getattr(lib, _n).restype = C.c_int
 lib.o3d_tune_save.argtypes = [C.c_char_p]
@@ -32,7 +34,7 @@ lib.o3d_tune_load.argtypes = [C.c_char_p]
 
 class Batch:
     def __init__(self, instances, core="rm", precision="comp", self_term=0, tile=0, tpt=0,
-                 cores=None, precisions=None, self_terms=None):
+                 cores=None, precisions=None, self_terms=None, method="auto"):
         self.counts = np.array([p.n for p in instances], np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(self.counts)[:-1]]).astype(np.int64)
         self.total = int(self.counts.sum())
@@ -52,7 +54,7 @@ class Batch:
         self.all.r = self.drad.ptr
         for k in range(9):
             self.all.ug[k] = self.dgrad.row(k)
-        o = _lib.make_opts(core, True, precision, self_term, tile, tpt)
+        o = _lib.make_opts(core, True, precision, self_term, tile, tpt, method=method)
         k = len(instances)
         arr = lambda v, f: None if v is None else np.ascontiguousarray([f(x) for x in v], np.int32)
         self._cores = arr(cores, lambda c: CORES[c] if isinstance(c, str) else int(c))
@@ -87,7 +89,10 @@ class Batch:
     def config(self):
         t, p, g, n = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int64()
         check(lib.o3d_batch_config(self.h, C.byref(t), C.byref(p), C.byref(g), C.byref(n)))
-        return {"tile": t.value, "tpt": p.value, "grid": g.value, "items": n.value}
+        nf, wb = C.c_int32(), C.c_int64()
+        check(lib.o3d_batch_factored(self.h, C.byref(nf), C.byref(wb)))
+        return {"tile": t.value, "tpt": p.value, "grid": g.value, "items": n.value,
+                "factored_instances": nf.value, "factored_scratch_bytes": wb.value}
 
     def results(self):
         v = self.dvel.to_host()[:, : self.total]

@@ -6,6 +6,16 @@
 // persistent grid: each workgroup takes the next item from a device counter.  An
 // item runs the same p2p_chunk as the single-collection kernel, so a target's
 // sums do not depend on the launch that computed it.
+//
+// Instances large enough for the factored far field (the same rule as a single
+// call with method AUTO, or all of them with method FACTORED) leave the queue and
+// run that pipeline on FF_LANES streams of the batch (longest first, each to the
+// least-loaded lane), each lane with scratch of its own.  The lanes fork from the
+// caller's stream after the zero fill and join it at the end, so they overlap
+// one another and the persistent kernel: an instance of 65,536 particles has
+// only 256 far-field workgroups, half the GPU's slots.  The pipeline's sorts
+// record memory fills, which a graph must not hold, so a captured graph covers
+// the queue part and a replay enqueues the factored instances after it.
 #include "o3d_api_internal.hpp"
 #include "p2p_kernels.hpp"
 
@@ -109,11 +119,24 @@ struct o3d_batch {
   BatchItem* d_items[3] = {nullptr, nullptr, nullptr};  // per tpt 1, 2, 4
   int32_t nitems[3] = {0, 0, 0};
   unsigned int* d_counter = nullptr;
+  std::vector<int> ff;     // instances evaluated by the factored method, largest first
+  struct Lane {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    void* ws = nullptr;      // reused instance after instance (stream order)
+    size_t bytes = 0;
+    std::vector<int> inst;
+  };
+  std::vector<Lane> lanes; // lane 0 holds the largest instance, so its scratch fits any
+  hipEvent_t fork = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  o3d_points cap_all{};    // the arrays the captured graph is bound to
 };
 
 namespace {
+
+constexpr size_t FF_LANES = 4;
 
 int tpt_slot(int tpt) { return tpt == 1 ? 0 : (tpt == 2 ? 1 : 2); }
 
@@ -164,14 +187,93 @@ BatchArgs args_of(o3d_batch* b, const o3d_points* all) {
   return a;
 }
 
-int enqueue_run(o3d_batch* b, const o3d_points* all, hipStream_t st) {
-  BatchArgs a = args_of(b, all);
+// instance I's slice of the shared arrays, as one collection
+o3d_points slice(const o3d_points* all, const BatchInst& I) {
+  o3d_points p = *all;
+  p.n = I.n;
+  for (int d = 0; d < 3; ++d) {
+    p.x[d] = all->x[d] + I.off;
+    p.s[d] = all->s[d] + I.off;
+    p.u[d] = all->u[d] + I.off;
+  }
+  p.r = all->r + I.off;
+  for (int k = 0; k < 9; ++k) p.ug[k] = all->ug[k] + I.off;
+  return p;
+}
+
+o3d_opts factored_opts(const BatchInst& I) {
+  o3d_opts o{};
+  o.core = I.core;
+  o.results = O3D_VELANDGRAD;
+  o.precision = I.prec;
+  o.self_term = I.self_excl;
+  o.method = O3D_METHOD_FACTORED;
+  return o;
+}
+
+void zero_outputs(o3d_batch* b, const TgtView& t, hipStream_t st) {
   const int zb = (int)std::min<int64_t>((b->total_n + BLOCK - 1) / BLOCK, 4096);
-  if (b->total_n > 0) hipLaunchKernelGGL(k_zero_outputs, dim3(std::max(zb, 1)), dim3(BLOCK), 0, st, a.t, b->total_n);
+  if (b->total_n > 0) hipLaunchKernelGGL(k_zero_outputs, dim3(std::max(zb, 1)), dim3(BLOCK), 0, st, t, b->total_n);
+}
+
+// zero fill, queue reset and the persistent kernel: kernels only, so it can be captured
+int enqueue_queue(o3d_batch* b, const o3d_points* all, hipStream_t st) {
+  BatchArgs a = args_of(b, all);
+  zero_outputs(b, a.t, st);
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
   HIPCHK(hipGetLastError());
   if (a.nitems > 0) HIPCHK(batch_launch_kernel(b, a, st));
   return O3D_OK;
+}
+
+int factored_one(o3d_batch* b, const o3d_points* all, int i, hipStream_t st, void* ws, size_t bytes) {
+  const BatchInst& I = b->inst[i];
+  o3d_points p = slice(all, I);
+  const o3d_opts o = factored_opts(I);
+  return p2p_factored_ws(&p, &p, &o, st, ws, bytes);
+}
+
+// the factored instances on the lanes, accumulating into outputs zeroed on st
+// before this call; st waits for them at join_factored
+int fork_factored(o3d_batch* b, const o3d_points* all, hipStream_t st) {
+  if (b->ff.empty()) return O3D_OK;
+  HIPCHK(hipEventRecord(b->fork, st));
+  for (auto& L : b->lanes) {
+    if (L.inst.empty()) continue;
+    HIPCHK(hipStreamWaitEvent(L.st, b->fork, 0));
+    for (int i : L.inst) {
+      const int rc = factored_one(b, all, i, L.st, L.ws, L.bytes);
+      if (rc != O3D_OK) return rc;
+    }
+    HIPCHK(hipEventRecord(L.done, L.st));
+  }
+  return O3D_OK;
+}
+
+int join_factored(o3d_batch* b, hipStream_t st) {
+  for (auto& L : b->lanes)
+    if (!L.inst.empty()) HIPCHK(hipStreamWaitEvent(st, L.done, 0));
+  return O3D_OK;
+}
+
+// the factored instances one after another on st (the per-instance baseline)
+int factored_serial(o3d_batch* b, const o3d_points* all, hipStream_t st) {
+  for (int i : b->ff) {
+    const int rc = factored_one(b, all, i, st, b->lanes[0].ws, b->lanes[0].bytes);
+    if (rc != O3D_OK) return rc;
+  }
+  return O3D_OK;
+}
+
+int enqueue_run(o3d_batch* b, const o3d_points* all, hipStream_t st) {
+  BatchArgs a = args_of(b, all);
+  zero_outputs(b, a.t, st);
+  int rc = fork_factored(b, all, st);
+  if (rc != O3D_OK) return rc;
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, st, b->d_counter);
+  HIPCHK(hipGetLastError());
+  if (a.nitems > 0) HIPCHK(batch_launch_kernel(b, a, st));
+  return join_factored(b, st);
 }
 
 void drop_graph(o3d_batch* b) {
@@ -245,6 +347,12 @@ int o3d_batch_create(o3d_batch** out, int32_t ninst, const int64_t* off, const i
       delete b;
       return fail(O3D_EINVAL, "instance %d too large", i);
     }
+    const bool ffok = ff_supported(I.core, true, I.prec) && I.n <= INT32_MAX;
+    if (opts->method == O3D_METHOD_FACTORED && !ffok) {
+      delete b;
+      return fail(O3D_EINVAL, "instance %d: the factored method needs the RM or EXP core and FP32 or compensated "
+                              "precision", i);
+    }
     b->total_n = std::max(b->total_n, I.off + I.n);
     b->inst.push_back(I);
   }
@@ -252,15 +360,48 @@ int o3d_batch_create(o3d_batch** out, int32_t ninst, const int64_t* off, const i
   std::vector<int> order(ninst);
   std::iota(order.begin(), order.end(), 0);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return b->inst[x].n > b->inst[y].n; });
+  std::vector<char> factored(ninst, 0);
+  for (int i : order) {
+    const BatchInst& I = b->inst[i];
+    const bool ffok = ff_supported(I.core, true, I.prec) && I.n <= INT32_MAX;
+    if (I.n > 0 && ffok &&
+        (opts->method == O3D_METHOD_FACTORED || (opts->method == O3D_METHOD_AUTO && I.n >= FF_AUTO_MIN_N))) {
+      factored[i] = 1;
+      b->ff.push_back(i);
+    }
+  }
   const int tpts[3] = {1, 2, 4};
   std::vector<BatchItem> items[3];
   for (int s = 0; s < 3; ++s) {
     const int64_t per = (int64_t)BLOCK * tpts[s];
     for (int i : order)
-      for (int64_t c = 0; c * per < b->inst[i].n; ++c) items[s].push_back(BatchItem{i, (int32_t)c});
+      if (!factored[i])
+        for (int64_t c = 0; c * per < b->inst[i].n; ++c) items[s].push_back(BatchItem{i, (int32_t)c});
     b->nitems[s] = (int32_t)items[s].size();
   }
   auto cleanup = [&](int code) { o3d_batch_destroy(b); return code; };
+  if (!b->ff.empty()) {
+    // longest first, each to the least-loaded lane (load = pairs)
+    b->lanes.resize(std::min<size_t>(FF_LANES, b->ff.size()));
+    std::vector<double> load(b->lanes.size(), 0.0);
+    for (int i : b->ff) {
+      const size_t k = (size_t)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[k] += (double)b->inst[i].n * (double)b->inst[i].n;
+      b->lanes[k].inst.push_back(i);
+      const size_t need = ff_workspace_bytes(b->inst[i].n, b->inst[i].n, true, 0);
+      if (need == 0) return cleanup(fail(O3D_EDEVICE, "instance %d: factored scratch query failed", i));
+      b->lanes[k].bytes = std::max(b->lanes[k].bytes, need);
+    }
+    if (hipEventCreateWithFlags(&b->fork, hipEventDisableTiming) != hipSuccess)
+      return cleanup(fail(O3D_EDEVICE, "event"));
+    for (auto& L : b->lanes) {
+      if (hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&L.done, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(O3D_EDEVICE, "factored lane stream"));
+      if (hipMalloc(&L.ws, L.bytes) != hipSuccess)
+        return cleanup(fail(O3D_ENOMEM, "factored scratch of %zu bytes", L.bytes));
+    }
+  }
   if (hipMalloc(&b->d_counter, sizeof(unsigned int)) != hipSuccess) return cleanup(fail(O3D_ENOMEM, "counter"));
   if (ninst > 0) {
     if (hipMalloc(&b->d_inst, sizeof(BatchInst) * ninst) != hipSuccess) return cleanup(fail(O3D_ENOMEM, "instances"));
@@ -288,6 +429,15 @@ int o3d_batch_destroy(o3d_batch* b) {
   if (b->d_inst) hipFree(b->d_inst);
   for (auto* p : b->d_items) if (p) hipFree(p);
   if (b->d_counter) hipFree(b->d_counter);
+  for (auto& L : b->lanes) {
+    if (L.st) {
+      hipStreamSynchronize(L.st);
+      hipStreamDestroy(L.st);
+    }
+    if (L.done) hipEventDestroy(L.done);
+    if (L.ws) hipFree(L.ws);
+  }
+  if (b->fork) hipEventDestroy(b->fork);
   delete b;
   return O3D_OK;
 }
@@ -298,6 +448,15 @@ int o3d_batch_config(const o3d_batch* b, int32_t* tile, int32_t* tpt, int32_t* g
   if (tpt) *tpt = b->tpt;
   if (grid) *grid = b->grid;
   if (items) *items = b->nitems[tpt_slot(b->tpt)];
+  return O3D_OK;
+}
+
+int o3d_batch_factored(const o3d_batch* b, int32_t* ninst, int64_t* scratch_bytes) {
+  if (!b) return fail(O3D_EINVAL, "NULL batch");
+  if (ninst) *ninst = (int32_t)b->ff.size();
+  size_t bytes = 0;
+  for (const auto& L : b->lanes) bytes += L.bytes;
+  if (scratch_bytes) *scratch_bytes = (int64_t)bytes;
   return O3D_OK;
 }
 
@@ -316,10 +475,14 @@ int o3d_batch_run_separate(o3d_batch* b, const o3d_points* all, void* stream) {
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   BatchArgs a = args_of(b, all);
-  const int zb = (int)std::min<int64_t>((b->total_n + BLOCK - 1) / BLOCK, 4096);
-  if (b->total_n > 0) hipLaunchKernelGGL(k_zero_outputs, dim3(std::max(zb, 1)), dim3(BLOCK), 0, st, a.t, b->total_n);
-  for (const BatchInst& I : b->inst) {
-    if (I.n == 0) continue;
+  zero_outputs(b, a.t, st);
+  // the factored instances one after another, each an ordinary single-collection call
+  if ((rc = factored_serial(b, all, st)) != O3D_OK) return rc;
+  std::vector<char> factored(b->inst.size(), 0);
+  for (int i : b->ff) factored[i] = 1;
+  for (size_t i = 0; i < b->inst.size(); ++i) {
+    const BatchInst& I = b->inst[i];
+    if (I.n == 0 || factored[i]) continue;
     P2PArgs p{};
     p.s = SrcView{a.s.x + I.off, a.s.y + I.off, a.s.z + I.off, a.s.r + I.off, a.s.ax + I.off, a.s.ay + I.off, a.s.az + I.off};
     p.t.x = a.t.x + I.off; p.t.y = a.t.y + I.off; p.t.z = a.t.z + I.off; p.t.r = a.t.r + I.off;
@@ -343,7 +506,7 @@ int o3d_batch_capture(o3d_batch* b, const o3d_points* all, void* stream) {
   drop_graph(b);
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  rc = enqueue_run(b, all, st);
+  rc = enqueue_queue(b, all, st);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (rc != O3D_OK) { if (g) hipGraphDestroy(g); return rc; }
@@ -351,6 +514,7 @@ int o3d_batch_capture(o3d_batch* b, const o3d_points* all, void* stream) {
   b->graph = g;
   e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
   if (e != hipSuccess) { drop_graph(b); return fail(O3D_EDEVICE, "graph instantiate: %s", hipGetErrorString(e)); }
+  b->cap_all = *all;
   return O3D_OK;
 }
 
@@ -358,7 +522,8 @@ int o3d_batch_replay(o3d_batch* b, void* stream) {
   clear_error();
   if (!b || !b->exec) return fail(O3D_EINVAL, "no captured graph");
   HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
-  return O3D_OK;
+  const int rc = fork_factored(b, &b->cap_all, (hipStream_t)stream);
+  return rc != O3D_OK ? rc : join_factored(b, (hipStream_t)stream);
 }
 
 int o3d_batch_tune(o3d_batch* b, const o3d_points* all, void* stream, const int32_t* tiles, int32_t ntiles,
@@ -381,9 +546,9 @@ int o3d_batch_tune(o3d_batch* b, const o3d_points* all, void* stream, const int3
   for (int i = 0; i < ntiles; ++i)
     for (int j = 0; j < ntpts; ++j) {
       if ((rc = set_config(b, tiles[i], tpts[j])) != O3D_OK) break;
-      if ((rc = enqueue_run(b, all, st)) != O3D_OK) break;   // warm-up
+      if ((rc = enqueue_queue(b, all, st)) != O3D_OK) break;   // warm-up
       HIPCHK(hipEventRecord(e0, st));
-      for (int r = 0; r < reps && rc == O3D_OK; ++r) rc = enqueue_run(b, all, st);
+      for (int r = 0; r < reps && rc == O3D_OK; ++r) rc = enqueue_queue(b, all, st);
       HIPCHK(hipEventRecord(e1, st));
       HIPCHK(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -395,6 +560,8 @@ int o3d_batch_tune(o3d_batch* b, const o3d_points* all, void* stream, const int3
   hipEventDestroy(e1);
   if (rc != O3D_OK) return rc;
   if ((rc = set_config(b, bt, bp)) != O3D_OK) return rc;
+  // the candidates only change the queue part; complete the outputs once
+  if ((rc = fork_factored(b, all, st)) != O3D_OK || (rc = join_factored(b, st)) != O3D_OK) return rc;
   if (best_tile) *best_tile = bt;
   if (best_tpt) *best_tpt = bp;
   if (best_ms) *best_ms = best;

@@ -8,7 +8,8 @@
 //   also adds the far part and accumulates into the caller's arrays.
 // Scratch lives in one device workspace per stream (calls on one stream are
 // ordered, so they can share it); it grows when a larger call comes, after the
-// stream has drained.
+// stream has drained.  A caller that owns its scratch (a batch) passes it in
+// instead, sized by ff_workspace_bytes.
 #include <hipcub/hipcub.hpp>
 
 #include "ff_kernels.hpp"
@@ -81,160 +82,198 @@ struct Ev {
   ~Ev() { if (e) (void)hipEventDestroy(e); }
 };
 
+bool same_points(const o3d_points* src, const o3d_points* targ) {
+  return src->n == targ->n && src->x[0] == targ->x[0] && src->x[1] == targ->x[1] && src->x[2] == targ->x[2];
+}
+
+// hipcub scratch for the sorts and scans of one call
+size_t lib_tmp_bytes(int64_t ns, int64_t nt, bool same, hipStream_t st) {
+  const int64_t nb = (nt + FF_TB - 1) / FF_TB;
+  size_t tb_s = 0, tb_t = 0, tb_scan = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb_s, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                         (const int*)nullptr, (int*)nullptr, (int)ns, 0, 30, st) != hipSuccess)
+    return 0;
+  if (!same && hipcub::DeviceRadixSort::SortPairs(nullptr, tb_t, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                                  (const int*)nullptr, (int*)nullptr, (int)nt, 0, 30, st) != hipSuccess)
+    return 0;
+  if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb_scan, (const int*)nullptr, (int*)nullptr, (int)nb, st) != hipSuccess)
+    return 0;
+  return std::max(std::max(tb_s, tb_t), tb_scan);
+}
+
+// the workspace's sub-arrays in order; with base == nullptr only .bytes is meaningful
+struct Layout {
+  unsigned *bb, *ks_in, *ks_out, *kt_in, *kt_out, *mask, *ctr;
+  int *is_in, *is_out, *it_in, *it_out, *nn, *noff, *items, *ioff, *mfold;
+  void* tmp;
+  float4 *sp, *sa, *tq, *tsph, *bsph;
+  float *tcs, *tam, *bcs;
+  double *farv, *part;
+  unsigned long long* npair;
+  size_t bytes;
+};
+
+Layout layout(char* base, int64_t ns, int64_t nt, bool same, int T, size_t tmp_bytes) {
+  const int64_t nb = (nt + FF_TB - 1) / FF_TB;
+  const int64_t ntiles = (ns + T - 1) / T;
+  const int64_t mwords = (ntiles + 31) / 32;
+  Carve cv{base};
+  Layout L{};
+  L.bb = cv.take<unsigned>(8);
+  L.ks_in = cv.take<unsigned>(ns);
+  L.ks_out = cv.take<unsigned>(ns);
+  L.is_in = cv.take<int>(ns);
+  L.is_out = cv.take<int>(ns);
+  L.kt_in = same ? nullptr : cv.take<unsigned>(nt);
+  L.kt_out = same ? nullptr : cv.take<unsigned>(nt);
+  L.it_in = same ? nullptr : cv.take<int>(nt);
+  L.it_out = same ? nullptr : cv.take<int>(nt);
+  L.tmp = cv.take<char>(tmp_bytes);
+  L.sp = cv.take<float4>(ns);
+  L.sa = cv.take<float4>(ns);
+  L.tq = cv.take<float4>(nt);
+  L.tsph = cv.take<float4>(ntiles);
+  L.tcs = cv.take<float>(ntiles);
+  L.tam = cv.take<float>(ntiles);
+  L.bsph = cv.take<float4>(nb);
+  L.bcs = cv.take<float>(nb);
+  L.mask = cv.take<unsigned>((size_t)(nb * mwords));
+  L.farv = cv.take<double>((size_t)12 * nt);
+  L.npair = cv.take<unsigned long long>(1);
+  L.nn = cv.take<int>(nb);
+  L.noff = cv.take<int>(nb);
+  L.items = cv.take<int>(nb);
+  L.ioff = cv.take<int>(nb);
+  L.mfold = cv.take<int>(1);
+  L.ctr = cv.take<unsigned>(1);
+  L.part = cv.take<double>((size_t)3 * nb * 12 * FF_TB);   // at most 3 items per block
+  L.bytes = cv.off;
+  return L;
+}
+
+// ws == nullptr: the stream's own workspace (caller holds g_ws_mu); otherwise the
+// caller's, of ws_bytes
 template <int CORE, bool THICK, int PREC>
-int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStream_t st, o3d_factored_profile* prof) {
+int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStream_t st, o3d_factored_profile* prof,
+        void* ws, size_t ws_bytes) {
   const int64_t ns = src->n, nt = targ->n;
-  const bool same = ns == nt && src->x[0] == targ->x[0] && src->x[1] == targ->x[1] && src->x[2] == targ->x[2];
+  const bool same = same_points(src, targ);
   const int64_t nb = (nt + FF_TB - 1) / FF_TB;
   const int ntiles = (int)((ns + T - 1) / T);
   const int mwords = (ntiles + 31) / 32;
 
-  size_t tb_s = 0, tb_t = 0;
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb_s, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                            (const int*)nullptr, (int*)nullptr, (int)ns, 0, 30, st));
+  const size_t tmp_bytes = lib_tmp_bytes(ns, nt, same, st);
+  if (tmp_bytes == 0) return fail(O3D_EDEVICE, "factored path: sort/scan scratch query failed");
+
+  Layout L = layout(nullptr, ns, nt, same, T, tmp_bytes);
+  char* base = static_cast<char*>(ws);
+  if (ws) {
+    if (ws_bytes < L.bytes) return fail(O3D_EINVAL, "factored path: workspace of %zu bytes, %zu needed", ws_bytes, L.bytes);
+  } else {
+    void* p = nullptr;
+    int rc = workspace(st, L.bytes, &p);
+    if (rc != O3D_OK) return rc;
+    base = static_cast<char*>(p);
+  }
+  L = layout(base, ns, nt, same, T, tmp_bytes);
+
+  SrcView sv{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
+  TgtView tv{};
+  tv.x = targ->x[0]; tv.y = targ->x[1]; tv.z = targ->x[2]; tv.r = targ->r;
+  tv.u = targ->u[0]; tv.v = targ->u[1]; tv.w = targ->u[2];
+  for (int k = 0; k < 9; ++k) tv.g[k] = targ->ug[k];
+
+  Ev ev[4];
+  if (prof) {
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e.e));
+    HIPCHK(hipEventRecord(ev[0].e, st));
+  }
+  hipLaunchKernelGGL(k_ff_bbox_init, dim3(1), dim3(64), 0, st, L.bb);
+  hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(ns, 256), 2048u)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, L.bb);
   if (!same)
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb_t, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                              (const int*)nullptr, (int*)nullptr, (int)nt, 0, 30, st));
-  size_t tb_scan = 0;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb_scan, (const int*)nullptr, (int*)nullptr, (int)nb, st));
-  size_t tmp_bytes = std::max(std::max(tb_s, tb_t), tb_scan);
-
-  // layout (first pass sizes it, second pass places it)
-  void* base = nullptr;
-  for (int pass = 0; pass < 2; ++pass) {
-    Carve cv{static_cast<char*>(base)};
-    unsigned* bb = cv.take<unsigned>(8);
-    unsigned* ks_in = cv.take<unsigned>(ns);
-    unsigned* ks_out = cv.take<unsigned>(ns);
-    int* is_in = cv.take<int>(ns);
-    int* is_out = cv.take<int>(ns);
-    unsigned* kt_in = same ? nullptr : cv.take<unsigned>(nt);
-    unsigned* kt_out = same ? nullptr : cv.take<unsigned>(nt);
-    int* it_in = same ? nullptr : cv.take<int>(nt);
-    int* it_out = same ? nullptr : cv.take<int>(nt);
-    void* tmp = cv.take<char>(tmp_bytes);
-    float4* sp = cv.take<float4>(ns);
-    float4* sa = cv.take<float4>(ns);
-    float4* tq = cv.take<float4>(nt);
-    float4* tsph = cv.take<float4>(ntiles);
-    float* tcs = cv.take<float>(ntiles);
-    float* tam = cv.take<float>(ntiles);
-    float4* bsph = cv.take<float4>(nb);
-    float* bcs = cv.take<float>(nb);
-    unsigned* mask = cv.take<unsigned>((size_t)nb * mwords);
-    double* farv = cv.take<double>((size_t)12 * nt);
-    unsigned long long* npair = cv.take<unsigned long long>(1);
-    int* nn = cv.take<int>(nb);
-    int* noff = cv.take<int>(nb);
-    int* items = cv.take<int>(nb);
-    int* ioff = cv.take<int>(nb);
-    int* mfold = cv.take<int>(1);
-    unsigned* ctr = cv.take<unsigned>(1);
-    double* part = cv.take<double>((size_t)3 * nb * 12 * FF_TB);   // at most 3 items per block
-    if (pass == 0) {
-      int rc = workspace(st, cv.off, &base);
-      if (rc != O3D_OK) return rc;
-      continue;
-    }
-
-    SrcView sv{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
-    TgtView tv{};
-    tv.x = targ->x[0]; tv.y = targ->x[1]; tv.z = targ->x[2]; tv.r = targ->r;
-    tv.u = targ->u[0]; tv.v = targ->u[1]; tv.w = targ->u[2];
-    for (int k = 0; k < 9; ++k) tv.g[k] = targ->ug[k];
-
-    Ev ev[4];
-    if (prof) {
-      for (auto& e : ev) HIPCHK(hipEventCreate(&e.e));
-      HIPCHK(hipEventRecord(ev[0].e, st));
-    }
-    hipLaunchKernelGGL(k_ff_bbox_init, dim3(1), dim3(64), 0, st, bb);
-    hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(ns, 256), 2048u)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, bb);
-    if (!same)
-      hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(nt, 256), 2048u)), dim3(256), 0, st, tv.x, tv.y, tv.z, nt,
-                         bb);
-    hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(ns, 256)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, bb, ks_in, is_in);
-    HIPCHK(hipGetLastError());
-    size_t tb = tmp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, ks_in, ks_out, is_in, is_out, (int)ns, 0, 30, st));
-    const int* tperm = is_out;
-    if (!same) {
-      hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(nt, 256)), dim3(256), 0, st, tv.x, tv.y, tv.z, nt, bb, kt_in, it_in);
-      HIPCHK(hipGetLastError());
-      tb = tmp_bytes;
-      HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kt_in, kt_out, it_in, it_out, (int)nt, 0, 30, st));
-      tperm = it_out;
-    }
-    hipLaunchKernelGGL((k_ff_gather_src<CORE>), dim3(grid_of(ns, 256)), dim3(256), 0, st, sv, is_out, ns, sp, sa);
-    hipLaunchKernelGGL((k_ff_gather_tgt<CORE, THICK>), dim3(grid_of(nt, 256)), dim3(256), 0, st, tv, tperm, nt, tq);
-    hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)ntiles), dim3(256), 0, st, sp, sa, ns, T, tsph, tcs, tam);
-    hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)nb), dim3(256), 0, st, tq, (const float4*)nullptr, nt, FF_TB, bsph,
-                       bcs, (float*)nullptr);
-    hipLaunchKernelGGL((k_ff_classify<CORE>), dim3(grid_of(nb * mwords, 256)), dim3(256), 0, st, bsph, bcs, nb, tsph,
-                       tcs, ntiles, mwords, mask);
-    hipLaunchKernelGGL(k_ff_plan_count, dim3(grid_of(nb, 256)), dim3(256), 0, st, mask, nb, mwords, ntiles, nn);
+    hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(nt, 256), 2048u)), dim3(256), 0, st, tv.x, tv.y, tv.z, nt,
+                       L.bb);
+  hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(ns, 256)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, L.bb, L.ks_in, L.is_in);
+  HIPCHK(hipGetLastError());
+  size_t tb = tmp_bytes;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(L.tmp, tb, L.ks_in, L.ks_out, L.is_in, L.is_out, (int)ns, 0, 30, st));
+  const int* tperm = L.is_out;
+  if (!same) {
+    hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(nt, 256)), dim3(256), 0, st, tv.x, tv.y, tv.z, nt, L.bb, L.kt_in, L.it_in);
     HIPCHK(hipGetLastError());
     tb = tmp_bytes;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, nn, noff, (int)nb, st));
-    hipLaunchKernelGGL(k_ff_plan_items, dim3(grid_of(nb, 256)), dim3(256), 0, st, nn, noff, nb, items, mfold);
-    HIPCHK(hipGetLastError());
-    tb = tmp_bytes;
-    HIPCHK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, items, ioff, (int)nb, st));
-    hipLaunchKernelGGL(k_ff_zero_u32, dim3(1), dim3(64), 0, st, ctr);
-    HIPCHK(hipGetLastError());
+    HIPCHK(hipcub::DeviceRadixSort::SortPairs(L.tmp, tb, L.kt_in, L.kt_out, L.it_in, L.it_out, (int)nt, 0, 30, st));
+    tperm = L.it_out;
+  }
+  hipLaunchKernelGGL((k_ff_gather_src<CORE>), dim3(grid_of(ns, 256)), dim3(256), 0, st, sv, L.is_out, ns, L.sp, L.sa);
+  hipLaunchKernelGGL((k_ff_gather_tgt<CORE, THICK>), dim3(grid_of(nt, 256)), dim3(256), 0, st, tv, tperm, nt, L.tq);
+  hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)ntiles), dim3(256), 0, st, L.sp, L.sa, ns, T, L.tsph, L.tcs, L.tam);
+  hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)nb), dim3(256), 0, st, L.tq, (const float4*)nullptr, nt, FF_TB, L.bsph,
+                     L.bcs, (float*)nullptr);
+  hipLaunchKernelGGL((k_ff_classify<CORE>), dim3(grid_of(nb * mwords, 256)), dim3(256), 0, st, L.bsph, L.bcs, nb, L.tsph,
+                     L.tcs, ntiles, mwords, L.mask);
+  hipLaunchKernelGGL(k_ff_plan_count, dim3(grid_of(nb, 256)), dim3(256), 0, st, L.mask, nb, mwords, ntiles, L.nn);
+  HIPCHK(hipGetLastError());
+  tb = tmp_bytes;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(L.tmp, tb, L.nn, L.noff, (int)nb, st));
+  hipLaunchKernelGGL(k_ff_plan_items, dim3(grid_of(nb, 256)), dim3(256), 0, st, L.nn, L.noff, nb, L.items, L.mfold);
+  HIPCHK(hipGetLastError());
+  tb = tmp_bytes;
+  HIPCHK(hipcub::DeviceScan::ExclusiveSum(L.tmp, tb, L.items, L.ioff, (int)nb, st));
+  hipLaunchKernelGGL(k_ff_zero_u32, dim3(1), dim3(64), 0, st, L.ctr);
+  HIPCHK(hipGetLastError());
 
-    FFArgs a{};
-    a.sp = sp; a.sa = sa; a.tq = tq; a.tperm = tperm;
-    a.tsph = tsph; a.tcs = tcs; a.tam = tam; a.bsph = bsph; a.bcs = bcs;
-    a.mask = mask; a.farv = farv;
-    a.nn = nn; a.items = items; a.ioff = ioff; a.mfold = mfold; a.ctr = ctr; a.part = part;
-    a.s = sv; a.t = tv;
-    a.ns = ns; a.nt = nt;
-    a.tile = T; a.mwords = mwords; a.ntiles = ntiles;
-    a.self_excl = self_excl ? 1 : 0;
-    const size_t lds_far = (size_t)256 * FF_SL * sizeof(__bf16) + (size_t)2 * 32 * FF_TP * sizeof(_Float16);
-    if (prof) HIPCHK(hipEventRecord(ev[1].e, st));
-    hipLaunchKernelGGL((k_ff_far<CORE, THICK, PREC>), dim3((unsigned)nb), dim3(FF_FAR_THREADS), lds_far, st, a);
+  FFArgs a{};
+  a.sp = L.sp; a.sa = L.sa; a.tq = L.tq; a.tperm = tperm;
+  a.tsph = L.tsph; a.tcs = L.tcs; a.tam = L.tam; a.bsph = L.bsph; a.bcs = L.bcs;
+  a.mask = L.mask; a.farv = L.farv;
+  a.nn = L.nn; a.items = L.items; a.ioff = L.ioff; a.mfold = L.mfold; a.ctr = L.ctr; a.part = L.part;
+  a.s = sv; a.t = tv;
+  a.ns = ns; a.nt = nt;
+  a.tile = T; a.mwords = mwords; a.ntiles = ntiles;
+  a.self_excl = self_excl ? 1 : 0;
+  const size_t lds_far = (size_t)256 * FF_SL * sizeof(__bf16) + (size_t)2 * 32 * FF_TP * sizeof(_Float16);
+  if (prof) HIPCHK(hipEventRecord(ev[1].e, st));
+  hipLaunchKernelGGL((k_ff_far<CORE, THICK, PREC>), dim3((unsigned)nb), dim3(FF_FAR_THREADS), lds_far, st, a);
+  HIPCHK(hipGetLastError());
+  if (prof) HIPCHK(hipEventRecord(ev[2].e, st));
+  const size_t lds_near = (size_t)T * 2 * sizeof(float4);
+  const unsigned gnear = (unsigned)std::min<int64_t>(3 * nb, near_grid<CORE, PREC>(lds_near));
+  hipLaunchKernelGGL((k_ff_near<CORE, PREC>), dim3(gnear), dim3(FF_TB), lds_near, st, a);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL((k_ff_combine<CORE, PREC>), dim3((unsigned)nb), dim3(FF_TB), 0, st, a);
+  HIPCHK(hipGetLastError());
+  if (prof) {
+    HIPCHK(hipEventRecord(ev[3].e, st));
+    hipLaunchKernelGGL(k_ff_zero_u64, dim3(1), dim3(64), 0, st, L.npair);
+    hipLaunchKernelGGL(k_ff_count, dim3(grid_of(nb, 256)), dim3(256), 0, st, L.mask, nb, mwords, ntiles, T, ns, nt,
+                       L.npair);
     HIPCHK(hipGetLastError());
-    if (prof) HIPCHK(hipEventRecord(ev[2].e, st));
-    const size_t lds_near = (size_t)T * 2 * sizeof(float4);
-    const unsigned gnear = (unsigned)std::min<int64_t>(3 * nb, near_grid<CORE, PREC>(lds_near));
-    hipLaunchKernelGGL((k_ff_near<CORE, PREC>), dim3(gnear), dim3(FF_TB), lds_near, st, a);
-    HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_ff_combine<CORE, PREC>), dim3((unsigned)nb), dim3(FF_TB), 0, st, a);
-    HIPCHK(hipGetLastError());
-    if (prof) {
-      HIPCHK(hipEventRecord(ev[3].e, st));
-      hipLaunchKernelGGL(k_ff_zero_u64, dim3(1), dim3(64), 0, st, npair);
-      hipLaunchKernelGGL(k_ff_count, dim3(grid_of(nb, 256)), dim3(256), 0, st, mask, nb, mwords, ntiles, T, ns, nt,
-                         npair);
-      HIPCHK(hipGetLastError());
-      unsigned long long far_pairs = 0;
-      HIPCHK(hipMemcpyAsync(&far_pairs, npair, sizeof far_pairs, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipStreamSynchronize(st));
-      float ms[3];
-      for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k].e, ev[k + 1].e));
-      prof->prep_ms = ms[0];
-      prof->far_ms = ms[1];
-      prof->near_ms = ms[2];
-      prof->total_ms = (double)ms[0] + ms[1] + ms[2];
-      prof->far_pairs = (double)far_pairs;
-      prof->pairs = (double)ns * (double)nt;
-      prof->far_tile_fraction = 0.0;
-      if (nb > 0 && ntiles > 0) prof->far_tile_fraction = prof->far_pairs / prof->pairs;
-    }
+    unsigned long long far_pairs = 0;
+    HIPCHK(hipMemcpyAsync(&far_pairs, L.npair, sizeof far_pairs, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms[3];
+    for (int k = 0; k < 3; ++k) HIPCHK(hipEventElapsedTime(&ms[k], ev[k].e, ev[k + 1].e));
+    prof->prep_ms = ms[0];
+    prof->far_ms = ms[1];
+    prof->near_ms = ms[2];
+    prof->total_ms = (double)ms[0] + ms[1] + ms[2];
+    prof->far_pairs = (double)far_pairs;
+    prof->pairs = (double)ns * (double)nt;
+    prof->far_tile_fraction = 0.0;
+    if (nb > 0 && ntiles > 0) prof->far_tile_fraction = prof->far_pairs / prof->pairs;
   }
   return O3D_OK;
 }
 
 template <int CORE>
 int run_core(const o3d_points* src, o3d_points* targ, bool thick, int prec, bool self_excl, int T, hipStream_t st,
-             o3d_factored_profile* prof) {
+             o3d_factored_profile* prof, void* ws, size_t wsb) {
   if (thick)
-    return prec == PREC_COMP ? run<CORE, true, PREC_COMP>(src, targ, self_excl, T, st, prof)
-                             : run<CORE, true, PREC_FP32>(src, targ, self_excl, T, st, prof);
-  return prec == PREC_COMP ? run<CORE, false, PREC_COMP>(src, targ, self_excl, T, st, prof)
-                           : run<CORE, false, PREC_FP32>(src, targ, self_excl, T, st, prof);
+    return prec == PREC_COMP ? run<CORE, true, PREC_COMP>(src, targ, self_excl, T, st, prof, ws, wsb)
+                             : run<CORE, true, PREC_FP32>(src, targ, self_excl, T, st, prof, ws, wsb);
+  return prec == PREC_COMP ? run<CORE, false, PREC_COMP>(src, targ, self_excl, T, st, prof, ws, wsb)
+                           : run<CORE, false, PREC_FP32>(src, targ, self_excl, T, st, prof, ws, wsb);
 }
 
 }  // namespace
@@ -243,22 +282,52 @@ bool ff_supported(int core, bool grad, int prec) {
   return (core == CORE_RM || core == CORE_EXP) && grad && (prec == PREC_FP32 || prec == PREC_COMP);
 }
 
-// device-pointer point->point by the factored far field; arguments already checked
-int p2p_factored(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st,
-                 o3d_factored_profile* prof) {
+namespace {
+int check_factored(const o3d_points* src, const o3d_points* targ, const o3d_opts* o, int* T) {
   const bool grad = o->results == O3D_VELANDGRAD;
   if (!ff_supported(o->core, grad, o->precision))
     return fail(O3D_EINVAL, "factored method: needs the RM or EXP core, velocity and gradient, and FP32 or "
                             "compensated precision");
   if (src->n > INT_MAX || targ->n > INT_MAX)
     return fail(O3D_EINVAL, "factored method: at most %d sources and %d targets", INT_MAX, INT_MAX);
-  const int T = o->tile > 0 ? o->tile : 256;
-  if (T != 64 && T != 128 && T != 256) return fail(O3D_EINVAL, "factored method: tile must be 0, 64, 128 or 256");
+  *T = o->tile > 0 ? o->tile : 256;
+  if (*T != 64 && *T != 128 && *T != 256) return fail(O3D_EINVAL, "factored method: tile must be 0, 64, 128 or 256");
+  return O3D_OK;
+}
+
+int dispatch(const o3d_points* src, o3d_points* targ, const o3d_opts* o, int T, hipStream_t st,
+             o3d_factored_profile* prof, void* ws, size_t wsb) {
   const bool thick = targ->r != nullptr;
   const bool self_excl = o->self_term == O3D_SELF_EXCLUDE;
+  if (o->core == CORE_RM) return run_core<CORE_RM>(src, targ, thick, o->precision, self_excl, T, st, prof, ws, wsb);
+  return run_core<CORE_EXP>(src, targ, thick, o->precision, self_excl, T, st, prof, ws, wsb);
+}
+}  // namespace
+
+// device-pointer point->point by the factored far field; arguments already checked
+int p2p_factored(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st,
+                 o3d_factored_profile* prof) {
+  int T = 0, rc = check_factored(src, targ, o, &T);
+  if (rc != O3D_OK) return rc;
   std::lock_guard<std::mutex> lk(g_ws_mu);
-  if (o->core == CORE_RM) return run_core<CORE_RM>(src, targ, thick, o->precision, self_excl, T, st, prof);
-  return run_core<CORE_EXP>(src, targ, thick, o->precision, self_excl, T, st, prof);
+  return dispatch(src, targ, o, T, st, prof, nullptr, 0);
+}
+
+// the same on scratch the caller owns (ws_bytes >= ff_workspace_bytes of this shape);
+// nothing is allocated and no lock is taken
+int p2p_factored_ws(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, void* ws,
+                    size_t ws_bytes) {
+  int T = 0, rc = check_factored(src, targ, o, &T);
+  if (rc != O3D_OK) return rc;
+  if (!ws) return fail(O3D_EINVAL, "factored method: NULL workspace");
+  return dispatch(src, targ, o, T, st, nullptr, ws, ws_bytes);
+}
+
+size_t ff_workspace_bytes(int64_t ns, int64_t nt, bool same, int tile) {
+  const int T = tile > 0 ? tile : 256;
+  const size_t tmp = lib_tmp_bytes(ns, nt, same, nullptr);
+  if (tmp == 0) return 0;
+  return layout(nullptr, ns, nt, same, T, tmp).bytes;
 }
 
 }  // namespace o3d

@@ -203,7 +203,7 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   if (o->precision == PREC_FP64) return fail(O3D_EINVAL, "internal: FP64 routed to p2p_dev");
   if (o->method == O3D_METHOD_FACTORED) return p2p_factored(src, targ, o, st);
   if (o->method == O3D_METHOD_AUTO && !tun && o->tile == 0 && o->tpt == 0 && ff_supported(o->core, grad, o->precision) &&
-      src->n >= 65536 && targ->n >= 65536 && src->n <= INT32_MAX && targ->n <= INT32_MAX)
+      src->n >= FF_AUTO_MIN_N && targ->n >= FF_AUTO_MIN_N && src->n <= INT32_MAX && targ->n <= INT32_MAX)
     return p2p_factored(src, targ, o, st);
   P2PArgs a{};
   a.s = SrcView{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};

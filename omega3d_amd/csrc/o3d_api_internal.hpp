@@ -30,6 +30,11 @@ void fill_stats(o3d_stats* stats, double secs, double flops, double inter);
 bool ff_supported(int core, bool grad, int prec);
 int p2p_factored(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st,
                  o3d_factored_profile* prof = nullptr);
+int p2p_factored_ws(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, void* ws,
+                    size_t ws_bytes);
+size_t ff_workspace_bytes(int64_t ns, int64_t nt, bool same, int tile);
+// a collection large enough for the factored method to pay when the caller leaves the choice open
+constexpr int64_t FF_AUTO_MIN_N = 65536;
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);
 
 }  // namespace o3d

@@ -16,6 +16,13 @@
 // arena space are free, so new work joins ticks while earlier work is running.
 // The evaluation of a chunk is p2p_chunk, the same code as the single-collection
 // kernel, so a step's result does not depend on its companions.
+//
+// A step of a large session with the RM or EXP core (by default >= 65,536
+// particles) skips the ticks: both of its stages run the factored far-field
+// method, each followed by its stage update, on a second stream that overlaps the
+// ticks, on scratch the server sizes for its largest such session.  Which way a
+// session's steps go depends only on the session and the server's options, so a
+// step's result still does not depend on its companions.
 #include "o3d_api_internal.hpp"
 #include "p2p_kernels.hpp"
 
@@ -190,6 +197,17 @@ __device__ void move_particle(const MoveInst& m, int64_t i) {
   }
 }
 
+// one request's stage update over all its particles (the factored steps)
+__global__ __launch_bounds__(BLOCK) void k_serve_move_one(MoveInst m) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < m.n; i += stride) move_particle(m, i);
+}
+
+__global__ void k_serve_zero(float* p, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0.f;
+}
+
 // persistent: move items are (request, chunk of BLOCK particles)
 __global__ __launch_bounds__(BLOCK) void k_serve_move(const MoveInst* mi, const ServeItem* items, const ServeTables* tab) {
   const int nitems = tab->n_move;
@@ -287,6 +305,8 @@ struct Request {
   float* work = nullptr;
   bool cancel = false;
   bool finished_planning = false;
+  bool factored = false;    // both stages by the factored method on the second stream
+  bool ff_launched = false;
   float* w(int k, int64_t n) const { return work + (size_t)k * n; }   // row k of 30
 };
 
@@ -294,6 +314,11 @@ struct Tick {
   hipEvent_t done = nullptr;
   std::vector<int64_t> completed;   // requests whose stage 2 ran in this tick
   int slot = 0;
+};
+
+struct FFJob {
+  hipEvent_t done = nullptr;
+  int64_t rid = 0;
 };
 
 }  // namespace
@@ -316,6 +341,12 @@ struct o3d_server {
   std::deque<int64_t> queue;       // queued request ids in arrival order
   std::vector<int64_t> active;     // admitted, in admission order
   std::deque<Tick> inflight;
+  // factored steps: their stream, their scratch and the steps enqueued on it
+  hipStream_t ff_stream = nullptr;
+  void* ffws = nullptr;
+  size_t ffws_bytes = 0;
+  std::deque<FFJob> ff_inflight;
+  int64_t n_factored = 0;
   Clock::time_point t0 = Clock::now();
   // per-tick device tables, double buffered
   size_t inst_cap = 0, item_cap = 0, move_cap = 0;
@@ -434,6 +465,7 @@ double chunk_cost(const o3d_server* S, const Session& ses, int64_t chunk) {
 }
 
 void finish_request(o3d_server* S, Request& r, int state, int status, double when) {
+  if (r.state >= O3D_RQ_DONE) return;   // already ended (a failure path ended it first)
   r.state = state;
   r.status = status;
   r.done_s = when;
@@ -453,6 +485,13 @@ void finish_request(o3d_server* S, Request& r, int state, int status, double whe
   } else {
     S->n_cancel++;
   }
+}
+
+// steps of this session go to the factored method
+bool session_factored(const o3d_server* S, const Session& ses) {
+  if (ses.core != CORE_RM && ses.core != CORE_EXP) return false;
+  if (S->opts.method == O3D_METHOD_FACTORED) return true;
+  return S->opts.method == O3D_METHOD_AUTO && ses.n >= FF_AUTO_MIN_N;
 }
 
 // admit queued requests (arrival order; a request of a session with an admitted
@@ -494,7 +533,9 @@ void admit(o3d_server* S) {
     r.stage = 1;
     r.next_chunk = 0;
     r.nchunks = (ses.n + (int64_t)BLOCK * S->tpt - 1) / ((int64_t)BLOCK * S->tpt);
-    r.finished_planning = false;
+    r.factored = session_factored(S, ses);
+    r.finished_planning = r.factored;   // the ticks never see it
+    r.ff_launched = false;
     S->active.push_back(r.id);
     it = S->queue.erase(it);
   }
@@ -538,6 +579,100 @@ MoveInst move_inst(const Session& ses, const Request& r) {
   }
   m.elong = ses.elong();
   return m;
+}
+
+// Both stages of a factored step on the second stream: zero the stage's 12 result
+// rows (the factored method accumulates), evaluate the session on itself (self
+// pair included, as in the ticks), then the stage update.
+int enqueue_factored_step(o3d_server* S, Request& r, const Session& ses) {
+  hipStream_t st = S->ff_stream;
+  const int64_t n = ses.n;
+  const unsigned zgrid = (unsigned)std::min<int64_t>((12 * n + BLOCK - 1) / BLOCK, 4096);
+  const unsigned mgrid = (unsigned)std::min<int64_t>((n + BLOCK - 1) / BLOCK, (int64_t)S->move_grid);
+  for (int stage = 1; stage <= 2; ++stage) {
+    r.stage = stage;
+    const ServeInst I = eval_inst(ses, r);
+    hipLaunchKernelGGL(k_serve_zero, dim3(zgrid), dim3(BLOCK), 0, st, I.t.u, 12 * n);   // rows u, v, w, g[0..8]
+    HIPCHK(hipGetLastError());
+    o3d_points p{};
+    p.n = n;
+    p.x[0] = I.s.x; p.x[1] = I.s.y; p.x[2] = I.s.z;
+    p.s[0] = I.s.ax; p.s[1] = I.s.ay; p.s[2] = I.s.az;
+    p.r = I.s.r;
+    p.u[0] = I.t.u; p.u[1] = I.t.v; p.u[2] = I.t.w;
+    for (int k = 0; k < 9; ++k) p.ug[k] = I.t.g[k];
+    o3d_opts o{};
+    o.core = ses.core;
+    o.results = O3D_VELANDGRAD;
+    o.precision = r.prec;
+    o.self_term = O3D_SELF_INCLUDE;
+    o.method = O3D_METHOD_FACTORED;
+    const int rc = p2p_factored_ws(&p, &p, &o, st, S->ffws, S->ffws_bytes);
+    if (rc != O3D_OK) return rc;
+    hipLaunchKernelGGL(k_serve_move_one, dim3(mgrid), dim3(BLOCK), 0, st, move_inst(ses, r));
+    HIPCHK(hipGetLastError());
+  }
+  FFJob j;
+  j.rid = r.id;
+  HIPCHK(hipEventCreateWithFlags(&j.done, hipEventDisableTiming));
+  HIPCHK(hipEventRecord(j.done, st));
+  S->ff_inflight.push_back(j);
+  r.ff_launched = true;
+  S->n_factored++;
+  return O3D_OK;
+}
+
+// Size the factored scratch for a new session (caller holds S->mu, under which alone
+// the loop enqueues factored steps).  The new block is allocated before the old one
+// is freed, so a failure leaves the sessions already open working.
+int ensure_ff_scratch(o3d_server* S, int64_t n, int32_t core) {
+  Session probe{};
+  probe.n = n;
+  probe.core = core;
+  if (!session_factored(S, probe)) return O3D_OK;
+  const size_t need = ff_workspace_bytes(n, n, true, 0);
+  if (need == 0) return fail(O3D_EDEVICE, "factored scratch query failed");
+  if (need <= S->ffws_bytes) return O3D_OK;
+  void* p = nullptr;
+  if (hipMalloc(&p, need) != hipSuccess) return fail(O3D_ENOMEM, "factored scratch of %zu bytes", need);
+  if (S->ffws) {
+    const hipError_t e = hipStreamSynchronize(S->ff_stream);
+    if (e != hipSuccess) {
+      hipFree(p);
+      return fail(O3D_EDEVICE, "factored stream: %s", hipGetErrorString(e));
+    }
+    hipFree(S->ffws);
+  }
+  S->ffws = p;
+  S->ffws_bytes = need;
+  return O3D_OK;
+}
+
+int launch_factored(o3d_server* S) {
+  for (int64_t id : S->active) {
+    Request& r = S->requests.at(id);
+    if (!r.factored || r.ff_launched) continue;
+    const int rc = enqueue_factored_step(S, r, S->sessions.at(r.sid));
+    if (rc != O3D_OK) return rc;
+  }
+  return O3D_OK;
+}
+
+// finished factored steps (in stream order)
+void retire_factored(o3d_server* S) {
+  while (!S->ff_inflight.empty()) {
+    FFJob& j = S->ff_inflight.front();
+    const hipError_t q = hipEventQuery(j.done);
+    if (q == hipErrorNotReady) return;
+    const double when = S->now();
+    if (q != hipSuccess) S->loop_status = fail(O3D_EDEVICE, "factored step failed: %s", hipGetErrorString(q));
+    Request& r = S->requests.at(j.rid);
+    S->active.erase(std::remove(S->active.begin(), S->active.end(), j.rid), S->active.end());
+    finish_request(S, r, q == hipSuccess ? O3D_RQ_DONE : O3D_RQ_FAILED, q == hipSuccess ? O3D_OK : O3D_EDEVICE, when);
+    hipEventDestroy(j.done);
+    S->ff_inflight.pop_front();
+    S->done_cv.notify_all();
+  }
 }
 
 // plan and enqueue one tick; returns false when there is nothing to run
@@ -676,21 +811,22 @@ void loop_main(o3d_server* S) {
   std::unique_lock<std::mutex> lk(S->mu);
   while (true) {
     retire(S, false);
-    if (S->stop && S->queue.empty() && S->active.empty() && S->inflight.empty()) break;
+    retire_factored(S);
+    if (S->stop && S->queue.empty() && S->active.empty() && S->inflight.empty() && S->ff_inflight.empty()) break;
     admit(S);
     bool ran = false;
-    if (S->inflight.size() < 2) {
-      int rc = plan_tick(S, &ran);
-      if (rc != O3D_OK) {
-        S->loop_status = rc;
-        // fail everything that was admitted; the stream state is unknown
-        for (int64_t id : S->active) finish_request(S, S->requests.at(id), O3D_RQ_FAILED, rc, S->now());
-        S->active.clear();
-        S->done_cv.notify_all();
-      }
+    int rc = launch_factored(S);
+    if (rc == O3D_OK && S->inflight.size() < 2) rc = plan_tick(S, &ran);
+    if (rc != O3D_OK) {
+      S->loop_status = rc;
+      // fail everything that was admitted; the stream state is unknown (ticks and
+      // factored steps still in flight retire later and find their requests ended)
+      for (int64_t id : S->active) finish_request(S, S->requests.at(id), O3D_RQ_FAILED, rc, S->now());
+      S->active.clear();
+      S->done_cv.notify_all();
     }
     if (ran) continue;
-    if (!S->inflight.empty()) {
+    if (!S->inflight.empty() || !S->ff_inflight.empty()) {
       // let submissions in while the GPU works
       lk.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(200));
@@ -733,6 +869,7 @@ int o3d_server_create(o3d_server** out, const o3d_server_opts* opts) {
     return fail(O3D_ENOMEM, "arena of %zu bytes", S->opts.hbm_budget);
   S->arena.init(S->opts.hbm_budget);
   HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  HIPCHK(hipStreamCreateWithFlags(&S->ff_stream, hipStreamNonBlocking));
   for (int b = 0; b < 2; ++b) {
     HIPCHK(hipMalloc(&S->d_tab[b], sizeof(ServeTables)));
     HIPCHK(hipHostMalloc(&S->h_tab[b], sizeof(ServeTables), hipHostMallocDefault));
@@ -762,6 +899,7 @@ int o3d_server_destroy(o3d_server* S) {
   S->cv.notify_all();
   if (S->loop.joinable()) S->loop.join();
   hipStreamSynchronize(S->stream);
+  hipStreamSynchronize(S->ff_stream);
   for (int b = 0; b < 2; ++b) {
     if (S->graph[b]) hipGraphExecDestroy(S->graph[b]);
     if (S->graph_src[b]) hipGraphDestroy(S->graph_src[b]);
@@ -770,6 +908,8 @@ int o3d_server_destroy(o3d_server* S) {
     hipHostFree(S->h_tab[b]);
   }
   hipStreamDestroy(S->stream);
+  hipStreamDestroy(S->ff_stream);
+  if (S->ffws) hipFree(S->ffws);
   hipFree(S->arena_base);
   delete S;
   return O3D_OK;
@@ -804,6 +944,11 @@ int o3d_server_open_session(o3d_server* S, const o3d_points* p, int32_t core, in
     if (!S->arena.alloc(sizeof(float) * host.size(), &off))
       return fail(O3D_EADMIT, "session of %lld particles (%zu bytes) does not fit the arena (%zu of %zu used)",
                   (long long)n, sizeof(float) * host.size(), S->arena.used, S->arena.size);
+    const int frc = ensure_ff_scratch(S, n, core);
+    if (frc != O3D_OK) {
+      S->arena.release(off, sizeof(float) * host.size());
+      return frc;
+    }
     ses.id = S->next_sid++;
     ses.n = n;
     ses.core = core;
@@ -948,6 +1093,8 @@ int o3d_server_get_stats(o3d_server* S, o3d_server_stats* st) {
   st->hbm_budget = S->arena.size;
   st->hbm_peak = S->arena.peak;
   st->ticks = S->tick_no;
+  st->factored_steps = S->n_factored;
+  st->factored_scratch = S->ffws_bytes;
   return O3D_OK;
 }
 
@@ -959,6 +1106,7 @@ int o3d_server_reset_stats(o3d_server* S) {
   S->first_submit = -1;
   S->last_done = 0;
   S->n_sub = S->n_done = S->n_fail = S->n_cancel = 0;
+  S->n_factored = 0;
   S->arena.peak = S->arena.used;
   return O3D_OK;
 }

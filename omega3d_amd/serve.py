@@ -20,7 +20,8 @@ from ._lib import CORES, PREC, check, lib
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("max_active", C.c_int32), ("tile", C.c_int32), ("tpt", C.c_int32),
-                ("use_graph", C.c_int32), ("adaptive_tick", C.c_int32), ("tick_interactions", C.c_double)]
+                ("use_graph", C.c_int32), ("adaptive_tick", C.c_int32), ("method", C.c_int32),
+                ("tick_interactions", C.c_double)]
 
 
 class RequestInfo(C.Structure):
@@ -34,7 +35,7 @@ class ServerStats(C.Structure):
                 ("p50_s", C.c_double), ("p99_s", C.c_double), ("mean_s", C.c_double),
                 ("interactions", C.c_double), ("wall_s", C.c_double), ("ginter_per_s", C.c_double),
                 ("hbm_used", C.c_size_t), ("hbm_budget", C.c_size_t), ("hbm_peak", C.c_size_t),
-                ("ticks", C.c_int64)]
+                ("ticks", C.c_int64), ("factored_steps", C.c_int64), ("factored_scratch", C.c_size_t)]
 
 
 STATES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
@@ -63,8 +64,9 @@ def inject_fault(count, kind=1):
 
 class Server:
     def __init__(self, hbm_budget=8 << 30, max_active=64, tile=0, tpt=0, use_graph=True, tick_interactions=0.0,
-                 adaptive_tick=True):
-        o = ServerOpts(hbm_budget, max_active, tile, tpt, int(use_graph), int(adaptive_tick), tick_interactions)
+                 adaptive_tick=True, method="auto"):
+        m = _lib.METHODS[method] if isinstance(method, str) else int(method)
+        o = ServerOpts(hbm_budget, max_active, tile, tpt, int(use_graph), int(adaptive_tick), m, tick_interactions)
         h = C.c_void_p()
         check(lib.o3d_server_create(C.byref(h), C.byref(o)))
         self.h = h.value

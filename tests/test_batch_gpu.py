@@ -118,3 +118,66 @@ def test_batch_empty_and_tiny(mods):
             continue
         ov, og = oracle.p2p("wl", p.pos, p.str, p.rad, p.pos, p.rad, True, "f64")
         assert rel_errors(v, ov)[0] <= 2e-6 and rel_errors(g, og)[0] <= 2e-6
+
+
+def test_batch_factored_instances(mods):
+    """Instances of >= 65,536 particles leave the queue for the factored method (AUTO):
+    each is bitwise the single-collection factored call, the small ones bitwise the
+    direct call; run_separate and graph replays (with eager work between them) give
+    the same bits; the large instances meet the compensated bar against the oracle."""
+    _lib, batch, infl = mods
+    big = [scenes.ring(70001, R=1.3, sigma=0.02, seed=11), scenes.ring(100000, R=0.8, sigma=0.03, seed=12)]
+    inst = small_instances(6)[:3] + big[:1] + small_instances(6)[3:] + big[1:]
+    b = batch.Batch(inst, core="exp", precision="comp", tile=256, tpt=1)
+    s = batch.stream_create()
+    try:
+        cfg = b.config()
+        assert cfg["factored_instances"] == 2 and cfg["factored_scratch_bytes"] > 0
+        b.run(s)
+        _lib.check(_lib.lib.o3d_stream_synchronize(s))
+        eager = b.results()
+        for (v, g), p in zip(eager, inst):
+            m = "factored" if p.n >= 65536 else "direct"
+            va, ga = infl.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", tile=256 if m == "direct" else 0,
+                              tpt=1 if m == "direct" else 0, method=m)
+            np.testing.assert_array_equal(v, va)
+            np.testing.assert_array_equal(g, ga)
+            if p.n >= 65536:
+                sel = np.arange(0, p.n, p.n // 512)[:512]
+                ov, og = oracle.p2p("exp", p.pos, p.str, p.rad, p.pos[:, sel], p.rad[sel], True, "f64")
+                assert rel_errors(v[:, sel], ov)[0] <= 1e-6 and rel_errors(g[:, sel], og)[0] <= 1e-6
+        b.run_separate(s)
+        _lib.check(_lib.lib.o3d_stream_synchronize(s))
+        for (v, g), (v2, g2) in zip(eager, b.results()):
+            np.testing.assert_array_equal(v, v2)
+            np.testing.assert_array_equal(g, g2)
+        b.capture(s)
+        for rep in range(3):
+            b.replay(s)
+            _lib.check(_lib.lib.o3d_stream_synchronize(s))
+            for (v, g), (v2, g2) in zip(eager, b.results()):
+                np.testing.assert_array_equal(v, v2)
+                np.testing.assert_array_equal(g, g2)
+            p = inst[rep]
+            infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp")
+    finally:
+        b.close()
+        batch.stream_destroy(s)
+
+
+def test_batch_method_choice(mods):
+    _lib, batch, infl = mods
+    inst = small_instances(4) + [scenes.ring(66000, seed=13)]
+    b = batch.Batch(inst, core="rm", precision="fp32", method="direct")
+    assert b.config()["factored_instances"] == 0
+    b.close()
+    b = batch.Batch(inst, core="rm", precision="fp32", method="factored")
+    assert b.config()["factored_instances"] == len(inst)
+    b.run()
+    for (v, g), p in zip(b.results(), inst):
+        va, ga = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "fp32", method="factored")
+        np.testing.assert_array_equal(v, va)
+        np.testing.assert_array_equal(g, ga)
+    b.close()
+    with pytest.raises(_lib.O3DError):
+        batch.Batch(inst, cores=["rm", "wl", "rm", "rm", "rm"], precision="comp", method="factored")

@@ -113,6 +113,75 @@ def test_alone_vs_mix_bitwise(serve):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("core,prec", [("rm", "comp"), ("exp", "fp32")])
+def test_factored_step_matches_oracle(serve, core, prec):
+    """method FACTORED: both stages by the factored far field on the second stream;
+    the same bars as the ticks' steps against the oracle's RK2 restatement."""
+    p = _ring(3000, 11)
+    dt, fs = 0.02, (0.05, 0.0, -0.02)
+    srv = serve.Server(hbm_budget=256 << 20, method="factored")
+    try:
+        sid = srv.open_session(p, core)
+        info = srv.wait(srv.submit(sid, dt, prec, fs))
+        assert info["state"] == "done" and info["rc"] == 0, info
+        x, s, e = srv.read_session(sid)
+        st = srv.stats()
+        assert st["factored_steps"] == 1 and st["factored_scratch"] > 0 and st["ticks"] == 0
+    finally:
+        srv.close()
+    x32, s32, e32 = rk2.rk2_step(core, p.pos, p.str, p.rad, np.ones(p.n, np.float32), dt, fs, "f32d")
+    x64, s64, e64 = rk2.rk2_step(core, p.pos, p.str, p.rad, np.ones(p.n), dt, fs, "f64")
+    dx, dx64 = x - p.pos, x64 - p.pos.astype(np.float64)
+    ds, ds64 = s - p.str, s64 - p.str.astype(np.float64)
+    print(core, prec, "factored: dx vs f64", rel_errors(dx, dx64), "ds vs f64", rel_errors(ds, ds64))
+    _close_float(x, x32, x32 - p.pos)
+    _close_float(s, s32, s32 - p.str)
+    assert rel_errors(dx, dx64)[0] <= 1e-4
+    assert rel_errors(ds, ds64)[0] <= 1e-4
+    assert rel_errors((e - 1)[None, :], (e64 - 1)[None, :])[0] <= 1e-4
+
+
+def test_large_session_factored_alone_vs_mix(serve):
+    """A 70,000-particle RM session takes the factored way by default (AUTO): its step
+    agrees with the ticks' step (method DIRECT) to float rounding of the state, and
+    gives the same bits alone and among concurrent small requests."""
+    big = _ring(70000, 41, core_sigma=0.03)
+    dt = 0.01
+
+    def step_alone(method):
+        srv = serve.Server(hbm_budget=1 << 30, method=method)
+        try:
+            sid = srv.open_session(big, "rm")
+            assert srv.wait(srv.submit(sid, dt, "comp"))["state"] == "done"
+            st = srv.stats()
+            return srv.read_session(sid), st
+        finally:
+            srv.close()
+
+    alone, st = step_alone("auto")
+    assert st["factored_steps"] == 1 and st["ticks"] == 0
+    direct, st_d = step_alone("direct")
+    assert st_d["factored_steps"] == 0 and st_d["ticks"] > 0
+    _close_float(alone[0], direct[0], direct[0] - big.pos)
+    _close_float(alone[1], direct[1], direct[1] - big.str)
+    srv = serve.Server(hbm_budget=1 << 30, tile=256, tpt=1, tick_interactions=5e6, adaptive_tick=False)
+    try:
+        others = [srv.open_session(_ring(500 + 900 * k, 50 + k), ["rm", "exp", "wl"][k % 3]) for k in range(5)]
+        sid = srv.open_session(big, "rm")
+        rids = [srv.submit(o, 0.01, ["comp", "fp32"][k % 2]) for k, o in enumerate(others)]
+        mine = srv.submit(sid, dt, "comp")
+        rids += [srv.submit(o, 0.01, "comp") for o in others]
+        for r in rids + [mine]:
+            assert srv.wait(r)["state"] == "done"
+        mixed = srv.read_session(sid)
+        st = srv.stats()
+        assert st["factored_steps"] == 1 and st["completed"] == len(rids) + 1
+    finally:
+        srv.close()
+    for a, b in zip(alone, mixed):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_admission_cancel_deadline(serve):
     from omega3d_amd import _lib
     big = _ring(20000, 41)
