@@ -43,7 +43,8 @@ def lib():
     global _lib
     if _lib is None:
         _lib = _load(os.path.join(HERE, "liboracle.so"))
-        for name in ("orc_p2p_f", "orc_p2p_d", "orc_pan_f", "orc_pan_d", "orc_p2p_vcstyle"):
+        for name in ("orc_p2p_f", "orc_p2p_d", "orc_pan_f", "orc_pan_d", "orc_p2p_vcstyle", "orc_tri_integrals",
+                     "orc_pan_g7_near", "orc_pan_g7_d"):
             getattr(_lib, name).restype = C.c_int
         _lib.orc_flops_pair.restype = C.c_int
         _lib.orc_flops_pair.argtypes = [C.c_int] * 4
@@ -180,6 +181,50 @@ def ptpan(core, spos, sstr, nodes, idx, area, mode="f32d", maxlev=RECURSIVE_LEVE
     if rc != 0:
         raise RuntimeError(f"oracle ptpan failed rc={rc}")
     return vel
+
+
+# ---- panels by the 7-point rule with the closed-form near field (o3d_panel_quad.cpp) ----
+
+def g7_rule():
+    """(bary (7, 3), weights (7,)) of the 7-point degree-5 triangle rule, weights summing to 1."""
+    bary, w = np.zeros((7, 3)), np.zeros(7)
+    lib().orc_g7_rule(bary.ctypes.data_as(_vp), w.ctypes.data_as(_vp))
+    return bary, w
+
+
+def tri_integrals(v, x, grads=True):
+    """I = int_T (x - y)/|x - y|^3 dA and dI[j, l] = dI_l/dx_j over the triangle v (3, 3) (rows = vertices), FP64."""
+    v = np.ascontiguousarray(v, np.float64).reshape(9)
+    x = np.ascontiguousarray(x, np.float64).reshape(3)
+    I, dI = np.zeros(3), np.zeros(9)
+    lib().orc_tri_integrals(v.ctypes.data_as(_vp), x.ctypes.data_as(_vp), I.ctypes.data_as(_vp),
+                            dI.ctypes.data_as(_vp) if grads else None)
+    return I, dI.reshape(3, 3)
+
+
+def pan_g7_near(v, t):
+    """the float near/far decision for one (target, panel): True = closed form"""
+    v = np.ascontiguousarray(v, np.float32).reshape(9)
+    t = np.asarray(t, np.float32)
+    return bool(lib().orc_pan_g7_near(v.ctypes.data_as(_vp), C.c_float(t[0]), C.c_float(t[1]), C.c_float(t[2])))
+
+
+def pan_g7(nodes, idx, ts, area, ss, tpos, grads=True, nthreads=0):
+    """panels -> points by the 7-point rule per panel with the closed-form near field
+    (within twice the longest edge of the centroid), FP64; returns (vel, grad, near_pairs)."""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    nodes, ts, area, ss, tpos = (_as(a, np.float32) for a in (nodes, ts, area, ss, tpos))
+    nt = tpos.shape[1]
+    vel = np.zeros((3, nt))
+    grad = np.zeros((9, nt)) if grads else None
+    nnear = C.c_int64(0)
+    rc = lib().orc_pan_g7_d(int(grads), _i64(area.shape[0]), _ptr(nodes[0]), _ptr(nodes[1]), _ptr(nodes[2]),
+                            _ptr(idx), _ptr(ts[0]), _ptr(ts[1]), _ptr(ts[2]), _ptr(area), _ptr(ss),
+                            _i64(nt), _ptr(tpos[0]), _ptr(tpos[1]), _ptr(tpos[2]),
+                            _ptr(vel[0]), _ptr(vel[1]), _ptr(vel[2]), _ptrs9(grad), C.byref(nnear), int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle pan_g7 failed rc={rc}")
+    return vel, grad, nnear.value
 
 
 # ---- the reference's own kernels (oracle/_ref), same conventions -------------
