@@ -56,6 +56,8 @@ def parse():
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--sweep", action="store_true",
                     help="time every (precision, tile, tpt) on the N=1M workload, print them and exit")
+    ap.add_argument("--valu-probe", action="store_true",
+                    help="measure the FP32 VALU issue cost per wave64 instruction (fmac, add, mul, rsq), print and exit")
     return ap.parse_args()
 
 
@@ -75,6 +77,16 @@ def main():
     _lib.check(L.o3d_init(0))
     dev = _lib.device_info()
     log(f"bench: device {dev['name']} CUs {dev['cus']}")
+    if args.valu_probe:
+        out = {}
+        for op, name in enumerate(("v_fmac_f32", "v_add_f32", "v_mul_f32", "v_rsq_f32")):
+            ms, wi = C.c_float(), C.c_double()
+            _lib.check(L.o3d_valu_probe(op, C.byref(ms), C.byref(wi)))
+            simd_cycles = dev["cus"] * 4 * dev["clock_khz"] * 1e3 * ms.value * 1e-3
+            out[name] = {"ms": round(ms.value, 4), "wave_instructions": wi.value,
+                         "cycles_per_wave_instruction_at_max_clock": round(simd_cycles / wi.value, 3)}
+        print(json.dumps({"valu_probe": out, "clock_mhz": dev["clock_khz"] // 1000, "cus": dev["cus"]}))
+        return
 
     t0 = time.perf_counter()
     c = scenes.colliding_rings(n_each=args.n_each)

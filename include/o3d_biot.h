@@ -63,8 +63,17 @@ enum o3d_self { O3D_SELF_INCLUDE = 0, O3D_SELF_EXCLUDE = 1 };
  *             cores; other pairs as in DIRECT.  RM and EXP cores, velocity and
  *             gradient, FP32 or compensated precision; tile 64, 128 or 256.
  *   AUTO:     FACTORED where it applies and both counts are >= 65536 with tile
- *             and tpt left at 0, DIRECT otherwise. */
-enum o3d_method { O3D_METHOD_AUTO = 0, O3D_METHOD_DIRECT = 1, O3D_METHOD_FACTORED = 2 };
+ *             and tpt left at 0, DIRECT otherwise.
+ * For panel sources (o3d_panels_affect_points[_dev]) the field selects the rule:
+ *   AUTO / DIRECT:      the reference's recursive centroid rule (Kernels.h:1028-1211);
+ *   O3D_PANEL_GAUSS7:   no counterpart in the reference -- per (target, panel) pair a
+ *             7-point degree-5 rule (Strang & Fix; Dunavant) when the target is at
+ *             least twice the panel's longest edge from its centroid, otherwise the
+ *             exact integral of the panel's constant vortex and source sheet
+ *             densities (ts/area, ss) over the flat triangle, in FP64 (edge
+ *             logarithms and the signed solid angle).  The near/far decision is
+ *             taken in float with a fixed operation order. */
+enum o3d_method { O3D_METHOD_AUTO = 0, O3D_METHOD_DIRECT = 1, O3D_METHOD_FACTORED = 2, O3D_PANEL_GAUSS7 = 3 };
 
 enum o3d_status {
   O3D_OK = 0,
@@ -148,6 +157,22 @@ int o3d_points_affect_points(const o3d_points* src, o3d_points* targ,
  * ignored, as in the reference (point kernels for every target, Influence.h:873). */
 int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ,
                              const o3d_opts* opts, o3d_stats* stats);
+
+typedef struct o3d_panel_profile {
+  double total_ms;     /* the full evaluation (far points and near closed forms)       */
+  double far_ms;       /* the same launch with the near pairs left out                 */
+  double pairs;        /* target-panel pairs                                            */
+  double near_pairs;   /* pairs evaluated in closed form                                */
+  double far_pairs;    /* pairs evaluated by the 7-point rule                           */
+} o3d_panel_profile;
+
+/* The 7-point rule (opts->method = O3D_PANEL_GAUSS7) on DEVICE arrays, timed on
+ * opts->stream: the full evaluation accumulates into targ as
+ * o3d_panels_affect_points_dev does; a far-only pass (into library scratch) and a
+ * count of the near pairs follow.  Velocity and gradient, FP32 or compensated.
+ * No counterpart in the reference. */
+int o3d_panels_g7_profile_dev(const o3d_panels* src, o3d_points* targ, const o3d_opts* opts,
+                              o3d_panel_profile* prof);
 
 /* points_affect_panels (Influence.h:1107-1221): the velocity of each target panel
  * induced by vortex particles, the reference's boundary-element right-hand side
@@ -382,6 +407,11 @@ int o3d_event_elapsed_ms(float* ms, void* start, void* stop);
  * (bytes a multiple of 16): the known-traffic reference that HBM byte counters
  * (FETCH_SIZE, WRITE_SIZE) are calibrated against. */
 int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream);
+/* VALU issue-rate probe (no counterpart in the reference; calibrates the FP32
+ * rooflines): a grid of 8 workgroups per CU runs 16 independent chains of one
+ * instruction per lane (op 0 v_fmac_f32, 1 v_add_f32, 2 v_mul_f32, 3 v_rsq_f32);
+ * reports the second launch's time and its wave64 instruction count.  Synchronous. */
+int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions);
 /* device name, CU count, clock (kHz), HBM bytes */
 int o3d_device_info(char* name, int name_len, int* cus, int* clock_khz, size_t* hbm_bytes);
 

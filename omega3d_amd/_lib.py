@@ -60,7 +60,8 @@ _int_fns = ["o3d_init", "o3d_abi_version", "o3d_points_affect_points", "o3d_poin
             "o3d_dev_malloc", "o3d_dev_free", "o3d_memcpy_htod", "o3d_memcpy_dtoh", "o3d_memset_dev",
             "o3d_device_synchronize", "o3d_stream_create", "o3d_stream_destroy", "o3d_stream_synchronize",
             "o3d_event_create", "o3d_event_destroy", "o3d_event_record", "o3d_event_elapsed_ms",
-            "o3d_device_info", "o3d_copy_probe", "o3d_factored_profile_dev",
+            "o3d_device_info", "o3d_copy_probe", "o3d_valu_probe", "o3d_factored_profile_dev",
+            "o3d_panels_g7_profile_dev",
             "o3d_points_affect_panels", "o3d_points_affect_panels_dev"]
 for _n in _int_fns:
     if hasattr(lib, _n):
@@ -95,12 +96,17 @@ def last_error():
     return lib.o3d_last_error().decode(errors="replace")
 
 
-METHODS = {"auto": 0, "direct": 1, "factored": 2}
+METHODS = {"auto": 0, "direct": 1, "factored": 2, "gauss7": 3}
 
 
 class FactoredProfile(C.Structure):
     _fields_ = [("prep_ms", C.c_double), ("far_ms", C.c_double), ("near_ms", C.c_double), ("total_ms", C.c_double),
                 ("far_pairs", C.c_double), ("pairs", C.c_double), ("far_tile_fraction", C.c_double)]
+
+
+class PanelProfile(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("far_ms", C.c_double), ("pairs", C.c_double),
+                ("near_pairs", C.c_double), ("far_pairs", C.c_double)]
 
 
 def make_opts(core="wl", grads=True, precision="comp", self_term=0, tile=0, tpt=0, maxlev=-1, stream=None,

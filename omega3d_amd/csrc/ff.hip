@@ -30,28 +30,6 @@ struct Work {
 std::mutex g_ws_mu;
 std::map<hipStream_t, Work> g_ws;
 
-// caller holds g_ws_mu for the whole enqueue, so that no other thread frees or
-// grows this stream's workspace between its sizing and the last launch using it
-int workspace(hipStream_t st, size_t bytes, void** out) {
-  Work& w = g_ws[st];
-  if (w.bytes < bytes) {
-    if (w.p) {
-      HIPCHK(hipStreamSynchronize(st));
-      HIPCHK(hipFree(w.p));
-      w.p = nullptr;
-      w.bytes = 0;
-    }
-    const size_t want = bytes + bytes / 4;
-    if (hipMalloc(&w.p, want) != hipSuccess) {
-      w.p = nullptr;
-      return fail(O3D_ENOMEM, "factored path: device allocation of %zu bytes failed", want);
-    }
-    w.bytes = want;
-  }
-  *out = w.p;
-  return O3D_OK;
-}
-
 // sequential sub-allocation of the workspace, 256-byte aligned
 struct Carve {
   char* base;
@@ -171,7 +149,7 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
     if (ws_bytes < L.bytes) return fail(O3D_EINVAL, "factored path: workspace of %zu bytes, %zu needed", ws_bytes, L.bytes);
   } else {
     void* p = nullptr;
-    int rc = workspace(st, L.bytes, &p);
+    int rc = stream_workspace(st, L.bytes, &p);
     if (rc != O3D_OK) return rc;
     base = static_cast<char*>(p);
   }
@@ -277,6 +255,31 @@ int run_core(const o3d_points* src, o3d_points* targ, bool thick, int prec, bool
 }
 
 }  // namespace
+
+std::mutex& stream_ws_mutex() { return g_ws_mu; }
+
+// caller holds stream_ws_mutex() for the whole enqueue, so that no other thread
+// frees or grows this stream's workspace between its sizing and the last launch
+// using it; a larger request waits for the stream to drain, then reallocates
+int stream_workspace(hipStream_t st, size_t bytes, void** out) {
+  Work& w = g_ws[st];
+  if (w.bytes < bytes) {
+    if (w.p) {
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipFree(w.p));
+      w.p = nullptr;
+      w.bytes = 0;
+    }
+    const size_t want = bytes + bytes / 4;
+    if (hipMalloc(&w.p, want) != hipSuccess) {
+      w.p = nullptr;
+      return fail(O3D_ENOMEM, "device workspace of %zu bytes: allocation failed", want);
+    }
+    w.bytes = want;
+  }
+  *out = w.p;
+  return O3D_OK;
+}
 
 bool ff_supported(int core, bool grad, int prec) {
   return (core == CORE_RM || core == CORE_EXP) && grad && (prec == PREC_FP32 || prec == PREC_COMP);

@@ -345,6 +345,31 @@ __global__ void k_copy_probe(float4* __restrict__ d, const float4* __restrict__ 
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) d[i] = s[i];
 }
 
+// VALU issue-rate probe: 16 independent chains of one FP32 instruction per lane
+// (op 0 v_fmac_f32, 1 v_add_f32, 2 v_mul_f32, 3 v_rsq_f32), iters times
+template <int OP>
+__global__ __launch_bounds__(BLOCK) void k_valu_probe(float* out, int iters, float b) {
+  float x[16], y[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    x[j] = threadIdx.x * 1e-3f + j;
+    y[j] = 1.0f + (j + 1) * 1e-4f;   // never 1: every chain keeps its instruction
+  }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      if (OP == 0) x[j] = fmaf(y[j], b, x[j]);
+      else if (OP == 1) x[j] = x[j] + y[j];
+      else if (OP == 2) x[j] = x[j] * y[j];
+      else x[j] = __builtin_amdgcn_rsqf(x[j]);
+    }
+  }
+  float s = 0.f;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s += x[j];
+  out[(int64_t)blockIdx.x * BLOCK + threadIdx.x] = s;
+}
+
 }  // namespace o3d
 
 using namespace o3d;
@@ -556,6 +581,36 @@ int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream) {
     hipLaunchKernelGGL(k_copy_probe, dim3(std::max(blocks, 1)), dim3(BLOCK), 0, (hipStream_t)stream, (float4*)dst,
                        (const float4*)src, n);
   HIPCHK(hipGetLastError());
+  return O3D_OK;
+}
+
+int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
+  clear_error();
+  if (op < 0 || op > 3 || !ms || !wave_instructions) return fail(O3D_EINVAL, "valu probe: op 0..3 and outputs");
+  int rc = ensure_init();
+  if (rc) return rc;
+  const int blocks = 8 * num_cus(), iters = op == 3 ? 1024 : 4096;
+  std::lock_guard<std::mutex> lk(pool_mutex());
+  void* buf = nullptr;
+  if ((rc = pool_buffer(4, sizeof(float) * (size_t)blocks * BLOCK, &buf)) != O3D_OK) return rc;
+  hipStream_t st = lib_stream();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  for (int rep = 0; rep < 2; ++rep) {   // the first launch warms up
+    HIPCHK(hipEventRecord(e0, st));
+    if (op == 0) hipLaunchKernelGGL(k_valu_probe<0>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 1e-7f);
+    if (op == 1) hipLaunchKernelGGL(k_valu_probe<1>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
+    if (op == 2) hipLaunchKernelGGL(k_valu_probe<2>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
+    if (op == 3) hipLaunchKernelGGL(k_valu_probe<3>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+  }
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *wave_instructions = (double)blocks * (BLOCK / 64) * (double)iters * 16.0;
   return O3D_OK;
 }
 

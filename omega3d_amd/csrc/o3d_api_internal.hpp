@@ -33,6 +33,10 @@ int p2p_factored(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hip
 int p2p_factored_ws(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, void* ws,
                     size_t ws_bytes);
 size_t ff_workspace_bytes(int64_t ns, int64_t nt, bool same, int tile);
+// one growable device workspace per stream (calls on a stream are ordered, so they
+// share it); hold stream_ws_mutex() from stream_workspace until the last enqueue
+std::mutex& stream_ws_mutex();
+int stream_workspace(hipStream_t st, size_t bytes, void** out);
 // a collection large enough for the factored method to pay when the caller leaves the choice open
 constexpr int64_t FF_AUTO_MIN_N = 65536;
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);

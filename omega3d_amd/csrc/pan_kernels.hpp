@@ -38,6 +38,8 @@ struct PanArgs {
   int64_t nt;
   int32_t tile;
   int32_t maxlev;
+  int32_t* near_list;          // 7-point rule: targets with a near pair (or nullptr)
+  unsigned int* near_count;
 };
 
 constexpr int PAN_F4 = 6;   // float4 per staged panel

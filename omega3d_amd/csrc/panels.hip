@@ -1,12 +1,22 @@
 // panels.hip -- panel -> point kernels and the panels_affect_points entry points.
+// o3d_opts.method selects the panel rule: AUTO/DIRECT the reference's recursive
+// centroid rule (pan_kernels.hpp), O3D_PANEL_GAUSS7 the 7-point rule with the
+// closed-form near field (pan_g7.hpp).
 #include "o3d_api_internal.hpp"
+#include "pan_g7.hpp"
 #include "pan_kernels.hpp"
 
+#include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cstring>
 #include <vector>
 
 namespace o3d {
+
+__global__ void k_ff_zero_u64_pan(unsigned long long* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+}
 
 static hipError_t launch_pan(const PanArgs& a, bool grad, int prec, hipStream_t st) {
   if (a.nt <= 0 || a.np <= 0) return hipSuccess;
@@ -18,6 +28,65 @@ static hipError_t launch_pan(const PanArgs& a, bool grad, int prec, hipStream_t 
   O3D_PL(false, PREC_FP32) O3D_PL(false, PREC_COMP) O3D_PL(false, PREC_FP64)
 #undef O3D_PL
   return hipErrorInvalidValue;
+}
+
+// far kernel, then (near) the near kernel over the targets the far kernel listed;
+// with near, a.near_list / a.near_count must point at scratch for nt entries and a
+// counter (zeroed here), and nnear, if not NULL, counts the near pairs
+static hipError_t launch_pan_g7(PanArgs a, bool grad, int prec, bool near, hipStream_t st, float4* geom = nullptr,
+                                unsigned long long* nnear = nullptr) {
+  if (a.nt <= 0 || a.np <= 0) return hipSuccess;
+  if (!near) a.near_list = nullptr;
+  const size_t lds = (size_t)a.tile * G7_F4 * sizeof(float4);
+  const dim3 grid((unsigned)((a.nt + BLOCK - 1) / BLOCK));
+  // one wave per listed target, grid-stride: at most 16 waves per CU
+  const dim3 grid_near((unsigned)std::min<int64_t>((a.nt + 3) / 4, 4 * (int64_t)num_cus()));
+  if (near) {
+    if (!a.near_list || !a.near_count || !geom) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pan_g7_zero, dim3(1), dim3(64), 0, st, a.near_count);
+    hipLaunchKernelGGL(k_pan_g7_geom, dim3((unsigned)((a.np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, a, geom);
+  }
+  hipError_t e = hipErrorInvalidValue;
+#define O3D_PL(G, P)                                                                           \
+  if (grad == G && prec == P) {                                                                \
+    hipLaunchKernelGGL((k_pan_g7<G, P>), grid, dim3(BLOCK), lds, st, a);                       \
+    e = hipGetLastError();                                                                     \
+    if (e == hipSuccess && near) {                                                             \
+      hipLaunchKernelGGL((k_pan_g7_near<G, P>), grid_near, dim3(BLOCK), 0, st, a, geom, nnear); \
+      e = hipGetLastError();                                                                   \
+    }                                                                                          \
+  }
+  O3D_PL(true, PREC_FP32) O3D_PL(true, PREC_COMP) O3D_PL(true, PREC_FP64)
+  O3D_PL(false, PREC_FP32) O3D_PL(false, PREC_COMP) O3D_PL(false, PREC_FP64)
+#undef O3D_PL
+  return e;
+}
+
+// the near-target list, its counter and the panels' decision geometry in the
+// stream's workspace (caller holds stream_ws_mutex() until the launches are enqueued)
+static int g7_scratch(PanArgs* a, hipStream_t st, float4** geom) {
+  if (a->nt > INT32_MAX) return fail(O3D_EINVAL, "the 7-point rule takes at most %d targets", INT32_MAX);
+  void* ws = nullptr;
+  const size_t list = (sizeof(int32_t) * (size_t)a->nt + 255) & ~(size_t)255;
+  const int rc = stream_workspace(st, 256 + list + sizeof(float4) * (size_t)a->np, &ws);
+  if (rc != O3D_OK) return rc;
+  a->near_count = (unsigned int*)ws;
+  a->near_list = (int32_t*)((char*)ws + 256);
+  *geom = (float4*)((char*)ws + 256 + list);
+  return O3D_OK;
+}
+
+// the panel rule and the options check_opts validates (method is the panel rule here)
+static int panel_opts(const o3d_opts* o, o3d_opts* oc, bool* g7) {
+  if (!o) return fail(O3D_EINVAL, "opts is NULL");
+  *oc = *o;
+  *g7 = o->method == O3D_PANEL_GAUSS7;
+  if (o->method == O3D_METHOD_FACTORED) return fail(O3D_EINVAL, "the factored method is for point sources");
+  if (*g7) {
+    oc->method = O3D_METHOD_AUTO;
+    if (o->tile > 256) return fail(O3D_EINVAL, "tile %d: the 7-point rule stages at most 256 panels per tile", o->tile);
+  }
+  return check_opts(oc);
 }
 
 static int check_panels(const o3d_panels* p) {
@@ -49,19 +118,15 @@ static int maxlev_of(const o3d_opts* o, int* ml) {
   return O3D_OK;
 }
 
-static double pan_flops(bool grad, int64_t nt, int64_t np) {
-  // leaf-only count: one flops_0vs_0p(g) per panel-target pair plus the per-target
-  // term; the subdivision work beyond the first leaf is not counted
+static double pan_flops(bool grad, int64_t nt, int64_t np, bool g7) {
+  // recursive rule: leaf-only count, one flops_0vs_0p(g) per panel-target pair plus
+  // the per-target term (subdivision beyond the first leaf is not counted);
+  // 7-point rule: seven of them per pair (near pairs included at that count)
   const int f = o3d_flops_per_interaction(CORE_RM, grad ? O3D_VELANDGRAD : O3D_VELONLY, 0, 1);
-  return (double)nt * ((grad ? 12.0 : 3.0) + (double)f * (double)np);
+  return (double)nt * ((grad ? 12.0 : 3.0) + (g7 ? 7.0 : 1.0) * (double)f * (double)np);
 }
 
-// float device arrays; FP32 or compensated
-static int pan_dev(const o3d_panels* p, o3d_points* t, const o3d_opts* o, int prec, void* const* out64,
-                   hipStream_t st) {
-  const bool grad = o->results == O3D_VELANDGRAD;
-  int ml, rc;
-  if ((rc = maxlev_of(o, &ml)) != O3D_OK) return rc;
+static PanArgs pan_args(const o3d_panels* p, const o3d_points* t, bool grad, void* const* out64) {
   PanArgs a{};
   a.nx = p->x[0]; a.ny = p->x[1]; a.nz = p->x[2];
   a.idx = p->idx;
@@ -76,9 +141,27 @@ static int pan_dev(const o3d_panels* p, o3d_points* t, const o3d_opts* o, int pr
     for (int k = 0; k < 9; ++k) a.g[k] = grad ? t->ug[k] : nullptr;
   }
   a.nt = t->n;
+  return a;
+}
+
+// float device arrays (FP32 or compensated), or FP64 scratch rows out64
+static int pan_dev(const o3d_panels* p, o3d_points* t, const o3d_opts* o, int prec, void* const* out64,
+                   hipStream_t st, bool g7) {
+  const bool grad = o->results == O3D_VELANDGRAD;
+  int ml, rc;
+  if ((rc = maxlev_of(o, &ml)) != O3D_OK) return rc;
+  PanArgs a = pan_args(p, t, grad, out64);
   a.tile = o->tile > 0 ? o->tile : 128;
   a.maxlev = ml;
-  hipError_t e = launch_pan(a, grad, prec, st);
+  if (g7) {
+    std::lock_guard<std::mutex> lk(stream_ws_mutex());
+    float4* geom = nullptr;
+    if ((rc = g7_scratch(&a, st, &geom)) != O3D_OK) return rc;
+    const hipError_t e = launch_pan_g7(a, grad, prec, true, st, geom);
+    if (e != hipSuccess) return fail(O3D_EDEVICE, "panel launch failed: %s", hipGetErrorString(e));
+    return O3D_OK;
+  }
+  const hipError_t e = launch_pan(a, grad, prec, st);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "panel launch failed: %s", hipGetErrorString(e));
   return O3D_OK;
 }
@@ -89,10 +172,13 @@ using namespace o3d;
 
 extern "C" {
 
-int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ, const o3d_opts* o, o3d_stats* stats) {
+int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ, const o3d_opts* opts, o3d_stats* stats) {
   clear_error();
   int rc;
-  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  o3d_opts oc;
+  bool g7;
+  if ((rc = panel_opts(opts, &oc, &g7)) != O3D_OK) return rc;
+  const o3d_opts* o = &oc;
   const bool grad = o->results == O3D_VELANDGRAD;
   if ((rc = check_panels(src)) != O3D_OK) return rc;
   if ((rc = check_targets(targ, grad)) != O3D_OK) return rc;
@@ -136,7 +222,7 @@ int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ, const o3d_
     for (int k = 0; k < nout; ++k) outs[k] = ob + osz * nt * k;
     if (o->precision == PREC_FP64) {
       HIPCHK(hipMemsetAsync(ob, 0, osz * nout * nt, st));
-      rc = pan_dev(&d, &dtg, o, PREC_FP64, outs, st);
+      rc = pan_dev(&d, &dtg, o, PREC_FP64, outs, st, g7);
     } else {
       for (int k = 0; k < 3; ++k) {
         dtg.u[k] = (float*)outs[k];
@@ -147,7 +233,7 @@ int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ, const o3d_
           dtg.ug[k] = (float*)outs[3 + k];
           HIPCHK(hipMemcpyAsync(dtg.ug[k], targ->ug[k], sizeof(float) * nt, hipMemcpyHostToDevice, st));
         }
-      rc = pan_dev(&d, &dtg, o, o->precision, nullptr, st);
+      rc = pan_dev(&d, &dtg, o, o->precision, nullptr, st, g7);
     }
     if (rc != O3D_OK) return rc;
     if (o->precision == PREC_FP64) {
@@ -170,14 +256,17 @@ int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ, const o3d_
     }
   }
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  fill_stats(stats, secs, pan_flops(grad, nt, np), (double)nt * (double)np);
+  fill_stats(stats, secs, pan_flops(grad, nt, np, g7), (double)nt * (double)np);
   return O3D_OK;
 }
 
-int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ, const o3d_opts* o, o3d_stats* stats) {
+int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ, const o3d_opts* opts, o3d_stats* stats) {
   clear_error();
   int rc;
-  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  o3d_opts oc;
+  bool g7;
+  if ((rc = panel_opts(opts, &oc, &g7)) != O3D_OK) return rc;
+  const o3d_opts* o = &oc;
   const bool grad = o->results == O3D_VELANDGRAD;
   if ((rc = check_panels(src)) != O3D_OK) return rc;
   if ((rc = check_targets(targ, grad)) != O3D_OK) return rc;
@@ -185,10 +274,69 @@ int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ, const 
     return fail(O3D_EINVAL, "FP64 panels on device float arrays are not offered; use the host call");
   if ((rc = ensure_init()) != O3D_OK) return rc;
   if (src->np > 0 && targ->n > 0) {
-    rc = pan_dev(src, targ, o, o->precision, nullptr, (hipStream_t)o->stream);
+    rc = pan_dev(src, targ, o, o->precision, nullptr, (hipStream_t)o->stream, g7);
     if (rc != O3D_OK) return rc;
   }
-  fill_stats(stats, 0.0, pan_flops(grad, targ->n, src->np), (double)targ->n * (double)src->np);
+  fill_stats(stats, 0.0, pan_flops(grad, targ->n, src->np, g7), (double)targ->n * (double)src->np);
+  return O3D_OK;
+}
+
+// The 7-point rule on device arrays, timed: one full evaluation into targ (as
+// o3d_panels_affect_points_dev), one far-field-only pass into library scratch, and
+// a count of the near pairs.  Velocity and gradient, FP32 or compensated.
+int o3d_panels_g7_profile_dev(const o3d_panels* src, o3d_points* targ, const o3d_opts* opts,
+                              o3d_panel_profile* out) {
+  clear_error();
+  int rc;
+  if (!out) return fail(O3D_EINVAL, "out is NULL");
+  *out = o3d_panel_profile{};
+  o3d_opts oc;
+  bool g7;
+  if ((rc = panel_opts(opts, &oc, &g7)) != O3D_OK) return rc;
+  if (!g7) return fail(O3D_EINVAL, "the profile is for the 7-point rule (method O3D_PANEL_GAUSS7)");
+  if (oc.results != O3D_VELANDGRAD || oc.precision == PREC_FP64)
+    return fail(O3D_EINVAL, "the profile runs velocity and gradient in FP32 or compensated precision");
+  if ((rc = check_panels(src)) != O3D_OK) return rc;
+  if ((rc = check_targets(targ, true)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  const int64_t nt = targ->n, np = src->np;
+  if (nt == 0 || np == 0) return O3D_OK;
+  hipStream_t st = (hipStream_t)oc.stream;
+  std::lock_guard<std::mutex> lk(pool_mutex());
+  void* scratch;
+  if ((rc = pool_buffer(4, sizeof(float) * 12 * (size_t)nt + 256, &scratch)) != O3D_OK) return rc;
+  float* f = (float*)scratch;
+  o3d_points t2 = *targ;
+  for (int d = 0; d < 3; ++d) t2.u[d] = f + d * nt;
+  for (int k = 0; k < 9; ++k) t2.ug[k] = f + (3 + k) * nt;
+  unsigned long long* cnt = (unsigned long long*)(f + 12 * nt);
+  PanArgs full = pan_args(src, targ, true, nullptr);
+  PanArgs far = pan_args(src, &t2, true, nullptr);
+  full.tile = far.tile = oc.tile > 0 ? oc.tile : 128;
+  std::lock_guard<std::mutex> lk2(stream_ws_mutex());
+  float4* geom = nullptr;
+  if ((rc = g7_scratch(&full, st, &geom)) != O3D_OK) return rc;
+  hipEvent_t ev[4];
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  hipLaunchKernelGGL(k_ff_zero_u64_pan, dim3(1), dim3(64), 0, st, cnt);
+  HIPCHK(hipEventRecord(ev[0], st));
+  HIPCHK(launch_pan_g7(full, true, oc.precision, true, st, geom, cnt));
+  HIPCHK(hipEventRecord(ev[1], st));
+  HIPCHK(launch_pan_g7(far, true, oc.precision, false, st));
+  HIPCHK(hipEventRecord(ev[2], st));
+  unsigned long long nnear = 0;
+  HIPCHK(hipMemcpyAsync(&nnear, cnt, sizeof nnear, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(ev[3], st));
+  HIPCHK(hipEventSynchronize(ev[3]));
+  float ms0 = 0.f, ms1 = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms0, ev[0], ev[1]));
+  HIPCHK(hipEventElapsedTime(&ms1, ev[1], ev[2]));
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  out->total_ms = ms0;
+  out->far_ms = ms1;
+  out->pairs = (double)nt * (double)np;
+  out->near_pairs = (double)nnear;
+  out->far_pairs = out->pairs - out->near_pairs;
   return O3D_OK;
 }
 

@@ -174,8 +174,9 @@ def ptpan(spos, sstr, nodes, idx, area, precision="comp", maxlev=-1, vel=None, s
 
 
 def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, precision="comp", maxlev=-1,
-        vel=None, grad=None, stats=False):
-    """Host-array panel->point sum (recursive centroid rule) on the GPU."""
+        vel=None, grad=None, stats=False, rule="recursive", tile=0):
+    """Host-array panel->point sum on the GPU: rule "recursive" is the reference's
+    centroid subdivision, "gauss7" the 7-point rule with the closed-form near field."""
     nodes, ts, area, ss, tpos = _c32(nodes), _c32(ts), _c32(area), _c32(ss), _c32(tpos)
     idx = np.ascontiguousarray(idx, dtype=np.uint32)
     nt = tpos.shape[1]
@@ -191,7 +192,7 @@ def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, precision="comp", maxl
     p.area = area.ctypes.data
     p.ss = ss.ctypes.data if ss is not None else None
     t = _pts_struct(tpos, None, None, vel, grad)
-    o = make_opts(core, grads, precision, 0, 0, 0, maxlev)
+    o = make_opts(core, grads, precision, 0, tile, 0, maxlev, method={"recursive": "auto", "gauss7": "gauss7"}[rule])
     st = _lib.Stats()
     check(lib.o3d_panels_affect_points(C.byref(p), C.byref(t), C.byref(o), C.byref(st)))
     return (vel, grad, st) if stats else (vel, grad)
