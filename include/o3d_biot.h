@@ -193,6 +193,32 @@ int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ,
 int o3d_points_affect_panels_dev(const o3d_points* src, const o3d_panels* targ, float* const vel[3],
                                  const o3d_opts* opts, o3d_stats* stats);
 
+/* ---- BEM right-hand side of one boundary surface (BEMHelper.h:83-117) ---- */
+typedef struct o3d_bem_rhs_opts {
+  double fs[3];        /* freestream (Surfaces::finalize_vels, Surfaces.h:877-887)          */
+  double vel[3];       /* the body's velocity at this time (Body::get_vel)                  */
+  double rotvel[3];    /* the body's angular velocity vector (Body::get_rotvel_vec)         */
+  float tc[3];         /* the surface's transformed geometric centre (Surfaces tc)          */
+  int32_t body;        /* 0: no body, or attached to the ground: no motion term
+                          (Surfaces.h:561-562)                                             */
+  int32_t nunk;        /* unknowns per panel (RHS.h:71-95): 1 normal, 2 tangential (x1,
+                          x2), 3 tangential then normal                                    */
+} o3d_bem_rhs_opts;
+
+/* DEVICE arrays, on opts->stream.  The panel-centre velocities pu[3] (np floats
+ * each; prior contents discarded) are zeroed, receive points_affect_panels from
+ * every particle collection vort[0..nvort) (velocity only, opts->precision FP32 or
+ * compensated), are finalized (fs + pu/(4 pi)) and lose the body's motion at the
+ * panel centroids (factor -1, BEMHelper.h:104); rhs (nunk * np floats) receives
+ * minus their components on each panel's basis (x1, x2, normal; Surfaces.h:766-816,
+ * RHS.h:47-98).  Float/double mixing as the reference's.  The augmented rows
+ * (total circulation, BEMHelper.h:120-143) are the caller's. */
+int o3d_bem_rhs_dev(const o3d_points* vort, int32_t nvort, const o3d_panels* bdry, const o3d_bem_rhs_opts* b,
+                    const o3d_opts* opts, float* const pu[3], float* rhs);
+/* The last two steps alone, on velocities pu[3] as points_affect_panels left them. */
+int o3d_bem_rhs_from_vels_dev(const o3d_panels* bdry, const o3d_bem_rhs_opts* b, float* const pu[3], float* rhs,
+                              void* stream);
+
 /* Diagnostics of the factored method (no counterpart in the reference): runs one
  * o3d_points_affect_points_dev evaluation with method FACTORED on device arrays,
  * synchronises, and reports the time of its stages (preparation: box, Morton keys,

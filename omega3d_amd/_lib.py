@@ -44,6 +44,11 @@ class Opts(C.Structure):
                 ("maxlev", C.c_int32), ("method", C.c_int32), ("stream", C.c_void_p)]
 
 
+class BemRhsOpts(C.Structure):
+    _fields_ = [("fs", C.c_double * 3), ("vel", C.c_double * 3), ("rotvel", C.c_double * 3), ("tc", C.c_float * 3),
+                ("body", C.c_int32), ("nunk", C.c_int32)]
+
+
 class Stats(C.Structure):
     _fields_ = [("seconds", C.c_double), ("flops", C.c_double), ("interactions", C.c_double)]
 
@@ -62,7 +67,8 @@ _int_fns = ["o3d_init", "o3d_abi_version", "o3d_points_affect_points", "o3d_poin
             "o3d_event_create", "o3d_event_destroy", "o3d_event_record", "o3d_event_elapsed_ms",
             "o3d_device_info", "o3d_copy_probe", "o3d_valu_probe", "o3d_factored_profile_dev",
             "o3d_panels_g7_profile_dev",
-            "o3d_points_affect_panels", "o3d_points_affect_panels_dev"]
+            "o3d_points_affect_panels", "o3d_points_affect_panels_dev", "o3d_bem_rhs_dev",
+            "o3d_bem_rhs_from_vels_dev"]
 for _n in _int_fns:
     if hasattr(lib, _n):
         getattr(lib, _n).restype = C.c_int

@@ -15,6 +15,7 @@
 #include "o3d_api_internal.hpp"
 #include "pan_kernels.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <vector>
 
@@ -152,6 +153,114 @@ static PtPanArgs ptpan_args(const o3d_points* src, const o3d_panels* targ, const
   return a;
 }
 
+// ---- BEM right-hand side of one boundary surface (BEMHelper.h:83-117) ----------
+// Per panel, in the reference's own float/double mixing, contraction off:
+//   finalize_vels      Surfaces.h:877-887   pu = fs + pu * (0.25/pi)              (double -> float)
+//   add_body_motion    Surfaces.h:556-597   pu += -1 * (float)vel                 (float)
+//                                           pu += -1 * (rotvel x (c - tc))        (double -> float),
+//                                           c - tc = float((float sum)/3.0 - tc)
+//   compute_bases      Surfaces.h:766-816   x1 = (n1 - n0)*(1.0/base), x2 = ((n2 - n0) - dp x1)
+//                                           *(1.0/height) (scalings in double), norm = x1 x x2
+//   vels_to_rhs_panels RHS.h:47-98          rhs = -(pu.x1), -(pu.x2) [, -(pu.norm)]; 1 unknown: -(pu.norm)
+struct BemArgs {
+  const float* nx; const float* ny; const float* nz;
+  const uint32_t* idx;
+  float* pu[3];
+  int64_t np;
+  double fs[3], vel[3], rotvel[3];
+  float tc[3];
+  int32_t body;
+  int32_t nunk;
+  float* rhs;
+};
+
+__global__ __launch_bounds__(BLOCK) void k_bem_rhs(BemArgs a) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= a.np) return;
+  const double factor = 0.25 / M_PI;
+  float u[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) u[d] = (float)(a.fs[d] + (double)a.pu[d][i] * factor);
+  const uint32_t id0 = a.idx[3 * i], id1 = a.idx[3 * i + 1], id2 = a.idx[3 * i + 2];
+  const float* xs[3] = {a.nx, a.ny, a.nz};
+  if (a.body) {
+    const float f = -1.0f;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = u[d] + f * (float)a.vel[d];
+    float c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) c[d] = (float)((double)(xs[d][id0] + xs[d][id1] + xs[d][id2]) / 3.0 - (double)a.tc[d]);
+    u[0] = (float)((double)u[0] + (double)f * (a.rotvel[1] * c[2] - a.rotvel[2] * c[1]));
+    u[1] = (float)((double)u[1] + (double)f * (a.rotvel[2] * c[0] - a.rotvel[0] * c[2]));
+    u[2] = (float)((double)u[2] + (double)f * (a.rotvel[0] * c[1] - a.rotvel[1] * c[0]));
+  }
+  float x1[3], x2[3], nm[3];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x1[j] = xs[j][id1] - xs[j][id0];
+  const float base = sqrtf(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x1[j] = (float)((double)x1[j] * (1.0 / (double)base));
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x2[j] = xs[j][id2] - xs[j][id0];
+  const float dp = x2[0] * x1[0] + x2[1] * x1[1] + x2[2] * x1[2];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x2[j] = x2[j] - dp * x1[j];
+  const float height = sqrtf(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x2[j] = (float)((double)x2[j] * (1.0 / (double)height));
+  nm[0] = x1[1] * x2[2] - x1[2] * x2[1];
+  nm[1] = x1[2] * x2[0] - x1[0] * x2[2];
+  nm[2] = x1[0] * x2[1] - x1[1] * x2[0];
+  const float t1 = -(u[0] * x1[0] + u[1] * x1[1] + u[2] * x1[2]);
+  const float t2 = -(u[0] * x2[0] + u[1] * x2[1] + u[2] * x2[2]);
+  const float tn = -(u[0] * nm[0] + u[1] * nm[1] + u[2] * nm[2]);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) a.pu[d][i] = u[d];
+  if (a.nunk == 1) {
+    a.rhs[i] = tn;
+  } else if (a.nunk == 2) {
+    a.rhs[2 * i] = t1;
+    a.rhs[2 * i + 1] = t2;
+  } else {
+    a.rhs[3 * i] = t1;
+    a.rhs[3 * i + 1] = t2;
+    a.rhs[3 * i + 2] = tn;
+  }
+}
+
+__global__ void k_bem_zero(float* p0, float* p1, float* p2, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    p0[i] = 0.f; p1[i] = 0.f; p2[i] = 0.f;
+  }
+}
+
+static int bem_rhs_args(const o3d_panels* bdry, const o3d_bem_rhs_opts* b, float* const* pu, float* rhs,
+                        BemArgs* a) {
+  if (!bdry || !b) return fail(O3D_EINVAL, "boundary panels or BEM options are NULL");
+  if (b->nunk < 1 || b->nunk > 3) return fail(O3D_EINVAL, "unknowns per panel %d: must be 1, 2 or 3", b->nunk);
+  if (bdry->np < 0 || bdry->nn < 0) return fail(O3D_EINVAL, "negative panel or node count");
+  if (bdry->np > 0) {
+    if (bdry->nn <= 0 || !bdry->idx) return fail(O3D_EINVAL, "panels without nodes or indices");
+    for (int d = 0; d < 3; ++d)
+      if (!bdry->x[d]) return fail(O3D_EINVAL, "panel node component %d is NULL", d);
+    if (!pu || !pu[0] || !pu[1] || !pu[2] || !rhs) return fail(O3D_EINVAL, "panel velocity or rhs arrays are NULL");
+  }
+  *a = BemArgs{};
+  a->nx = bdry->x[0]; a->ny = bdry->x[1]; a->nz = bdry->x[2];
+  a->idx = bdry->idx;
+  for (int d = 0; d < 3; ++d) {
+    a->pu[d] = bdry->np > 0 ? pu[d] : nullptr;
+    a->fs[d] = b->fs[d]; a->vel[d] = b->vel[d]; a->rotvel[d] = b->rotvel[d]; a->tc[d] = b->tc[d];
+  }
+  a->np = bdry->np;
+  a->body = b->body ? 1 : 0;
+  a->nunk = b->nunk;
+  a->rhs = rhs;
+  return O3D_OK;
+}
+
 static double ptpan_flops(int64_t np, int64_t ns) {
   // one leaf kernel_0vs_0p per (panel, particle) pair plus 3 per panel (Influence.h:1212)
   const int f = o3d_flops_per_interaction(CORE_RM, O3D_VELONLY, 0, 1);
@@ -244,6 +353,54 @@ int o3d_points_affect_panels_dev(const o3d_points* src, const o3d_panels* targ, 
     if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
   }
   fill_stats(stats, 0.0, ptpan_flops(targ->np, src->n), (double)targ->np * (double)src->n);
+  return O3D_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int o3d_bem_rhs_from_vels_dev(const o3d_panels* bdry, const o3d_bem_rhs_opts* b, float* const pu[3], float* rhs,
+                              void* stream) {
+  clear_error();
+  int rc;
+  BemArgs a;
+  if ((rc = bem_rhs_args(bdry, b, pu, rhs, &a)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  if (a.np == 0) return O3D_OK;
+  hipLaunchKernelGGL(k_bem_rhs, dim3((unsigned)((a.np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, (hipStream_t)stream, a);
+  HIPCHK(hipGetLastError());
+  return O3D_OK;
+}
+
+int o3d_bem_rhs_dev(const o3d_points* vort, int32_t nvort, const o3d_panels* bdry, const o3d_bem_rhs_opts* b,
+                    const o3d_opts* opts, float* const pu[3], float* rhs) {
+  clear_error();
+  int rc;
+  if (nvort < 0 || (nvort > 0 && !vort)) return fail(O3D_EINVAL, "bad particle collection list");
+  if ((rc = check_opts(opts)) != O3D_OK) return rc;
+  if (opts->results != O3D_VELONLY)
+    return fail(O3D_EINVAL, "the BEM right-hand side uses velocities only (BEMHelper.h:73: ResultsType(velonly))");
+  if (opts->precision == PREC_FP64) return fail(O3D_EINVAL, "device float arrays: FP32 or compensated precision");
+  BemArgs a;
+  if ((rc = bem_rhs_args(bdry, b, pu, rhs, &a)) != O3D_OK) return rc;
+  for (int32_t k = 0; k < nvort; ++k)
+    if ((rc = check_ptpan(&vort[k], bdry, pu, opts)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  if (a.np == 0) return O3D_OK;
+  hipStream_t st = (hipStream_t)opts->stream;
+  const unsigned zb = (unsigned)std::min<int64_t>((a.np + BLOCK - 1) / BLOCK, 1024);
+  hipLaunchKernelGGL(k_bem_zero, dim3(zb), dim3(BLOCK), 0, st, pu[0], pu[1], pu[2], a.np);   // zero_vels
+  HIPCHK(hipGetLastError());
+  for (int32_t k = 0; k < nvort; ++k) {   // accumulate from every particle collection
+    if (vort[k].n == 0) continue;
+    PtPanArgs pa = ptpan_args(&vort[k], bdry, opts);
+    pa.u = pu[0]; pa.v = pu[1]; pa.w = pu[2];
+    hipError_t e = launch_ptpan(pa, opts->precision, st);
+    if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
+  }
+  hipLaunchKernelGGL(k_bem_rhs, dim3((unsigned)((a.np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, a);
+  HIPCHK(hipGetLastError());
   return O3D_OK;
 }
 

@@ -173,6 +173,54 @@ def ptpan(spos, sstr, nodes, idx, area, precision="comp", maxlev=-1, vel=None, s
     return (vel, st) if stats else vel
 
 
+def bem_opts(fs, body=None, nunk=2):
+    """o3d_bem_rhs_opts: body = None (no body, or the ground) or a dict with vel,
+    rotvel (double 3-vectors) and tc (the surface's transformed centre)"""
+    b = _lib.BemRhsOpts()
+    for d in range(3):
+        b.fs[d] = float(fs[d])
+    if body is not None:
+        b.body = 1
+        for d in range(3):
+            b.vel[d] = float(body.get("vel", (0, 0, 0))[d])
+            b.rotvel[d] = float(body.get("rotvel", (0, 0, 0))[d])
+            b.tc[d] = float(body.get("tc", (0, 0, 0))[d])
+    b.nunk = int(nunk)
+    return b
+
+
+def bem_rhs(vorts, nodes, idx, area, fs, body=None, nunk=2, precision="comp", maxlev=-1):
+    """BEM right-hand side of one boundary surface (BEMHelper.h:83-117) on the GPU:
+    vorts is a list of (pos, str) particle collections; returns the panel-centre
+    velocities (finalized, body motion removed) and the rhs (nunk * P,)."""
+    D = _lib.DeviceArray.from_host
+    nodes, area = _c32(nodes), _c32(area)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    P = area.shape[0]
+    dn, di, da = D(nodes), D(idx.view(np.float32)), D(area)
+    pan = _lib.Panels()
+    pan.np, pan.nn = P, nodes.shape[1]
+    for d in range(3):
+        pan.x[d] = dn.row(d)
+    pan.idx, pan.area = di.ptr, da.ptr
+    keep, pts = [], (_lib.Points * max(1, len(vorts)))()
+    for k, (pos, st) in enumerate(vorts):
+        dp, ds = D(_c32(pos)), D(_c32(st))
+        keep += [dp, ds]
+        pts[k].n = pos.shape[1]
+        for d in range(3):
+            pts[k].x[d] = dp.row(d)
+            pts[k].s[d] = ds.row(d)
+    dpu = _lib.DeviceArray((3, max(P, 1)))
+    drhs = _lib.DeviceArray((max(nunk * P, 1),))
+    pu = (C.c_void_p * 3)(*[dpu.row(d) for d in range(3)])
+    o = make_opts("rm", False, precision, 0, 0, 0, maxlev)
+    b = bem_opts(fs, body, nunk)
+    check(lib.o3d_bem_rhs_dev(pts, len(vorts), C.byref(pan), C.byref(b), C.byref(o), pu, C.c_void_p(drhs.ptr)))
+    _lib.synchronize()
+    return dpu.to_host()[:, :P], drhs.to_host()[:nunk * P]
+
+
 def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, precision="comp", maxlev=-1,
         vel=None, grad=None, stats=False, rule="recursive", tile=0):
     """Host-array panel->point sum on the GPU: rule "recursive" is the reference's

@@ -44,7 +44,7 @@ def lib():
     if _lib is None:
         _lib = _load(os.path.join(HERE, "liboracle.so"))
         for name in ("orc_p2p_f", "orc_p2p_d", "orc_pan_f", "orc_pan_d", "orc_p2p_vcstyle", "orc_tri_integrals",
-                     "orc_pan_g7_near", "orc_pan_g7_d"):
+                     "orc_pan_g7_near", "orc_pan_g7_d", "orc_bem_rhs"):
             getattr(_lib, name).restype = C.c_int
         _lib.orc_flops_pair.restype = C.c_int
         _lib.orc_flops_pair.argtypes = [C.c_int] * 4
@@ -225,6 +225,28 @@ def pan_g7(nodes, idx, ts, area, ss, tpos, grads=True, nthreads=0):
     if rc != 0:
         raise RuntimeError(f"oracle pan_g7 failed rc={rc}")
     return vel, grad, nnear.value
+
+
+# ---- BEM right-hand side from panel velocities (o3d_bem.cpp) ------------------
+
+def bem_rhs(nodes, idx, pu, fs, body=False, vel=(0, 0, 0), rotvel=(0, 0, 0), tc=(0, 0, 0), nunk=2):
+    """finalize_vels + add_body_motion(-1) + vels_to_rhs_panels (Surfaces.h, RHS.h) on
+    the panel velocities pu (3, P) as points_affect_panels left them; returns
+    (finalized pu, rhs (nunk * P,)), float32"""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    nodes = _as(nodes, np.float32)
+    pu = np.array(pu, dtype=np.float32, copy=True)
+    P = pu.shape[1]
+    rhs = np.zeros(nunk * P, np.float32)
+    d3 = lambda v: np.ascontiguousarray(v, np.float64)
+    f3 = lambda v: np.ascontiguousarray(v, np.float32)
+    fs_, vel_, rv_, tc_ = d3(fs), d3(vel), d3(rotvel), f3(tc)
+    rc = lib().orc_bem_rhs(_i64(P), _ptr(nodes[0]), _ptr(nodes[1]), _ptr(nodes[2]), _ptr(idx),
+                           _ptr(pu[0]), _ptr(pu[1]), _ptr(pu[2]), _ptr(fs_), int(bool(body)), _ptr(vel_), _ptr(rv_),
+                           _ptr(tc_), int(nunk), _ptr(rhs))
+    if rc != 0:
+        raise RuntimeError(f"oracle bem_rhs failed rc={rc}")
+    return pu, rhs
 
 
 # ---- the reference's own kernels (oracle/_ref), same conventions -------------

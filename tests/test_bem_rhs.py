@@ -1,0 +1,76 @@
+"""The oracle's BEM right-hand side from panel velocities (oracle/o3d_bem.cpp:
+finalize_vels, add_body_motion and vels_to_rhs_panels as BEMHelper.h:83-117 chains
+them), pinned by mathematics in FP64 from the panel geometry:
+
+  * the panel bases recovered from the right-hand side of unit freestreams are
+    orthonormal, x1 along the first edge, x2 toward the third node, the normal
+    along (n1 - n0) x (n2 - n0);
+  * the freestream alone gives rhs = -(fs . basis);
+  * raw velocities are finalized as fs + v / (4 pi);
+  * a rigid rotation about tc subtracts omega x (centroid - tc);
+  * a body translating with the stream leaves exactly zero;
+  * the unknown counts 1, 2 and 3 lay out the same components.
+The float rounding sequence itself is pinned by reading only: Surfaces.h needs
+Eigen3 and cannot be compiled here.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from omega3d_amd import scenes
+
+
+@pytest.fixture(scope="module")
+def sphere():
+    return scenes.icosphere_panels(levels=2)
+
+
+def _geom(pan):
+    P = pan.area.shape[0]
+    v = pan.nodes.T.astype(np.float64)[pan.idx.reshape(P, 3)]
+    return P, v
+
+
+def test_bases_from_unit_freestreams(sphere):
+    P, v = _geom(sphere)
+    cols = []
+    for e in np.eye(3):
+        _, rhs = oracle.bem_rhs(sphere.nodes, sphere.idx, np.zeros((3, P)), e, nunk=3)
+        cols.append(-rhs.reshape(P, 3))          # (P, 3): component e of x1, x2, n
+    B = np.stack(cols, axis=2)                   # B[p, k, e]: basis vector k, component e
+    x1, x2, n = B[:, 0], B[:, 1], B[:, 2]
+    for a, b_, want in ((x1, x1, 1), (x2, x2, 1), (n, n, 1), (x1, x2, 0), (x1, n, 0), (x2, n, 0)):
+        assert np.abs((a * b_).sum(1) - want).max() < 3e-6
+    e01, e02 = v[:, 1] - v[:, 0], v[:, 2] - v[:, 0]
+    assert np.abs(x1 - e01 / np.linalg.norm(e01, axis=1)[:, None]).max() < 3e-6
+    nn = np.cross(e01, e02)
+    assert np.abs(n - nn / np.linalg.norm(nn, axis=1)[:, None]).max() < 3e-6
+    assert ((x2 * e02).sum(1) > 0).all()
+
+
+def test_freestream_and_finalize(sphere):
+    P, v = _geom(sphere)
+    rng = np.random.default_rng(4)
+    raw = rng.normal(size=(3, P)).astype(np.float32)
+    fs = np.array([0.3, -0.2, 1.1])
+    pu, rhs = oracle.bem_rhs(sphere.nodes, sphere.idx, raw, fs, nunk=2)
+    want = fs[:, None] + raw.astype(np.float64) / (4 * np.pi)
+    assert np.abs(pu - want).max() < 1e-6
+    _, rhs3 = oracle.bem_rhs(sphere.nodes, sphere.idx, raw, fs, nunk=3)
+    _, rhs1 = oracle.bem_rhs(sphere.nodes, sphere.idx, raw, fs, nunk=1)
+    r3 = rhs3.reshape(P, 3)
+    np.testing.assert_array_equal(rhs.reshape(P, 2), r3[:, :2])
+    np.testing.assert_array_equal(rhs1, r3[:, 2])
+
+
+def test_rigid_rotation_and_translation_with_the_stream(sphere):
+    P, v = _geom(sphere)
+    w = np.array([0.2, -0.7, 0.4])
+    tc = np.array([0.05, -0.02, 0.01], np.float32)
+    fs = np.array([1.0, 0.5, -0.25])
+    pu, _ = oracle.bem_rhs(sphere.nodes, sphere.idx, np.zeros((3, P)), fs, body=True, vel=(0, 0, 0), rotvel=w, tc=tc)
+    c = v.mean(1) - tc.astype(np.float64)
+    want = fs[:, None] - np.cross(w, c).T
+    assert np.abs(pu - want).max() < 1e-6
+    pu, rhs = oracle.bem_rhs(sphere.nodes, sphere.idx, np.zeros((3, P)), fs, body=True, vel=fs, nunk=3)
+    assert not pu.any() and not rhs.any()
