@@ -1,0 +1,318 @@
+"""Scene input and particle output in the reference's formats.
+
+JSON scenes (JsonHelper.cpp:55-190, FlowFeature.cpp:43-67 and each feature's
+from_json / init_elements):
+  * ``simparams.nominalDt``, ``flowparams.Re`` and ``flowparams.Uinf`` set the time
+    step, Reynolds number and freestream; the particle spacing is
+    ips = sqrt(8) * sqrt(dt / Re) and the core radius vdelta = 1.5 * ips
+    (Simulation.cpp:57-58, 77-79; Points.h:102-103);
+  * ``flowstructures`` of type "single particle", "vortex blob", "uniform block",
+    "block of random", "singular ring", "thick ring" and "particle emitter" become
+    particle collections by the reference's generators, restated here in their own
+    float/double mixing (FlowFeature.cpp:205-1019).  "block of random" is unseeded
+    in the reference (std::random_device, :561-562); here it takes a seed.  A
+    particle emitter adds one particle per step (step_elements, :702-712) and none
+    at start.  Disabled features are skipped.  ``bodies`` (boundary meshes) are
+    reported, not meshed.
+
+VTU output of a particle collection (Points.h:851-1039 through VtkXmlWriter.h):
+byte for byte what the reference writes -- the same element and attribute order
+(tinyxml2's printer, attributes in std::map order), base64 data whose UInt32 length
+header holds the length of the base64 text (the reference's own convention).
+
+The reference's generators cannot be compiled here (FlowFeature.h needs Eigen3
+through Body.h), so their restatement is pinned by geometry in tests/test_scene.py;
+the VTU writer is pinned byte for byte against the reference's own writer.
+"""
+import base64
+import json
+import math
+import struct
+
+import numpy as np
+
+f32 = np.float32
+
+
+def _f(x):
+    return f32(x)
+
+
+# ---- particle output ---------------------------------------------------------------
+
+def _b64_block(raw: bytes) -> str:
+    enc = base64.b64encode(raw).decode("ascii")
+    hdr = base64.b64encode(struct.pack("<I", len(enc))).decode("ascii")
+    return " " + hdr + enc + " "
+
+
+def _data_array(indent, attrs, raw):
+    a = " ".join(f'{k}="{v}"' for k, v in sorted(attrs.items()))
+    return f'{indent}<DataArray {a} format="binary">{_b64_block(raw)}</DataArray>\n'
+
+
+def points_vtu_bytes(pos, time, strength=None, radius=None, vel=None):
+    """The VTU file Points::write_vtk writes for n particles: pos (3, n), strength (3, n)
+    and radius (n,) for an active collection (None for field points), vel (3, n)."""
+    pos = np.asarray(pos, f32)
+    n = pos.shape[1]
+    vel = np.zeros((3, n), f32) if vel is None else np.asarray(vel, f32)
+    inter = lambda a: np.ascontiguousarray(np.asarray(a, f32).T).tobytes()
+    I1, I2, I3, I4 = " " * 4, " " * 8, " " * 12, " " * 16
+    out = ['<?xml version="1.0"?>\n',
+           '<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian" header_type="UInt32">\n',
+           f"{I1}<UnstructuredGrid>\n", f"{I2}<FieldData>\n",
+           _data_array(I3, {"Name": "TimeValue", "NumberOfTuples": "1", "type": "Float64"}, struct.pack("<d", time)),
+           f"{I2}</FieldData>\n",
+           f'{I2}<Piece NumberOfCells="{n}" NumberOfPoints="{n}">\n',
+           f"{I3}<Points>\n",
+           _data_array(I4, {"Name": "position", "NumberOfComponents": "3", "type": "Float32"}, inter(pos)),
+           f"{I3}</Points>\n", f"{I3}<Cells>\n",
+           _data_array(I4, {"Name": "connectivity", "type": "Int32"}, np.arange(n, dtype="<i4").tobytes()),
+           _data_array(I4, {"Name": "offsets", "type": "Int32"}, np.arange(1, n + 1, dtype="<i4").tobytes()),
+           _data_array(I4, {"Name": "types", "type": "UInt8"}, np.ones(n, np.uint8).tobytes()),
+           f"{I3}</Cells>\n"]
+    pd = {"Vectors": "velocity"}
+    if radius is not None:
+        pd["Scalars"] = "radius"
+    out.append(f"{I3}<PointData " + " ".join(f'{k}="{v}"' for k, v in sorted(pd.items())) + ">\n")
+    if strength is not None:
+        out.append(_data_array(I4, {"Name": "circulation", "NumberOfComponents": "3", "type": "Float32"},
+                               inter(strength)))
+    if radius is not None:
+        out.append(_data_array(I4, {"Name": "radius", "type": "Float32"}, np.asarray(radius, "<f4").tobytes()))
+    out.append(_data_array(I4, {"Name": "velocity", "NumberOfComponents": "3", "type": "Float32"}, inter(vel)))
+    out += [f"{I3}</PointData>\n", f"{I2}</Piece>\n", f"{I1}</UnstructuredGrid>\n", "</VTKFile>\n"]
+    return "".join(out).encode("ascii")
+
+
+def vtu_name(index, frame, inert=False):
+    """part_II_FFFFF.vtu / fldpt_II_FFFFF.vtu (Points.h:866-867)"""
+    return f"{'fldpt_' if inert else 'part_'}{index:02d}_{frame:05d}.vtu"
+
+
+def write_points_vtu(path, pos, time, strength=None, radius=None, vel=None):
+    with open(path, "wb") as fh:
+        fh.write(points_vtu_bytes(pos, time, strength, radius, vel))
+    return path
+
+
+def read_points_vtu(path):
+    """The arrays of a particle VTU written in this format: {name: array}, plus
+    TimeValue; vectors come back as (3, n)."""
+    import xml.etree.ElementTree as ET
+    root = ET.parse(path).getroot()
+    types = {"Float32": "<f4", "Float64": "<f8", "Int32": "<i4", "UInt8": "u1"}
+    out = {}
+    for da in root.iter("DataArray"):
+        text = da.text.strip()
+        hlen = 8                                       # base64 of the 4-byte header
+        raw = base64.b64decode(text[hlen:])
+        a = np.frombuffer(raw, types[da.get("type")])
+        nc = int(da.get("NumberOfComponents", "1"))
+        out[da.get("Name")] = a.reshape(-1, nc).T.copy() if nc > 1 else a.copy()
+    return out
+
+
+# ---- JSON scenes ---------------------------------------------------------------------
+
+def spacing(dt, re):
+    """(ips, vdelta) of Simulation::get_ips / get_vdelta (Simulation.cpp:57-58, 77-79)"""
+    core_size_ratio = _f(math.sqrt(8.0))
+    hnu = np.sqrt(_f(dt) / _f(re))                   # float dt, re (Simulation.h:170-171)
+    ips = _f(core_size_ratio * hnu)
+    return ips, _f(_f(1.5) * ips)
+
+
+def _normalize(v):
+    """normalizeVec (MathHelper.h:185-192) in float"""
+    v = [_f(x) for x in v]
+    s = _f(_f(v[0] * v[0]) + _f(v[1] * v[1]))
+    s = _f(s + _f(v[2] * v[2]))
+    ln = _f(1.0 / float(np.sqrt(s)))
+    return [_f(x * ln) for x in v]
+
+
+def _onb(n):
+    """branchlessONB (MathHelper.h:214-226) in float"""
+    sign = _f(math.copysign(1.0, n[2]))
+    a = _f(-1.0 / float(_f(sign + n[2])))
+    b = _f(_f(n[0] * n[1]) * a)
+    b1 = [_f(1.0 + float(_f(_f(_f(sign * n[0]) * n[0]) * a))), _f(sign * b), _f(-sign * n[0])]
+    b2 = [b, _f(sign + _f(_f(n[1] * n[1]) * a)), _f(-n[1])]
+    return b1, b2
+
+
+def _cs(theta):
+    t = float(theta)
+    return _f(math.cos(t)), _f(math.sin(t))
+
+
+def singular_ring(center, normal, majrad, circ, ips):
+    """SingularRing::init_elements (FlowFeature.cpp:788-831)"""
+    majrad, circ, ips = _f(majrad), _f(circ), _f(ips)
+    ndiam = int(1 + (2.0 * math.pi * float(majrad)) / float(ips))
+    this_ips = _f((2.0 * math.pi * float(majrad)) / float(_f(ndiam)))
+    n = _normalize(normal)
+    b1, b2 = _onb(n)
+    pos = np.zeros((3, ndiam), f32)
+    st = np.zeros((3, ndiam), f32)
+    for i in range(ndiam):
+        theta = _f(2.0 * math.pi * float(_f(i)) / float(_f(ndiam)))
+        c, s = _cs(theta)
+        for d in range(3):
+            pos[d, i] = _f(_f(center[d]) + _f(majrad * _f(_f(b1[d] * c) + _f(b2[d] * s))))
+            st[d, i] = _f(_f(this_ips * circ) * _f(_f(b2[d] * c) - _f(b1[d] * s)))
+    return pos, st
+
+
+def thick_ring(center, normal, majrad, minrad, circ, ips):
+    """ThickRing::init_elements (FlowFeature.cpp:940-1019)"""
+    majrad, minrad, circ, ips = _f(majrad), _f(minrad), _f(circ), _f(ips)
+    nlayers = int(1 + float(minrad) / float(ips))
+    disk = [(_f(0.0), _f(0.0), _f(1.0))]
+    for l in range(1, nlayers):
+        thisrad = _f(_f(l) * ips)
+        nthislayer = int(1 + (2.0 * math.pi * float(thisrad)) / float(ips))
+        for i in range(nthislayer):
+            phi = _f(2.0 * math.pi * float(_f(i)) / float(_f(nthislayer)))
+            c, s = _cs(phi)
+            disk.append((_f(thisrad * c), _f(thisrad * s), _f(_f(majrad + _f(thisrad * c)) / majrad)))
+    nd = len(disk)
+    ndiam = int(1 + (2.0 * math.pi * float(majrad)) / float(ips))
+    this_ips = _f((2.0 * math.pi * float(majrad)) / float(_f(ndiam)))
+    n = _normalize(normal)
+    b1, b2 = _onb(n)
+    pos = np.zeros((3, ndiam * nd), f32)
+    st = np.zeros((3, ndiam * nd), f32)
+    k = 0
+    for i in range(ndiam):
+        theta = _f(2.0 * math.pi * float(_f(i)) / float(_f(ndiam)))
+        ct, stt = _cs(theta)
+        for (dx, dy, dl) in disk:
+            sscale = _f(_f(_f(dl * this_ips) * circ) / _f(nd))
+            for d in range(3):
+                pos[d, k] = _f(_f(_f(center[d]) + _f(_f(majrad + dx) * _f(_f(b1[d] * ct) + _f(b2[d] * stt))))
+                               + _f(dy * n[d]))
+                st[d, k] = _f(sscale * _f(_f(b2[d] * ct) - _f(b1[d] * stt)))
+            k += 1
+    return pos, st
+
+
+def vortex_blob(center, strength, rad, softness, ips):
+    """VortexBlob::init_elements (FlowFeature.cpp:290-343)"""
+    rad, soft, ips = _f(rad), _f(softness), _f(ips)
+    irad = int(1 + (float(rad) + 0.5 * float(soft)) / float(ips))
+    xs, ws = [], []
+    for i in range(-irad, irad + 1):
+        for j in range(-irad, irad + 1):
+            for k in range(-irad, irad + 1):
+                dr = _f(_f(np.sqrt(_f(i * i + j * j + k * k))) * ips)
+                if float(dr) < float(rad) + 0.5 * float(soft):
+                    xs.append([_f(_f(center[0]) + _f(ips * _f(i))), _f(_f(center[1]) + _f(ips * _f(j))),
+                               _f(_f(center[2]) + _f(ips * _f(k)))])
+                    w = 1.0
+                    if float(dr) > float(rad) - 0.5 * float(soft):
+                        w = 0.5 - 0.5 * math.sin(math.pi * (float(dr) - float(rad)) / float(soft))
+                    ws.append(w)
+    tot = 0.0
+    for w in ws:                                     # sequential double sum, as the reference
+        tot += w
+    scale = 1.0 / tot
+    pos = np.array(xs, f32).T.copy() if xs else np.zeros((3, 0), f32)
+    st = np.zeros((3, len(ws)), f32)
+    for i, w in enumerate(ws):
+        for d in range(3):
+            st[d, i] = _f(float(_f(_f(strength[d]) * _f(w))) * scale)
+    return pos, st
+
+
+def uniform_block(center, size, strength, ips):
+    """UniformBlock::init_elements (FlowFeature.cpp:408-440); as in the reference, the
+    third count is taken from the y size"""
+    ips = _f(ips)
+    isz = int(1 + float(_f(size[0])) / float(ips))
+    jsz = int(1 + float(_f(size[1])) / float(ips))
+    ksz = int(1 + float(_f(size[1])) / float(ips))
+    tot = isz * jsz * ksz
+    each = _f(1.0 / float(_f(tot)))
+    pos = np.zeros((3, tot), f32)
+    st = np.zeros((3, tot), f32)
+    m = 0
+    for i in range(isz):
+        for j in range(jsz):
+            for k in range(ksz):
+                for d, (q, nq) in enumerate(((i, isz), (j, jsz), (k, ksz))):
+                    pos[d, m] = _f(float(_f(center[d])) + float(_f(size[d])) * ((float(q) + 0.5) / float(nq) - 0.5))
+                    st[d, m] = _f(_f(strength[d]) * each)
+                m += 1
+    return pos, st
+
+
+def block_of_random(center, size, maxstr, num, seed=0):
+    """BlockOfRandom::init_elements (FlowFeature.cpp:552-590) with a seeded generator
+    (the reference draws from an unseeded std::random_device)"""
+    rng = np.random.default_rng(seed)
+    pos = np.array([_f(center[d]) + _f(size[d]) * rng.uniform(-0.5, 0.5, num) for d in range(3)], f32)
+    st = (np.float32(maxstr) * rng.uniform(-0.5, 0.5, (3, num)) / np.float32(num)).astype(f32)
+    return pos, st
+
+
+class Scene:
+    """A parsed JSON scene: time step, Reynolds number, freestream, particle spacing
+    and core radius, and the particle collections of its enabled flow structures."""
+
+    def __init__(self, doc, seed=0):
+        if isinstance(doc, (str, bytes)) and not str(doc).lstrip().startswith("{"):
+            with open(doc) as fh:
+                doc = json.load(fh)
+        elif isinstance(doc, (str, bytes)):
+            doc = json.loads(doc)
+        ver = doc.get("version", {})
+        if "Omega2D" in ver:
+            raise ValueError("input file is an Omega2D file (JsonHelper.cpp:69-72)")
+        self.description = doc.get("description", "")
+        sp, fp = doc.get("simparams", {}), doc.get("flowparams", {})
+        self.dt = float(sp.get("nominalDt", 0.01))
+        self.viscous = sp.get("viscous", "none")
+        self.re = float(fp.get("Re", 100.0))
+        self.fs = [float(x) for x in fp.get("Uinf", [0.0, 0.0, 0.0])]
+        self.ips, self.vdelta = spacing(self.dt, self.re)
+        self.bodies = doc.get("bodies", [])
+        self.features = []       # (type, dict) of the enabled flow structures
+        self.emitters = []       # (position, strength) added each step
+        pos, st = [], []
+        for k, ff in enumerate(doc.get("flowstructures", [])):
+            if "type" not in ff or not ff.get("enabled", True):
+                continue
+            t = ff["type"]
+            c = ff.get("center", [0.0, 0.0, 0.0])
+            if t == "single particle":
+                p, s = np.array(c, f32).reshape(3, 1), np.array(ff["strength"], f32).reshape(3, 1)
+            elif t == "vortex blob":
+                p, s = vortex_blob(c, ff["strength"], ff["radius"], ff["softness"], self.ips)
+            elif t == "uniform block":
+                p, s = uniform_block(c, ff["size"], ff["strength"], self.ips)
+            elif t == "block of random":
+                p, s = block_of_random(c, ff["size"], ff["max strength"], int(ff["num"]), seed + k)
+            elif t == "singular ring":
+                p, s = singular_ring(c, ff["normal"], ff["major radius"], ff["circulation"], self.ips)
+            elif t == "thick ring":
+                p, s = thick_ring(c, ff["normal"], ff["major radius"], ff["minor radius"], ff["circulation"],
+                                  self.ips)
+            elif t == "particle emitter":
+                self.emitters.append((np.array(c, f32), np.array(ff["strength"], f32)))
+                self.features.append((t, ff))
+                continue
+            else:
+                continue                  # FlowFeature.cpp:58-61: not an available flow feature
+            self.features.append((t, ff))
+            pos.append(p)
+            st.append(s)
+        self.pos = np.concatenate(pos, axis=1) if pos else np.zeros((3, 0), f32)
+        self.str = np.concatenate(st, axis=1) if st else np.zeros((3, 0), f32)
+        self.rad = np.full(self.pos.shape[1], self.vdelta, f32)
+
+    @property
+    def n(self):
+        return self.pos.shape[1]

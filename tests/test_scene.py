@@ -1,0 +1,142 @@
+"""Scene input and particle output (omega3d_amd/scene.py).
+
+VTU output is checked byte for byte against files the reference's own writer
+produced (tests/golden/vtk_*.vtu from VtkXmlWriter.h, see make_vtk_golden.py), and
+against that writer live when oracle/_ref holds it.  The JSON scene generators
+cannot be built from the reference (FlowFeature.h needs Eigen3), so they are
+pinned by geometry: ring particles on the circle, in the plane normal to the axis,
+tangent strengths summing to circulation x circumference; the reference's counts
+(1 + 2 pi R / ips stations, layered disks); blob and block strengths summing to the
+given strength; the example scenes of the reference parse with the expected counts.
+"""
+import ctypes as C
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from omega3d_amd import scene
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+REFVTK = os.path.join(os.path.dirname(os.path.dirname(__file__)), "oracle", "_ref", "libo3dref_vtk.so")
+
+
+@pytest.mark.parametrize("name", ["vtk_part", "vtk_fldpt", "vtk_one"])
+def test_vtu_bytes_match_the_reference_writer(name):
+    g = np.load(os.path.join(GOLD, "vtk_inputs.npz"))
+    inert = bool(g[f"{name}_inert"])
+    got = scene.points_vtu_bytes(g[f"{name}_pos"], float(g[f"{name}_time"]),
+                                 None if inert else g[f"{name}_str"], None if inert else g[f"{name}_rad"],
+                                 g[f"{name}_vel"])
+    want = open(os.path.join(GOLD, name + ".vtu"), "rb").read()
+    assert got == want
+
+
+@pytest.mark.skipif(not os.path.exists(REFVTK), reason="oracle/_ref/libo3dref_vtk.so not built")
+def test_vtu_bytes_match_the_live_reference_writer(tmp_path):
+    L = C.CDLL(REFVTK)
+    L.ref_write_points_vtk.argtypes = [C.c_char_p, C.c_int64] + [C.c_void_p] * 10 + [C.c_double, C.c_int]
+    rng = np.random.default_rng(3)
+    for n, inert in ((1000, 0), (333, 1)):
+        a = [np.ascontiguousarray(rng.normal(size=n).astype(np.float32)) for _ in range(10)]
+        fn = str(tmp_path / f"r{n}.vtu")
+        L.ref_write_points_vtk(fn.encode(), n, *[x.ctypes.data_as(C.c_void_p) for x in a], 2.5, inert)
+        got = scene.points_vtu_bytes(np.stack(a[0:3]), 2.5, None if inert else np.stack(a[3:6]),
+                                     None if inert else a[6], np.stack(a[7:10]))
+        assert got == open(fn, "rb").read(), (n, inert)
+
+
+def test_vtu_round_trip(tmp_path):
+    rng = np.random.default_rng(1)
+    pos, st, vel = (rng.normal(size=(3, 50)).astype(np.float32) for _ in range(3))
+    rad = rng.uniform(0.01, 0.1, 50).astype(np.float32)
+    fn = scene.write_points_vtu(str(tmp_path / scene.vtu_name(0, 12)), pos, 0.5, st, rad, vel)
+    assert os.path.basename(fn) == "part_00_00012.vtu"
+    d = scene.read_points_vtu(fn)
+    np.testing.assert_array_equal(d["position"], pos)
+    np.testing.assert_array_equal(d["circulation"], st)
+    np.testing.assert_array_equal(d["radius"], rad)
+    np.testing.assert_array_equal(d["velocity"], vel)
+    assert d["TimeValue"][0] == 0.5 and (d["connectivity"] == np.arange(50)).all()
+
+
+def _ring_checks(pos, st, c, nrm, R, circ, ips):
+    n = np.asarray(nrm, float) / np.linalg.norm(nrm)
+    d = pos.astype(np.float64) - np.asarray(c, float)[:, None]
+    assert np.abs(np.linalg.norm(d, axis=0) - R).max() < 2e-6 * max(1, R)
+    assert np.abs(n @ d).max() < 2e-6
+    a = st.astype(np.float64)
+    assert np.abs((a * d).sum(0)).max() < 1e-6 * np.abs(a).max() * R and np.abs(n @ a).max() < 1e-6
+    nd = int(1 + 2 * math.pi * R / ips)
+    assert pos.shape[1] == nd
+    assert abs(np.linalg.norm(a, axis=0).sum() - abs(circ) * 2 * math.pi * R) < 1e-5 * abs(circ) * 2 * math.pi * R
+    # circulation sense: alpha along n x d
+    assert (np.sign(circ) * (np.cross(n, d.T) * a.T).sum(1) > 0).all()
+
+
+def test_example_scenes_parse():
+    counts = {}
+    for f in sorted(os.listdir(os.path.join(GOLD, "scenes"))):
+        s = scene.Scene(os.path.join(GOLD, "scenes", f))
+        counts[f] = s.n
+        assert s.rad.shape == (s.n,) and np.all(s.rad == s.vdelta)
+    assert counts["colliding_vortex_rings_nv.json"] == 2 * 158
+    assert counts["flow_over_sphere.json"] == 0           # a body and no flow structures
+
+
+def test_singular_rings_of_an_example():
+    doc = json.load(open(os.path.join(GOLD, "scenes", "colliding_vortex_rings_nv.json")))
+    s = scene.Scene(doc)
+    assert abs(float(s.ips) - math.sqrt(8) * math.sqrt(0.002 / 40.000003814697266)) < 1e-7
+    off = 0
+    for ff in doc["flowstructures"]:
+        nd = int(1 + 2 * math.pi * ff["major radius"] / float(s.ips))
+        _ring_checks(s.pos[:, off:off + nd], s.str[:, off:off + nd], ff["center"], ff["normal"],
+                     ff["major radius"], ff["circulation"], float(s.ips))
+        off += nd
+
+
+def test_thick_ring_blob_block_and_rules():
+    doc = {"simparams": {"nominalDt": 0.01}, "flowparams": {"Re": 200.0},
+           "flowstructures": [
+               {"type": "thick ring", "center": [0.1, 0.2, -0.3], "normal": [0.0, 0.0, 1.0], "major radius": 1.0,
+                "minor radius": 0.05, "circulation": 2.0},
+               {"type": "vortex blob", "center": [2, 0, 0], "strength": [0.0, 0.5, 1.0], "radius": 0.1,
+                "softness": 0.04},
+               {"type": "uniform block", "center": [-2, 0, 0], "size": [0.2, 0.1, 0.3], "strength": [1, 2, 3]},
+               {"type": "single particle", "center": [5, 5, 5], "strength": [0, 0, 1], "enabled": False},
+               {"type": "particle emitter", "center": [0, 0, 5], "strength": [0, 0, 1]},
+               {"type": "no such feature", "center": [0, 0, 0]}]}
+    s = scene.Scene(doc)
+    assert [t for t, _ in s.features] == ["thick ring", "vortex blob", "uniform block", "particle emitter"]
+    ips = float(s.ips)
+    # thick ring: stations x disk particles, all within the minor radius of the core circle
+    pos, st = scene.thick_ring([0.1, 0.2, -0.3], [0, 0, 1], 1.0, 0.05, 2.0, s.ips)
+    nlay = int(1 + 0.05 / ips)
+    ndisk = 1 + sum(int(1 + 2 * math.pi * (l * ips) / ips) for l in range(1, nlay))
+    nst = int(1 + 2 * math.pi / ips)
+    assert pos.shape[1] == nst * ndisk
+    d = pos.astype(np.float64) - np.array([[0.1], [0.2], [-0.3]])
+    rho = np.hypot(d[0], d[1])
+    assert np.max(np.hypot(rho - 1.0, d[2])) <= 0.05 + 1e-6
+    # per station the disk's strengths sum to this_ips * circulation * (mean of (R + x)/R)
+    tang = np.stack([-d[1] / rho, d[0] / rho, 0 * rho])
+    total = (st.astype(np.float64) * tang).sum()
+    assert abs(total - 2.0 * 2 * math.pi * 1.0) < 2e-3 * 2.0 * 2 * math.pi
+    # blob: weights normalised, so the strengths sum to the given vector
+    pb, sb = scene.vortex_blob([2, 0, 0], [0.0, 0.5, 1.0], 0.1, 0.04, s.ips)
+    np.testing.assert_allclose(sb.astype(np.float64).sum(1), [0.0, 0.5, 1.0], atol=1e-6)
+    assert np.linalg.norm(pb - np.array([[2], [0], [0]]), axis=0).max() < 0.1 + 0.02 + 1e-6
+    # block: the third count follows the y size, as in the reference; strengths sum
+    pk, sk = scene.uniform_block([-2, 0, 0], [0.2, 0.1, 0.3], [1, 2, 3], s.ips)
+    ni, nj = int(1 + 0.2 / ips), int(1 + 0.1 / ips)
+    assert pk.shape[1] == ni * nj * nj
+    np.testing.assert_allclose(sk.astype(np.float64).sum(1), [1, 2, 3], rtol=1e-5)
+    assert s.n == pos.shape[1] + pb.shape[1] + pk.shape[1] and len(s.emitters) == 1
+
+
+def test_omega2d_files_are_rejected():
+    with pytest.raises(ValueError):
+        scene.Scene({"version": {"Omega2D": 1}})
