@@ -293,7 +293,7 @@ typedef struct o3d_batch o3d_batch;
  * opts->method: AUTO sends every instance of >= 65,536 particles with an RM or EXP
  * core in FP32 or compensated precision to the factored far-field method; FACTORED
  * sends all instances there (an instance it cannot take is an error); DIRECT keeps
- * all in the persistent queue.  Factored instances run on up to 4 streams the
+ * all in the persistent queue.  Factored instances run on up to 8 streams the
  * batch creates here, each with scratch allocated here; they fork from the
  * caller's stream after the zero fill and the caller's stream waits for them. */
 int o3d_batch_create(o3d_batch** b, int32_t ninst, const int64_t* off, const int64_t* n,

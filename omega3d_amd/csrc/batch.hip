@@ -9,7 +9,7 @@
 //
 // Instances large enough for the factored far field (the same rule as a single
 // call with method AUTO, or all of them with method FACTORED) leave the queue and
-// run that pipeline on FF_LANES streams of the batch (longest first, each to the
+// run that pipeline on FF_LANES (8) streams of the batch (longest first, each to the
 // least-loaded lane), each lane with scratch of its own.  The lanes fork from the
 // caller's stream after the zero fill and join it at the end, so they overlap
 // one another and the persistent kernel: an instance of 65,536 particles has
@@ -136,7 +136,7 @@ struct o3d_batch {
 
 namespace {
 
-constexpr size_t FF_LANES = 4;
+constexpr size_t FF_LANES = 8;
 
 int tpt_slot(int tpt) { return tpt == 1 ? 0 : (tpt == 2 ? 1 : 2); }
 
