@@ -242,6 +242,26 @@ def main():
             log(f"bench: cpu baseline {cpu}")
 
     from omega3d_amd import workloads
+    single = workloads.single_ring_run(log=log)
+    if not args.no_cpu:
+        # the same single ring on the host cores: the reference's scalar loop restated
+        # (float storage and accumulation) and the Vc-style loop
+        import oracle
+        p0 = scenes.ring(8192, sigma=0.05)
+        nth = oracle.lib().orc_max_threads()
+        t = time.perf_counter()
+        oracle.p2p("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, True, "f32f", nthreads=nth)
+        t_exact = time.perf_counter() - t
+        t_vc = None
+        if oracle.lib().orc_have_avx2():
+            t = time.perf_counter()
+            oracle.p2p_vcstyle("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, 0, p0.n, nth)
+            t_vc = time.perf_counter() - t
+        single["cpu"] = {"cores": nth, "scalar_float_ms": round(t_exact * 1e3, 2),
+                         "vc_style_ms": None if t_vc is None else round(t_vc * 1e3, 2),
+                         "kind": "port", "what": "the oracle's restatement of Influence.h:202-545 on the host "
+                                                 "cores (self pair included), full workload"}
+        log(f"bench: single ring on the CPU {single['cpu']}")
     batched = None if args.no_batch else workloads.batched(log=log)
     served = None if args.no_serve else workloads.served(duration=args.serve_seconds, log=log)
     panels = None if args.no_panels else workloads.panels_run(log=log)
@@ -303,6 +323,7 @@ def main():
         "direct_method": direct,
         "errors_vs_fp64": errors,
         "cpu_baseline": cpu,
+        "single_ring": single,
         "batched": batched,
         "served": served,
         "panels": panels,

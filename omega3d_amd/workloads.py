@@ -162,6 +162,55 @@ def served(duration=20.0, sessions=16, lo=1024, hi=1048576, load=0.5, rate=None,
     return out
 
 
+def single_ring_run(n=8192, reps=20, log=print):
+    """The 8,192-particle single ring (torus R=1, a=0.1, sigma 0.05, |alpha| = 1/N), RM
+    core, velocity + gradient at every particle with the self term excluded, plain
+    FP32: kernel time and the error against the FP64 kernel on all targets."""
+    L = _lib.lib
+    p = scenes.ring(n, sigma=0.05)
+    D = _lib.DeviceArray.from_host
+    dpos, dstr, drad = D(p.pos), D(p.str), D(p.rad)
+    dv, dg = _lib.DeviceArray((3, n)), _lib.DeviceArray((9, n))
+    s = _lib.Points()
+    s.n = n
+    for d in range(3):
+        s.x[d], s.s[d], s.u[d] = dpos.row(d), dstr.row(d), dv.row(d)
+    s.r = drad.ptr
+    for k in range(9):
+        s.ug[k] = dg.row(k)
+    o = _lib.make_opts("rm", True, "fp32", self_term=1)
+    e0, e1 = _lib.Event(), _lib.Event()
+    dv.zero(); dg.zero()
+    _lib.check(L.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None))
+    _lib.synchronize()
+    e0.record()
+    for _ in range(reps):
+        dv.zero(); dg.zero()
+        _lib.check(L.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None))
+    e1.record()
+    _lib.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    vel, grad = dv.to_host(), dg.to_host()
+    d64 = lambda a: _lib.DeviceArray.from_host(np.ascontiguousarray(a, np.float64))
+    p64, s64_, r64 = d64(p.pos), d64(p.str), d64(p.rad)
+    v64, g64 = d64(np.zeros((3, n))), d64(np.zeros((9, n)))
+    q = _lib.Points64()
+    q.n = n
+    for d in range(3):
+        q.x[d], q.s[d], q.u[d] = p64.row(d), s64_.row(d), v64.row(d)
+    q.r = r64.ptr
+    for k in range(9):
+        q.ug[k] = g64.row(k)
+    o64 = _lib.make_opts("rm", True, "fp64", self_term=1)
+    _lib.check(L.o3d_points_affect_points64_dev(C.byref(q), C.byref(q), C.byref(o64), None))
+    _lib.synchronize()
+    out = {"workload": f"single ring, N={n}, RM core, sigma 0.05, velocity + gradient, self term excluded, plain FP32",
+           "gpu_ms": round(ms, 4), "gpu_ginter_s": round(float(n) * n / ms / 1e6, 2),
+           "errors_vs_fp64": dict(field_errors(vel, grad, v64.to_host(), g64.to_host()), targets=n)}
+    log("single_ring: " + json.dumps(out))
+    return out
+
+
 def panels_run(n_targets=524288, reps=2, check_targets=4096, log=print):
     """The icosphere workload on the GPU: timing, and the error of a target subset
     against the FP64 panel kernel."""
