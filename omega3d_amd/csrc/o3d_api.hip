@@ -130,6 +130,41 @@ hipError_t dispatch_p2p(int core, const P2PArgs& a, bool thick, bool grad, int p
   }
   return hipErrorInvalidValue;
 }
+hipError_t dispatch_p2p_split(int core, const P2PArgs& a, bool thick, bool grad, int prec, int nslice, int64_t slen,
+                              void* part, hipStream_t st) {
+  switch (core) {
+    case CORE_RM: return launch_p2p_split_rm(a, thick, grad, prec, nslice, slen, part, st);
+    case CORE_WL: return launch_p2p_split_wl(a, thick, grad, prec, nslice, slen, part, st);
+    case CORE_EXP: return launch_p2p_split_exp(a, thick, grad, prec, nslice, slen, part, st);
+    case CORE_V2: return launch_p2p_split_v2(a, thick, grad, prec, nslice, slen, part, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+// Source split of a call whose targets make fewer workgroups than the GPU holds at
+// once: slices of whole tiles, enough of them for about SPLIT_WGS_PER_CU
+// workgroups per compute unit, and a partial buffer of at most SPLIT_MAX_BYTES.
+// Returns the slice count (1: no split) and the slice length.
+constexpr int SPLIT_WGS_PER_CU = 16;
+constexpr size_t SPLIT_MAX_BYTES = (size_t)256 << 20;
+int split_plan(int64_t nt, int64_t ns, int tile, bool grad, int prec, int64_t* slen, size_t* bytes) {
+  const int64_t nb = (nt + BLOCK - 1) / BLOCK;
+  const int64_t want = (int64_t)SPLIT_WGS_PER_CU * num_cus();
+  if (nb >= want) return 1;
+  int64_t ntiles = (ns + tile - 1) / tile;
+  int64_t nsl = (want + nb - 1) / nb;
+  if (nsl > ntiles) nsl = ntiles;
+  if (nsl > 65535) nsl = 65535;
+  if (nsl < 2) return 1;
+  const int64_t tiles_per = (ntiles + nsl - 1) / nsl;
+  *slen = tiles_per * tile;
+  nsl = (ns + *slen - 1) / *slen;
+  const size_t cols = prec == PREC_COMP ? (grad ? 12 : 3) * sizeof(double) : (grad ? 14 : 3) * sizeof(float);
+  *bytes = (size_t)nsl * (size_t)nt * cols;
+  if (nsl < 2 || *bytes > SPLIT_MAX_BYTES) return 1;
+  return (int)nsl;
+}
+
 hipError_t dispatch_p2p64(int core, const P2PArgs64& a, bool thick, bool grad, hipStream_t st) {
   switch (core) {
     case CORE_RM: return launch_p2p64_rm(a, thick, grad, st);
@@ -214,14 +249,30 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   a.nt = targ->n;
   a.self_excl = o->self_term == O3D_SELF_EXCLUDE;
   Tuning req{o->tile, o->tpt};
+  const bool auto_cfg = !tun && o->tile == 0 && o->tpt == 0;
   if (tun) req = *tun;
-  else if (o->tile == 0 && o->tpt == 0) {
+  else if (auto_cfg) {
     Tuning t;
     if (tuned_lookup(o->core, o->results, o->precision, thick, a.nt, &t)) req = t;
   }
   int tile, tpt;
   choose_config(req, a.nt, a.ns, grad, o->precision, &tile, &tpt);
   a.tile = tile;
+  // too few targets to fill the GPU: split the sources (a configuration left to the
+  // library only; an explicit tile or lane count keeps the one-pass order that the
+  // batched and served kernels share)
+  int64_t slen = 0;
+  size_t pbytes = 0;
+  const int nsl = auto_cfg && tpt == 1 ? split_plan(a.nt, a.ns, tile, grad, o->precision, &slen, &pbytes) : 1;
+  if (nsl > 1) {
+    std::lock_guard<std::mutex> lk(stream_ws_mutex());
+    void* part = nullptr;
+    int rc = stream_workspace(st, pbytes, &part);
+    if (rc != O3D_OK) return rc;
+    hipError_t e = dispatch_p2p_split(o->core, a, thick, grad, o->precision, nsl, slen, part, st);
+    if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
+    return O3D_OK;
+  }
   hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
   return O3D_OK;

@@ -288,13 +288,27 @@ __device__ __forceinline__ void stage_tile(const SrcView& s, int64_t s0, int cnt
   if (CORE == CORE_EXP) atomicMax(tmax, mx);
 }
 
+// what p2p_span does with a chunk's sums: overwrite the outputs, add into them (the
+// reference's semantics), or store them as one slice of a source-split evaluation
+// (no self term; k_p2p_combine adds the slices in order and finishes).
+enum { OUT_SET = 0, OUT_ADD = 1, OUT_PART = 2 };
+
+// partial sums per target and slice: FP32 keeps the raw accumulators (NACC floats,
+// 3 without the gradient), compensated keeps hi + lo and the FP64 gradient (12 or 3
+// doubles).  Layout [slice][column][target].
+template <bool GRAD, int PREC>
+struct PartCols {
+  static constexpr int n = PREC == PREC_COMP ? (GRAD ? 12 : 3) : (GRAD ? NACC : 3);
+};
+
 // One chunk of BLOCK*TPT targets [t_base, t_base + BLOCK*TPT) against sources
-// [0, ns) of (s, t).  Shared by the single-collection, batched and serving kernels
-// so that a target's result does not depend on which launch computed it.
-// ACC: add into the outputs (the reference's semantics); false: overwrite them.
-template <int CORE, bool THICK, bool GRAD, int PREC, int TPT, bool ACC = true>
-__device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, int64_t ns, int64_t nt,
-                                          int64_t t_base, int tile, bool self_excl, float4* sh) {
+// [s_lo, s_hi) of (s, t), in tiles that start at s_lo.  Shared by the
+// single-collection, batched and serving kernels so that a target's result does not
+// depend on which launch computed it.  part/slice: OUT_PART's destination.
+template <int CORE, bool THICK, bool GRAD, int PREC, int TPT, int OUT>
+__device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int64_t s_lo, int64_t s_hi, int64_t nt,
+                                         int64_t t_base, int tile, bool self_excl, float4* sh, void* part,
+                                         int64_t slice) {
   __shared__ unsigned int s_tmax;
   const int tid = threadIdx.x;
   float tx[TPT], ty[TPT], tz[TPT], tt[TPT];
@@ -320,8 +334,8 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
     for (int c = 0; c < 9; ++c) gd[k][c] = 0.0;
   }
 
-  for (int64_t s0 = 0; s0 < ns; s0 += tile) {
-    const int cnt = (int)((ns - s0) < tile ? (ns - s0) : tile);
+  for (int64_t s0 = s_lo; s0 < s_hi; s0 += tile) {
+    const int cnt = (int)((s_hi - s0) < tile ? (s_hi - s0) : tile);
     __syncthreads();
     if (CORE == CORE_EXP && tid == 0) s_tmax = 0u;
     __syncthreads();
@@ -339,6 +353,29 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
 #pragma unroll
       for (int k = 0; k < TPT; ++k) fold_tile_comp(acc[k], hi[k], lo[k], gd[k], GRAD);
     }
+  }
+
+  if (OUT == OUT_PART) {
+    constexpr int NC = PartCols<GRAD, PREC>::n;
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+      const int64_t i = ti[k];
+      if (i >= nt) continue;
+      if (PREC == PREC_FP32) {
+        float* pf = (float*)part + slice * NC * nt + i;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) pf[c * nt] = acc[k][c];
+      } else {
+        double* pd = (double*)part + slice * NC * nt + i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) pd[c * nt] = (double)hi[k][c] + (double)lo[k][c];
+        if (GRAD) {
+#pragma unroll
+          for (int c = 0; c < 9; ++c) pd[(3 + c) * nt] = gd[k][c];
+        }
+      }
+    }
+    return;
   }
 
   // self pair j == i: velocity and dyad parts vanish at d = 0; only K(0) alpha_i
@@ -370,7 +407,7 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
         acc[k][13] = fmaf(-ks[k], saz[k], acc[k][13]);
         grad_from<float>(acc[k], r + 3);
       }
-      if (ACC) {
+      if (OUT == OUT_ADD) {
         t.u[i] += r[0]; t.v[i] += r[1]; t.w[i] += r[2];
         if (GRAD) {
 #pragma unroll
@@ -400,7 +437,7 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
         r[10] = gd[k][7] + kx;
         r[11] = gd[k][8];
       }
-      if (ACC) {
+      if (OUT == OUT_ADD) {
         t.u[i] = (float)((double)t.u[i] + r[0]);
         t.v[i] = (float)((double)t.v[i] + r[1]);
         t.w[i] = (float)((double)t.w[i] + r[2]);
@@ -419,11 +456,89 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
   }
 }
 
+// targets [t_base, t_base + BLOCK*TPT) against all ns sources.  ACC: add into the
+// outputs (the reference's semantics); false: overwrite them.
+template <int CORE, bool THICK, bool GRAD, int PREC, int TPT, bool ACC = true>
+__device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, int64_t ns, int64_t nt,
+                                          int64_t t_base, int tile, bool self_excl, float4* sh) {
+  p2p_span<CORE, THICK, GRAD, PREC, TPT, ACC ? OUT_ADD : OUT_SET>(s, t, 0, ns, nt, t_base, tile, self_excl, sh,
+                                                                  nullptr, 0);
+}
+
 template <int CORE, bool THICK, bool GRAD, int PREC, int TPT>
 __global__ __launch_bounds__(BLOCK) void k_p2p(P2PArgs a) {
   extern __shared__ float4 sh[];
   const int64_t t_base = (int64_t)blockIdx.x * (BLOCK * TPT);
   p2p_chunk<CORE, THICK, GRAD, PREC, TPT>(a.s, a.t, a.ns, a.nt, t_base, a.tile, a.self_excl != 0, sh);
+}
+
+// ---------------------------------------------------------------------------
+// Source split for calls with too few targets to fill the GPU: blockIdx.y takes
+// sources [y*slen, (y+1)*slen), and its sums go to slice y of part.
+// ---------------------------------------------------------------------------
+template <int CORE, bool THICK, bool GRAD, int PREC>
+__global__ __launch_bounds__(BLOCK) void k_p2p_split(P2PArgs a, void* part, int64_t slen) {
+  extern __shared__ float4 sh[];
+  const int64_t t_base = (int64_t)blockIdx.x * BLOCK;
+  const int64_t lo = (int64_t)blockIdx.y * slen;
+  const int64_t hi = lo + slen < a.ns ? lo + slen : a.ns;
+  p2p_span<CORE, THICK, GRAD, PREC, 1, OUT_PART>(a.s, a.t, lo, hi, a.nt, t_base, a.tile, false, sh, part,
+                                                 blockIdx.y);
+}
+
+// adds the nslice slices of each target in slice order (in double), takes out the
+// self pair as p2p_span does, forms the gradient and adds into the outputs
+template <int CORE, bool THICK, bool GRAD, int PREC>
+__global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* part, int nslice) {
+  constexpr int NC = PartCols<GRAD, PREC>::n;
+  const int64_t nt = a.nt;
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= nt) return;
+  double sum[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) sum[c] = 0.0;
+  for (int y = 0; y < nslice; ++y) {
+    if (PREC == PREC_FP32) {
+      const float* pf = (const float*)part + (int64_t)y * NC * nt + i;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sum[c] += (double)pf[c * nt];
+    } else {
+      const double* pd = (const double*)part + (int64_t)y * NC * nt + i;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) sum[c] += pd[c * nt];
+    }
+  }
+  double r[12];
+  r[0] = sum[0]; r[1] = sum[1]; r[2] = sum[2];
+  if constexpr (GRAD) {
+    double kx = 0.0, ky = 0.0, kz = 0.0;
+    if (a.self_excl) {
+      float ks, Bp;
+      kb<CORE>(0.0f, core_term<CORE>(a.s.r[i]) + (THICK ? core_term<CORE>(a.t.r[i]) : 0.0f), 0.0f, ks, Bp);
+      kx = (double)ks * a.s.ax[i]; ky = (double)ks * a.s.ay[i]; kz = (double)ks * a.s.az[i];
+    }
+    if constexpr (PREC == PREC_FP32) {
+      sum[11] -= kx; sum[12] -= ky; sum[13] -= kz;
+      grad_from<double>(sum, r + 3);
+    } else {
+      r[3] = sum[3];
+      r[4] = sum[4] - kz;
+      r[5] = sum[5] + ky;
+      r[6] = sum[6] + kz;
+      r[7] = sum[7];
+      r[8] = sum[8] - kx;
+      r[9] = sum[9] - ky;
+      r[10] = sum[10] + kx;
+      r[11] = sum[11];
+    }
+  }
+  a.t.u[i] = (float)((double)a.t.u[i] + r[0]);
+  a.t.v[i] = (float)((double)a.t.v[i] + r[1]);
+  a.t.w[i] = (float)((double)a.t.w[i] + r[2]);
+  if (GRAD) {
+#pragma unroll
+    for (int c = 0; c < 9; ++c) a.t.g[c][i] = (float)((double)a.t.g[c][i] + r[3 + c]);
+  }
 }
 
 // ===========================================================================
@@ -556,6 +671,38 @@ hipError_t launch_p2p(const P2PArgs& a, bool thick, bool grad, int prec, int tpt
   return hipErrorInvalidValue;
 }
 
+// the source split: nslice slices of slen sources each, part holding
+// nslice * nt * PartCols columns
+template <int CORE, bool THICK, bool GRAD, int PREC>
+hipError_t launch_p2p_split_prec(const P2PArgs& a, int nslice, int64_t slen, void* part, hipStream_t st) {
+  const size_t lds = (size_t)a.tile * 2 * sizeof(float4);
+  const unsigned nb = (unsigned)((a.nt + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL((k_p2p_split<CORE, THICK, GRAD, PREC>), dim3(nb, (unsigned)nslice), dim3(BLOCK), lds, st, a,
+                     part, slen);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_p2p_combine<CORE, THICK, GRAD, PREC>), dim3(nb), dim3(BLOCK), 0, st, a, (const void*)part,
+                     nslice);
+  return hipGetLastError();
+}
+
+template <int CORE>
+hipError_t launch_p2p_split(const P2PArgs& a, bool thick, bool grad, int prec, int nslice, int64_t slen, void* part,
+                            hipStream_t st) {
+  if (a.nt <= 0 || a.ns <= 0) return hipSuccess;
+  if (nslice < 1 || nslice > 65535 || slen < 1 || (int64_t)nslice * slen < a.ns || (int64_t)(nslice - 1) * slen >= a.ns)
+    return hipErrorInvalidValue;
+#define O3D_L(TH, GR)                                                                                 \
+  if (thick == TH && grad == GR) {                                                                    \
+    if (prec == PREC_FP32) return launch_p2p_split_prec<CORE, TH, GR, PREC_FP32>(a, nslice, slen, part, st); \
+    if (prec == PREC_COMP) return launch_p2p_split_prec<CORE, TH, GR, PREC_COMP>(a, nslice, slen, part, st); \
+    return hipErrorInvalidValue;                                                                      \
+  }
+  O3D_L(true, true) O3D_L(true, false) O3D_L(false, true) O3D_L(false, false)
+#undef O3D_L
+  return hipErrorInvalidValue;
+}
+
 template <int CORE>
 hipError_t launch_p2p64(const P2PArgs64& a, bool thick, bool grad, hipStream_t st) {
   if (a.nt <= 0 || a.ns <= 0) return hipSuccess;
@@ -578,5 +725,9 @@ hipError_t launch_p2p64(const P2PArgs64& a, bool thick, bool grad, hipStream_t s
   }                                                                                            \
   hipError_t launch_p2p64_##NAME(const P2PArgs64& a, bool thick, bool grad, hipStream_t st) {  \
     return launch_p2p64<CORE>(a, thick, grad, st);                                             \
+  }                                                                                            \
+  hipError_t launch_p2p_split_##NAME(const P2PArgs& a, bool thick, bool grad, int prec, int nslice, \
+                                     int64_t slen, void* part, hipStream_t st) {               \
+    return launch_p2p_split<CORE>(a, thick, grad, prec, nslice, slen, part, st);               \
   }                                                                                            \
   }
