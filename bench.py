@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-instance batched workload")
     ap.add_argument("--no-serve", action="store_true", help="skip the served RK2 workload")
     ap.add_argument("--no-panels", action="store_true", help="skip the icosphere panel workload")
+    ap.add_argument("--no-bem", action="store_true", help="skip the BEM influence-matrix workload")
+    ap.add_argument("--bem-only", action="store_true", help="run only the BEM influence-matrix workload")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--sweep", action="store_true",
                     help="time every (precision, tile, tpt) on the N=1M workload, print them and exit")
@@ -63,6 +65,44 @@ def parse():
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def bem_leg(args, log):
+    """The BEM influence matrix of the icosphere (omega3d_amd.workloads.bem_matrix_run),
+    with its CPU baseline and check on the host unless --no-cpu."""
+    from omega3d_amd import workloads
+    bem, smp = workloads.bem_matrix_run(log=log)
+    if not args.no_cpu:
+        # the sampled columns through the reference's own recursion on the host
+        # cores (oracle/_ref, S = float): the CPU baseline of this block, and the
+        # check of the GPU columns against it and the oracle's FP64 tree
+        import oracle
+        from omega3d_amd.metrics import nerr
+        b9, _ = oracle.bem_bases(smp["nodes"], smp["idx"])
+        full = (smp["nodes"], smp["idx"], smp["area"], b9)
+        js, nu = smp["sources"], smp["nunk"]
+        sub = (smp["nodes"], np.ascontiguousarray(smp["idx"][js]), smp["area"][js], np.ascontiguousarray(b9[:, js]))
+        keep = np.ones(smp["columns"].shape, bool)
+        for a_, j in enumerate(js):    # each panel's own block: constants, not a recursion
+            keep[j * nu:(j + 1) * nu, a_ * nu:(a_ + 1) * nu] = False
+        want = oracle.bem_matrix("wl", sub, full, nu, False, 3, mode="f64t")
+        bem["errors_vs_fp64_tree"] = {"max_rel": nerr(smp["columns"], want, keep), "columns": int(keep.shape[1])}
+        try:
+            t = time.perf_counter()
+            r32 = oracle.ref_bem_matrix("wl", sub, full, nu, False, 3, mode="f32")
+            secs = time.perf_counter() - t
+            P = smp["idx"].shape[0]
+            bem["cpu"] = {"kind": "reference", "cores": oracle.lib().orc_max_threads(),
+                          "what": "the reference's rkernel_2vs_2p through panels_on_panels_coeff's loop "
+                                  "(oracle/_ref, S = float, OpenMP over source panels)",
+                          "sample": f"{len(js)} source panels x all {P} target panels",
+                          "sample_s": round(secs, 3), "full_block_s_estimate": round(secs * P / len(js), 1),
+                          "reference_float_vs_fp64_tree": nerr(r32, want, keep),
+                          "gpu_vs_reference_float": nerr(smp["columns"], r32.astype(np.float64), keep)}
+        except FileNotFoundError:
+            bem["cpu"] = None
+        log(f"bench: bem matrix check {bem.get('errors_vs_fp64_tree')} cpu {bem.get('cpu')}")
+    return bem
 
 
 def main():
@@ -86,6 +126,10 @@ def main():
             out[name] = {"ms": round(ms.value, 4), "wave_instructions": wi.value,
                          "cycles_per_wave_instruction_at_max_clock": round(simd_cycles / wi.value, 3)}
         print(json.dumps({"valu_probe": out, "clock_mhz": dev["clock_khz"] // 1000, "cus": dev["cus"]}))
+        return
+
+    if args.bem_only:
+        print(json.dumps({"bem_matrix": bem_leg(args, log)}))
         return
 
     t0 = time.perf_counter()
@@ -265,6 +309,7 @@ def main():
     batched = None if args.no_batch else workloads.batched(log=log)
     served = None if args.no_serve else workloads.served(duration=args.serve_seconds, log=log)
     panels = None if args.no_panels else workloads.panels_run(log=log)
+    bem = None if args.no_bem else bem_leg(args, log)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "ff_far_exp_n1m_traffic.json" if factored else "p2p_exp_n1m_traffic.json")
@@ -327,6 +372,7 @@ def main():
         "batched": batched,
         "served": served,
         "panels": panels,
+        "bem_matrix": bem,
         "device": dev["name"],
     }
     print(json.dumps(out), flush=True)

@@ -219,6 +219,38 @@ int o3d_bem_rhs_dev(const o3d_points* vort, int32_t nvort, const o3d_panels* bdr
 int o3d_bem_rhs_from_vels_dev(const o3d_panels* bdry, const o3d_bem_rhs_opts* b, float* const pu[3], float* rhs,
                               void* stream);
 
+/* BEM influence matrix block of boundary surface src on boundary surface targ
+ * (panels_on_panels_coeff, Coefficients.h:169-444, over rkernel_2vs_2p,
+ * Kernels.h:1213-1311; the block Surfaces<S>'s CoefficientVisitor returns to
+ * BEMHelper.h:219-226).  Surfaces: nodes x[3], panel indices idx and areas (as
+ * Surfaces::compute_bases leaves them); ts/ss are not read.  nunk: unknowns per
+ * panel, 2 (vortex sheet along x1 and x2) or 3 (plus a source sheet), the same
+ * for both (Surfaces.h:261).  same: the two are one surface (the reference's
+ * &src == &targ), whose diagonal 2x2 or 3x3 blocks get the reference's constants
+ * (Coefficients.h:399-421); targ must then be src.  maxlev: recursion depth, -1 =
+ * the reference's RECURSIVE_LEVELS (3), at most 4.  coeffs: (nunk * targ->np)
+ * rows x (nunk * src->np) columns, column-major (column j*nunk + k: source panel
+ * j, unknown k; row i*nunk + m: target panel i, component on its x1, x2, normal),
+ * scaled by 1/(4 pi).  Every tree decision is the reference's float arithmetic;
+ * a recursed pair sums its leaves in FP64.  The device form synchronises the
+ * stream once (the count of recursed pairs sizes a list). */
+int o3d_bem_matrix(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, int32_t same, int32_t maxlev,
+                   float* coeffs);
+int o3d_bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, int32_t same, int32_t maxlev,
+                       float* coeffs, void* stream);
+/* Diagnostics (no counterpart in the reference): the device form with its stage
+ * times (root decisions and pair count; one-leaf pairs written and recursed pairs
+ * listed; recursed pairs) and the number of recursed pairs. */
+typedef struct o3d_bem_matrix_profile {
+  double count_ms;
+  double far_ms;
+  double near_ms;
+  double total_ms;
+  double near_pairs;
+} o3d_bem_matrix_profile;
+int o3d_bem_matrix_profile_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, int32_t same,
+                               int32_t maxlev, float* coeffs, void* stream, o3d_bem_matrix_profile* prof);
+
 /* Diagnostics of the factored method (no counterpart in the reference): runs one
  * o3d_points_affect_points_dev evaluation with method FACTORED on device arrays,
  * synchronises, and reports the time of its stages (preparation: box, Morton keys,

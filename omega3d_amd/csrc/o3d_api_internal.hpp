@@ -40,6 +40,8 @@ int stream_workspace(hipStream_t st, size_t bytes, void** out);
 // a collection large enough for the factored method to pay when the caller leaves the choice open
 constexpr int64_t FF_AUTO_MIN_N = 65536;
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);
+int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, int32_t same, int32_t maxlev,
+                   float* coeffs, hipStream_t st, o3d_bem_matrix_profile* prof);
 
 }  // namespace o3d
 

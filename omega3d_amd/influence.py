@@ -221,6 +221,32 @@ def bem_rhs(vorts, nodes, idx, area, fs, body=None, nunk=2, precision="comp", ma
     return dpu.to_host()[:, :P], drhs.to_host()[:nunk * P]
 
 
+def _surface_struct(nodes, idx, area):
+    p = _lib.Panels()
+    p.np, p.nn = area.shape[0], nodes.shape[1]
+    for d in range(3):
+        p.x[d] = nodes[d].ctypes.data
+    p.idx, p.area = idx.ctypes.data, area.ctypes.data
+    return p
+
+
+def bem_matrix(src, targ=None, nunk=2, maxlev=-1):
+    """BEM influence matrix block (panels_on_panels_coeff, Coefficients.h:169-444)
+    on the GPU.  src, targ: (nodes (3, N), idx (P, 3), area (P,)) surfaces; targ
+    None means the surface on itself (the reference's &src == &targ).  Returns a
+    (nunk * targ panels, nunk * src panels) float32 array [row, column]."""
+    same = targ is None
+    sn, si, sa = _c32(src[0]), np.ascontiguousarray(src[1], dtype=np.uint32), _c32(src[2])
+    tn, ti, ta = (sn, si, sa) if same else (_c32(targ[0]), np.ascontiguousarray(targ[1], dtype=np.uint32),
+                                            _c32(targ[2]))
+    ps, pt = _surface_struct(sn, si, sa), _surface_struct(tn, ti, ta)
+    ns, nt = sa.shape[0], ta.shape[0]
+    out = np.zeros(nunk * nunk * ns * nt, np.float32)
+    check(lib.o3d_bem_matrix(C.byref(ps), C.byref(pt), int(nunk), int(same), int(maxlev),
+                             out.ctypes.data_as(C.c_void_p)))
+    return out.reshape(nunk * ns, nunk * nt).T
+
+
 def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, precision="comp", maxlev=-1,
         vel=None, grad=None, stats=False, rule="recursive", tile=0):
     """Host-array panel->point sum on the GPU: rule "recursive" is the reference's

@@ -41,3 +41,14 @@ def self_term_gradient(core, sstr, srad):
     g[1], g[2], g[3] = a[2] * k, -a[1] * k, -a[2] * k
     g[5], g[6], g[7] = a[0] * k, a[1] * k, -a[0] * k
     return g
+
+
+def nerr(a, b, mask=None):
+    """Entrywise normwise error of a matrix block: max |a - b| / max |b| over the
+    entries where mask (default: all) is true."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if mask is None:
+        mask = np.ones(b.shape, bool)
+    den = np.abs(np.where(mask, b, 0.0)).max()
+    return float(np.abs(np.where(mask, a - b, 0.0)).max() / den) if den > 0 else 0.0

@@ -11,6 +11,7 @@
                    and aggregate interactions/s.
   panels()         20,480-panel icosphere -> 524,288 shell targets, recursive
                    centroid rule, compensated FP32.
+  bem_matrix_run() the BEM influence matrix of the same icosphere on itself.
 
     python -m omega3d_amd.workloads batched|served|panels
 """
@@ -209,6 +210,63 @@ def single_ring_run(n=8192, reps=20, log=print):
            "errors_vs_fp64": dict(field_errors(vel, grad, v64.to_host(), g64.to_host()), targets=n)}
     log("single_ring: " + json.dumps(out))
     return out
+
+
+def bem_matrix_run(levels=5, nunk=3, reps=3, sample_sources=32, log=print):
+    """The BEM influence matrix of the 20,480-panel icosphere on itself
+    (panels_on_panels_coeff, Coefficients.h:169-444; 3 unknowns per panel, maxlev 3):
+    a 61,440 x 61,440 float block written to HBM.  Stage times (root decisions and
+    count, one-leaf pairs written, recursed pairs) and the write rate of the whole
+    call and of the one-leaf kernel.  Returns (results, sample): sample holds the
+    surface and the columns of sample_sources spread source panels, for a check on
+    the host."""
+    L = _lib.lib
+    pnl = scenes.icosphere_panels(levels)
+    nodes, area = pnl.nodes, pnl.area
+    idx = pnl.idx.reshape(-1, 3)
+    P = idx.shape[0]
+    R = nunk * P
+    D = _lib.DeviceArray.from_host
+    dn, di, da = D(nodes), D(pnl.idx.view(np.float32)), D(area)
+    pan = _lib.Panels()
+    pan.np, pan.nn = P, nodes.shape[1]
+    for d in range(3):
+        pan.x[d] = dn.row(d)
+    pan.idx, pan.area = di.ptr, da.ptr
+    out = _lib.DeviceArray((R * R,))
+    profs = []
+    for _ in range(reps + 1):
+        prof = _lib.BemMatrixProfile()
+        _lib.check(L.o3d_bem_matrix_profile_dev(C.byref(pan), C.byref(pan), nunk, 1, -1, C.c_void_p(out.ptr), None,
+                                                C.byref(prof)))
+        _lib.synchronize()
+        profs.append(prof)
+    e0, e1 = _lib.Event(), _lib.Event()
+    e0.record()
+    for _ in range(reps):
+        _lib.check(L.o3d_bem_matrix_dev(C.byref(pan), C.byref(pan), nunk, 1, -1, C.c_void_p(out.ptr), None))
+    e1.record()
+    _lib.synchronize()
+    call_ms = e0.elapsed_ms(e1) / reps
+    best = min(profs[1:], key=lambda q: q.total_ms)
+    gb = 4.0 * R * R / 1e9
+    js = np.linspace(0, P - 1, sample_sources).astype(np.int64)
+    cols = np.empty((R, nunk * len(js)), np.float32)
+    col = np.empty(R, np.float32)
+    for a, j in enumerate(js):
+        for k in range(nunk):
+            _lib.check(L.o3d_memcpy_dtoh(col.ctypes.data, out.ptr + 4 * (int(j) * nunk + k) * R, 4 * R))
+            cols[:, a * nunk + k] = col
+    out.free()
+    res = {"workload": f"BEM influence matrix of the {P}-panel icosphere on itself, {nunk} unknowns per panel, "
+                       f"maxlev 3: {R} x {R} floats ({gb:.2f} GB)",
+           "call_ms": round(call_ms, 3), "count_ms": round(best.count_ms, 3), "far_ms": round(best.far_ms, 3),
+           "near_ms": round(best.near_ms, 3), "recursed_pairs": int(best.near_pairs), "panel_pairs": P * P,
+           "matrix_gb": round(gb, 3), "call_write_gbs": round(gb / call_ms * 1e3, 1),
+           "far_write_gbs": round(gb / best.far_ms * 1e3, 1)}
+    log("bem_matrix: " + json.dumps(res))
+    sample = {"nodes": nodes, "idx": idx, "area": area, "sources": js, "columns": cols, "nunk": nunk}
+    return res, sample
 
 
 def panels_run(n_targets=524288, reps=2, check_targets=4096, log=print):

@@ -44,7 +44,7 @@ def lib():
     if _lib is None:
         _lib = _load(os.path.join(HERE, "liboracle.so"))
         for name in ("orc_p2p_f", "orc_p2p_d", "orc_pan_f", "orc_pan_d", "orc_p2p_vcstyle", "orc_tri_integrals",
-                     "orc_pan_g7_near", "orc_pan_g7_d", "orc_bem_rhs"):
+                     "orc_pan_g7_near", "orc_pan_g7_d", "orc_bem_rhs", "orc_bem_bases", "orc_bem_matrix"):
             getattr(_lib, name).restype = C.c_int
         _lib.orc_flops_pair.restype = C.c_int
         _lib.orc_flops_pair.argtypes = [C.c_int] * 4
@@ -66,7 +66,8 @@ def ref_lib(core):
         p = ref_path(core)
         _refs[core] = C.CDLL(p) if os.path.exists(p) else None
         if _refs[core] is not None:
-            for name in ("ref_p2p_f", "ref_p2p_d", "ref_pan_f", "ref_pan_d", "ref_flops"):
+            for name in ("ref_p2p_f", "ref_p2p_d", "ref_pan_f", "ref_pan_d", "ref_flops", "ref_bem_matrix_f",
+                         "ref_bem_matrix_d"):
                 getattr(_refs[core], name).restype = C.c_int
     return _refs[core]
 
@@ -249,6 +250,53 @@ def bem_rhs(nodes, idx, pu, fs, body=False, vel=(0, 0, 0), rotvel=(0, 0, 0), tc=
     return pu, rhs
 
 
+def bem_bases(nodes, idx):
+    """Surfaces::compute_bases (Surfaces.h:766-816): (bases (9, P) -- x1, x2, normal
+    rows --, area (P,)), float32"""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32)
+    nodes = _as(nodes, np.float32)
+    P = idx.shape[0]
+    b = np.zeros((9, P), np.float32)
+    area = np.zeros(P, np.float32)
+    rc = lib().orc_bem_bases(_i64(P), _ptr(nodes[0]), _ptr(nodes[1]), _ptr(nodes[2]), _ptr(idx), _ptrs9(b),
+                             _ptr(area))
+    if rc != 0:
+        raise RuntimeError(f"oracle bem_bases failed rc={rc}")
+    return b, area
+
+
+def surface(nodes, idx):
+    """A boundary surface for bem_matrix: (nodes, idx, area, bases) with the area
+    and bases of compute_bases."""
+    b, area = bem_bases(nodes, idx)
+    return (_as(nodes, np.float32), np.ascontiguousarray(idx, dtype=np.uint32), area, b)
+
+
+def _bem_args(surf, dt):
+    nodes, idx, area, b = surf
+    nodes, area, b = _as(nodes, dt), _as(area, dt), _as(b, dt)
+    return [nodes, idx, area, b], [_i64(idx.shape[0]), _ptr(nodes[0]), _ptr(nodes[1]), _ptr(nodes[2]), _ptr(idx),
+                                   _ptr(area), _ptrs9(b)]
+
+
+def bem_matrix(core, src, targ, nunk=2, same=False, maxlev=RECURSIVE_LEVELS, mode="f32", nthreads=0):
+    """panels_on_panels_coeff (Coefficients.h:169-444) over rkernel_2vs_2p: the
+    BEM influence matrix of surface src on surface targ (each from surface()), as a
+    (nunk * targ panels, nunk * src panels) array [row, column].  mode 'f32': the
+    reference's float arithmetic (float32); 'f64t': the same tree (decided in
+    float) with FP64 leaves and sums (float64)."""
+    keep_s, a_s = _bem_args(src, np.float32)
+    keep_t, a_t = _bem_args(targ, np.float32)
+    ns, nt = src[1].shape[0], targ[1].shape[0]
+    m = {"f32": 0, "f64t": 1}[mode]
+    out = np.zeros(nunk * nunk * ns * nt, np.float32 if m == 0 else np.float64)
+    rc = lib().orc_bem_matrix(_core_id(core), m, int(maxlev), int(nunk), int(bool(same)), *a_s, *a_t,
+                              _ptr(out) if m == 0 else None, _ptr(out) if m == 1 else None, int(nthreads))
+    if rc != 0:
+        raise RuntimeError(f"oracle bem_matrix failed rc={rc}")
+    return out.reshape(nunk * ns, nunk * nt).T
+
+
 # ---- the reference's own kernels (oracle/_ref), same conventions -------------
 
 def ref_p2p(core, spos, sstr, srad, tpos, trad=None, grads=True, mode="f32d"):
@@ -319,3 +367,21 @@ def ref_ptpan(core, spos, sstr, nodes, idx, area, mode="f32d", maxlev=RECURSIVE_
 
 def ref_flops(core, which):
     return ref_lib(core).ref_flops(int(which))
+
+
+def ref_bem_matrix(core, src, targ, nunk=2, same=False, maxlev=RECURSIVE_LEVELS, mode="f32"):
+    """The same matrix through the reference's own rkernel_2vs_2p (S = float for
+    'f32', S = double for 'f64', with its own double-precision tree)."""
+    L = ref_lib(core)
+    if L is None:
+        raise FileNotFoundError(ref_path(core))
+    dt = np.float32 if mode == "f32" else np.float64
+    keep_s, a_s = _bem_args(src, dt)
+    keep_t, a_t = _bem_args(targ, dt)
+    ns, nt = src[1].shape[0], targ[1].shape[0]
+    out = np.zeros(nunk * nunk * ns * nt, dt)
+    f = L.ref_bem_matrix_f if mode == "f32" else L.ref_bem_matrix_d
+    rc = f(int(maxlev), int(nunk), int(bool(same)), *a_s, *a_t, _ptr(out))
+    if rc != 0:
+        raise RuntimeError(f"reference bem_matrix failed rc={rc}")
+    return out.reshape(nunk * ns, nunk * nt).T

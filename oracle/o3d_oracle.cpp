@@ -401,6 +401,136 @@ int pan_dispatch(int core, int grads, const Pan<T>& p, const Tgt<T>& t, int maxl
   return 1;
 }
 
+// ---------------------------------------------------------------------------
+// BEM influence matrix: panels_on_panels_coeff (Coefficients.h:169-444, scalar
+// branch :332-397, self block :399-421, scaling :428-433) over the panel-panel
+// recursion rkernel_2vs_2p (Kernels.h:1213-1311).  Per (source panel, target
+// panel, source unknown) the recursion splits both triangles into 4 (16 child
+// pairs, strength / 16, areas / 4) until the centroids are more than
+// 4 (sqrt(sa) + sqrt(ta)) apart or maxlev is reached; a leaf is kernel_0vs_0p, a
+// singular vortex + source point at the source centroid acting on the target
+// centroid.  The tree is decided in float (the reference's S); the leaves run in
+// L (float: the reference's arithmetic; double: the same tree in FP64).
+// ---------------------------------------------------------------------------
+template <int CORE, class L, class A>
+void rpanpan(const float* sf, const float* tf, const L* sl, const L* tl, L gx, L gy, L gz, L gs, float saf,
+             float taf, L sal, L tal, int lev, int maxlev, A* acc) {
+  if (lev == 0) { gx *= sal; gy *= sal; gz *= sal; gs *= sal; }
+  const float scx = (sf[0] + sf[3] + sf[6]) / 3.0f;
+  const float scy = (sf[1] + sf[4] + sf[7]) / 3.0f;
+  const float scz = (sf[2] + sf[5] + sf[8]) / 3.0f;
+  const float tcx = (tf[0] + tf[3] + tf[6]) / 3.0f;
+  const float tcy = (tf[1] + tf[4] + tf[7]) / 3.0f;
+  const float tcz = (tf[2] + tf[5] + tf[8]) / 3.0f;
+  const float trisize = std::sqrt(saf) + std::sqrt(taf);
+  const float ex = tcx - scx, ey = tcy - scy, ez = tcz - scz;
+  const float dist = std::sqrt(ex*ex + ey*ey + ez*ez);
+  if (dist > trisize*4.0f || lev == maxlev) {
+    const L sx = (sl[0] + sl[3] + sl[6]) / L(3), sy = (sl[1] + sl[4] + sl[7]) / L(3), sz = (sl[2] + sl[5] + sl[8]) / L(3);
+    const L tx = (tl[0] + tl[3] + tl[6]) / L(3), ty = (tl[1] + tl[4] + tl[7]) / L(3), tz = (tl[2] + tl[5] + tl[8]) / L(3);
+    pair<CORE, false, false, true, L, A>(sx, sy, sz, L(0), gx, gy, gz, gs, tx, ty, tz, L(0), acc);
+    return;
+  }
+  gx *= L(0.0625); gy *= L(0.0625); gz *= L(0.0625); gs *= L(0.0625);
+  const float scaf = 0.25f * saf, tcaf = 0.25f * taf;
+  const L scal = L(0.25) * sal, tcal = L(0.25) * tal;
+  // six nodes per triangle: corners 0, 2, 5 and edge midpoints 1 (0-1), 3 (0-2), 4 (1-2)
+  float snf[18], tnf[18];
+  L snl[18], tnl[18];
+  for (int d = 0; d < 3; ++d) {
+    snf[d] = sf[d]; snf[3+d] = 0.5f*(sf[d]+sf[3+d]); snf[6+d] = sf[3+d];
+    snf[9+d] = 0.5f*(sf[d]+sf[6+d]); snf[12+d] = 0.5f*(sf[3+d]+sf[6+d]); snf[15+d] = sf[6+d];
+    tnf[d] = tf[d]; tnf[3+d] = 0.5f*(tf[d]+tf[3+d]); tnf[6+d] = tf[3+d];
+    tnf[9+d] = 0.5f*(tf[d]+tf[6+d]); tnf[12+d] = 0.5f*(tf[3+d]+tf[6+d]); tnf[15+d] = tf[6+d];
+    snl[d] = sl[d]; snl[3+d] = L(0.5)*(sl[d]+sl[3+d]); snl[6+d] = sl[3+d];
+    snl[9+d] = L(0.5)*(sl[d]+sl[6+d]); snl[12+d] = L(0.5)*(sl[3+d]+sl[6+d]); snl[15+d] = sl[6+d];
+    tnl[d] = tl[d]; tnl[3+d] = L(0.5)*(tl[d]+tl[3+d]); tnl[6+d] = tl[3+d];
+    tnl[9+d] = L(0.5)*(tl[d]+tl[6+d]); tnl[12+d] = L(0.5)*(tl[3+d]+tl[6+d]); tnl[15+d] = tl[6+d];
+  }
+  static const int child[4][3] = {{0,1,3}, {1,2,4}, {1,4,3}, {3,4,5}};
+  for (int i = 0; i < 4; ++i) {
+    float csf[9]; L csl[9];
+    for (int v = 0; v < 3; ++v)
+      for (int d = 0; d < 3; ++d) { csf[3*v+d] = snf[3*child[i][v]+d]; csl[3*v+d] = snl[3*child[i][v]+d]; }
+    for (int j = 0; j < 4; ++j) {
+      float ctf[9]; L ctl[9];
+      for (int v = 0; v < 3; ++v)
+        for (int d = 0; d < 3; ++d) { ctf[3*v+d] = tnf[3*child[j][v]+d]; ctl[3*v+d] = tnl[3*child[j][v]+d]; }
+      rpanpan<CORE, L, A>(csf, ctf, csl, ctl, gx, gy, gz, gs, scaf, tcaf, scal, tcal, lev + 1, maxlev, acc);
+    }
+  }
+}
+
+// one boundary surface as the matrix builder sees it: nodes, panels, areas and
+// the per-panel basis (x1, x2, normal) that Surfaces::compute_bases keeps
+struct BemSurf {
+  int64_t np; const float *x, *y, *z; const uint32_t* idx; const float* area;
+  const float *b1[3], *b2[3], *nm[3];
+};
+
+// column-major block: column j*nunk + k (source panel j, unknown k: vortex along
+// x1, along x2, source), row i*nunk + m (target panel i, component on x1, x2,
+// normal).  O = float (L = A = float: the reference) or double.
+template <int CORE, class L, class A, class O>
+void bem_matrix_loop(const BemSurf& s, const BemSurf& t, int nunk, int same, int maxlev, O* coeffs, int nthreads) {
+  const int64_t ns = s.np, nt = t.np, R = nunk * nt;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+  for (int64_t j = 0; j < ns; ++j) {
+    float sf[9]; L sl[9];
+    for (int v = 0; v < 3; ++v) {
+      const uint32_t n = s.idx[3*j+v];
+      sf[3*v] = s.x[n]; sf[3*v+1] = s.y[n]; sf[3*v+2] = s.z[n];
+      for (int d = 0; d < 3; ++d) sl[3*v+d] = L(sf[3*v+d]);
+    }
+    const float saf = s.area[j];
+    for (int64_t i = 0; i < nt; ++i) {
+      float tf[9]; L tl[9];
+      for (int v = 0; v < 3; ++v) {
+        const uint32_t n = t.idx[3*i+v];
+        tf[3*v] = t.x[n]; tf[3*v+1] = t.y[n]; tf[3*v+2] = t.z[n];
+        for (int d = 0; d < 3; ++d) tl[3*v+d] = L(tf[3*v+d]);
+      }
+      const float taf = t.area[i];
+      for (int k = 0; k < nunk; ++k) {
+        const L gx = k == 0 ? L(s.b1[0][j]) : k == 1 ? L(s.b2[0][j]) : L(0);
+        const L gy = k == 0 ? L(s.b1[1][j]) : k == 1 ? L(s.b2[1][j]) : L(0);
+        const L gz = k == 0 ? L(s.b1[2][j]) : k == 1 ? L(s.b2[2][j]) : L(0);
+        const L gs = k == 2 ? L(1) : L(0);
+        A r[3] = {0, 0, 0};
+        rpanpan<CORE, L, A>(sf, tf, sl, tl, gx, gy, gz, gs, saf, taf, L(saf), L(taf), 0, maxlev, r);
+        O* c = coeffs + (j*nunk + k) * R + i*nunk;
+        c[0] = r[0]*A(t.b1[0][i]) + r[1]*A(t.b1[1][i]) + r[2]*A(t.b1[2][i]);
+        c[1] = r[0]*A(t.b2[0][i]) + r[1]*A(t.b2[1][i]) + r[2]*A(t.b2[2][i]);
+        if (nunk == 3) c[2] = r[0]*A(t.nm[0][i]) + r[1]*A(t.nm[1][i]) + r[2]*A(t.nm[2][i]);
+      }
+    }
+    if (same) {
+      // Coefficients.h:399-421: the panel on itself, set before the scaling
+      const O tp = O(2.0 * M_PI);
+      O* c = coeffs + (j*nunk) * R + j*nunk;
+      c[0] = O(0); c[1] = tp; if (nunk == 3) c[2] = O(0);
+      c += R;
+      c[0] = -tp; c[1] = O(0); if (nunk == 3) c[2] = O(0);
+      if (nunk == 3) { c += R; c[0] = O(0); c[1] = O(0); c[2] = tp; }
+    }
+  }
+  const O fac = O(1.0 / (4.0 * M_PI));
+  const int64_t total = R * (int64_t)(nunk * ns);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+  for (int64_t e = 0; e < total; ++e) coeffs[e] = coeffs[e] * fac;
+}
+
+template <class L, class A, class O>
+int bem_matrix_dispatch(int core, const BemSurf& s, const BemSurf& t, int nunk, int same, int maxlev, O* c, int nth) {
+  switch (core) {
+    case C_RM:  bem_matrix_loop<C_RM, L, A>(s, t, nunk, same, maxlev, c, nth); return 0;
+    case C_WL:  bem_matrix_loop<C_WL, L, A>(s, t, nunk, same, maxlev, c, nth); return 0;
+    case C_EXP: bem_matrix_loop<C_EXP, L, A>(s, t, nunk, same, maxlev, c, nth); return 0;
+    case C_V2:  bem_matrix_loop<C_V2, L, A>(s, t, nunk, same, maxlev, c, nth); return 0;
+  }
+  return 1;
+}
+
 int clamp_threads(int n) {
 #ifdef _OPENMP
   if (n <= 0) n = omp_get_max_threads();
@@ -633,6 +763,28 @@ int orc_ptpan_d(int core, int maxlev, int64_t ns, const double* sx, const double
   Src<double> s{ns, sx, sy, sz, nullptr, ssx, ssy, ssz};
   Pan<double> p{np, x, y, z, idx, nullptr, nullptr, nullptr, area, nullptr};
   return ptpan_dispatch<double, double>(core, s, p, tu, tv, tw, maxlev, clamp_threads(nthreads));
+}
+
+// BEM influence matrix (Coefficients.h:169-444).  Surfaces: nodes, panels,
+// areas and the bases b1, b2, nm (3 rows of np floats each, as compute_bases
+// leaves them).  nunk 2 or 3 for both; same: the two are one surface (the
+// diagonal blocks of Coefficients.h:399-421).  mode 0: the reference's float
+// arithmetic, float output c32; mode 1: the same tree with FP64 leaves and
+// sums, double output c64.  Sizes (nunk*ns) columns x (nunk*nt) rows, column-major.
+int orc_bem_matrix(int core, int mode, int maxlev, int nunk, int same, int64_t ns, const float* sx, const float* sy,
+                   const float* sz, const uint32_t* sidx, const float* sarea, const float* const* sbases,
+                   int64_t nt, const float* tx, const float* ty, const float* tz, const uint32_t* tidx,
+                   const float* tarea, const float* const* tbases, float* c32, double* c64, int nthreads) {
+  if (nunk != 2 && nunk != 3) return 2;
+  if (same && ns != nt) return 4;
+  BemSurf s{ns, sx, sy, sz, sidx, sarea, {sbases[0], sbases[1], sbases[2]}, {sbases[3], sbases[4], sbases[5]},
+            {sbases[6], sbases[7], sbases[8]}};
+  BemSurf t{nt, tx, ty, tz, tidx, tarea, {tbases[0], tbases[1], tbases[2]}, {tbases[3], tbases[4], tbases[5]},
+            {tbases[6], tbases[7], tbases[8]}};
+  const int nth = clamp_threads(nthreads);
+  if (mode == 0) return bem_matrix_dispatch<float, float>(core, s, t, nunk, same, maxlev, c32, nth);
+  if (mode == 1) return bem_matrix_dispatch<double, double>(core, s, t, nunk, same, maxlev, c64, nth);
+  return 3;
 }
 
 // 1 if the host has AVX2+FMA (the Vc-style loop needs them)

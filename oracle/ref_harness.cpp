@@ -12,9 +12,11 @@
 // panels_affect_points (Influence.h:729-775, 968-1006): zero accumulators of type A
 // per target, visit every source in index order, add into the caller's outputs;
 // and of points_affect_panels (Influence.h:1190-1210), which runs the panel kernel
-// backwards and subtracts from the panels' velocities.
+// backwards and subtracts from the panels' velocities; and of panels_on_panels_coeff
+// (Coefficients.h:169-444), the BEM influence matrix over rkernel_2vs_2p.
 
 #include "Kernels.h"
+#include <cmath>
 #include <cstdint>
 
 namespace {
@@ -94,9 +96,73 @@ void ref_ptpan_loop(int maxlev, int64_t ns, const S* sx, const S* sy, const S* s
   }
 }
 
+// panels_on_panels_coeff's scalar branch (Coefficients.h:332-397), its self
+// block (:399-421) and its scaling (:428-433), driving the reference's own
+// rkernel_2vs_2p; bases as Surfaces::compute_bases leaves them (b[0..2] x1,
+// b[3..5] x2, b[6..8] normal), column-major output (nunk*ns) x (nunk*nt)
+template <class S>
+void ref_bem_loop(int maxlev, int nunk, int same, int64_t ns, const S* sx, const S* sy, const S* sz,
+                  const uint32_t* si, const S* sa, const S* const* sb, int64_t nt, const S* tx, const S* ty,
+                  const S* tz, const uint32_t* ti, const S* ta, const S* const* tb, S* coeffs) {
+  const int64_t R = nunk * nt;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t j = 0; j < ns; ++j) {
+    const size_t a = si[3*j], b = si[3*j+1], c = si[3*j+2];
+    for (int64_t i = 0; i < nt; ++i) {
+      const size_t p = ti[3*i], q = ti[3*i+1], r = ti[3*i+2];
+      for (int k = 0; k < nunk; ++k) {
+        const S gx = k == 0 ? sb[0][j] : k == 1 ? sb[3][j] : S(0.0);
+        const S gy = k == 0 ? sb[1][j] : k == 1 ? sb[4][j] : S(0.0);
+        const S gz = k == 0 ? sb[2][j] : k == 1 ? sb[5][j] : S(0.0);
+        const S gs = k == 2 ? S(1.0) : S(0.0);
+        S u = 0.0, v = 0.0, w = 0.0;
+        (void) rkernel_2vs_2p<S, S>(sx[a], sy[a], sz[a], sx[b], sy[b], sz[b], sx[c], sy[c], sz[c],
+                                    gx, gy, gz, gs,
+                                    tx[p], ty[p], tz[p], tx[q], ty[q], tz[q], tx[r], ty[r], tz[r],
+                                    sa[j], ta[i], 0, maxlev, &u, &v, &w);
+        S* o = coeffs + (j*nunk + k) * R + i*nunk;
+        o[0] = u*tb[0][i] + v*tb[1][i] + w*tb[2][i];
+        o[1] = u*tb[3][i] + v*tb[4][i] + w*tb[5][i];
+        if (nunk == 3) o[2] = u*tb[6][i] + v*tb[7][i] + w*tb[8][i];
+      }
+    }
+    if (same) {
+      S* o = coeffs + (j*nunk) * R + j*nunk;
+      o[0] = 0.0; o[1] = 2.0*M_PI; if (nunk == 3) o[2] = 0.0;
+      o += R;
+      o[0] = -2.0*M_PI; o[1] = 0.0; if (nunk == 3) o[2] = 0.0;
+      if (nunk == 3) { o += R; o[0] = 0.0; o[1] = 0.0; o[2] = 2.0*M_PI; }
+    }
+  }
+  const S fac = 1.0 / (4.0 * M_PI);
+  const int64_t total = R * (int64_t)(nunk * ns);
+  for (int64_t e = 0; e < total; ++e) coeffs[e] = coeffs[e] * fac;
+}
+
 }  // namespace
 
 extern "C" {
+
+// BEM influence matrix block through the reference's rkernel_2vs_2p, S = float
+int ref_bem_matrix_f(int maxlev, int nunk, int same, int64_t ns, const float* sx, const float* sy, const float* sz,
+                     const uint32_t* si, const float* sa, const float* const* sb, int64_t nt, const float* tx,
+                     const float* ty, const float* tz, const uint32_t* ti, const float* ta, const float* const* tb,
+                     float* coeffs) {
+  if (nunk != 2 && nunk != 3) return 2;
+  ref_bem_loop<float>(maxlev, nunk, same, ns, sx, sy, sz, si, sa, sb, nt, tx, ty, tz, ti, ta, tb, coeffs);
+  return 0;
+}
+
+// the same with the reference's templates instantiated in double (its own
+// double-precision tree decisions)
+int ref_bem_matrix_d(int maxlev, int nunk, int same, int64_t ns, const double* sx, const double* sy,
+                     const double* sz, const uint32_t* si, const double* sa, const double* const* sb, int64_t nt,
+                     const double* tx, const double* ty, const double* tz, const uint32_t* ti, const double* ta,
+                     const double* const* tb, double* coeffs) {
+  if (nunk != 2 && nunk != 3) return 2;
+  ref_bem_loop<double>(maxlev, nunk, same, ns, sx, sy, sz, si, sa, sb, nt, tx, ty, tz, ti, ta, tb, coeffs);
+  return 0;
+}
 
 // points_affect_panels; accum: 0 = A float, 1 = A double (S = float)
 int ref_ptpan_f(int accum, int maxlev, int64_t ns, const float* sx, const float* sy, const float* sz,
