@@ -239,8 +239,8 @@ int o3d_bem_matrix(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
 int o3d_bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, int32_t same, int32_t maxlev,
                        float* coeffs, void* stream);
 /* Diagnostics (no counterpart in the reference): the device form with its stage
- * times (root decisions and pair count; one-leaf pairs written and recursed pairs
- * listed; recursed pairs) and the number of recursed pairs. */
+ * times (root decisions and pair count; one-leaf and self coefficients written
+ * and recursed pairs listed; recursed pairs) and the number of recursed pairs. */
 typedef struct o3d_bem_matrix_profile {
   double count_ms;
   double far_ms;

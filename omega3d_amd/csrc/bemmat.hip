@@ -25,10 +25,12 @@
 // coefficients of well-separated pairs (one leaf) -- for a fixed source the lanes
 // of a wave write consecutive rows, so the columns stream out coalesced, which is
 // what bounds this kernel (HBM writes) -- and lists the others.  A first pass of
-// the same kernel only counts them, to size the list.  k_bm_near: one wave per
-// listed pair: 16 lanes take the level-1 decisions, the children of the open
-// level-1 pairs are spread over the wave, each lane walks what lies below its
-// own, and a fixed
+// the same kernel only counts them, to size the list.  (Running the recursed
+// pairs on a second stream beside the writes was measured slower: the write
+// kernel's workgroups hold the CUs.)  k_bm_near: one wave per
+// listed pair: 16 lanes take the level-1 decisions and leave the open pairs'
+// triangles in LDS, the children of the open level-1 pairs are spread over the
+// wave, each lane walks what lies below its own, and a fixed
 // butterfly adds the lanes' G in FP64, so results do not depend on scheduling.
 // Every tree decision below the root is the reference's float arithmetic too, so
 // the GPU subdivides exactly where the reference does; leaves below the last
@@ -134,9 +136,12 @@ struct BmFarArgs {
 
 constexpr int BM_TILE = BLOCK;   // sources staged per LDS pass
 
-// COUNT: count the pairs that need the recursion (no writes); else write the
-// one-leaf and self coefficients and list the rest
-template <bool COUNT>
+// BM_COUNT: count the pairs that need the recursion; BM_WRITE: write the one-leaf
+// and self coefficients and list the others.  The same decisions in both, so the
+// list has exactly the counted length.
+enum { BM_COUNT = 0, BM_WRITE = 1 };
+
+template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_bm_far(BmFarArgs a) {
   __shared__ float4 sh[BM_TILE * 3];
   const int tid = threadIdx.x;
@@ -178,7 +183,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_far(BmFarArgs a) {
       const bool diag = a.same && j == i;
       const bool leaf = a.maxlev == 0 || well_sep(d2, t4);
       const bool nearp = valid && !diag && !leaf;
-      if (COUNT) {
+      if (MODE == BM_COUNT) {
         cnt += nearp ? 1ull : 0ull;
         continue;
       }
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_far(BmFarArgs a) {
       }
     }
   }
-  if (COUNT) {
+  if (MODE == BM_COUNT) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
     if ((tid & 63) == 0 && cnt) atomicAdd(a.count, cnt);
@@ -234,24 +239,21 @@ __global__ __launch_bounds__(BLOCK) void k_bm_far(BmFarArgs a) {
 }
 
 // child c of triangle v (9 floats) over the six nodes [v0, m01, v1, m02, m12, v2]:
-// {0,1,3}, {1,2,4}, {1,4,3}, {3,4,5} (Kernels.h:1284-1292), midpoints 0.5 (a + b)
+// {0,1,3}, {1,2,4}, {1,4,3}, {3,4,5} (Kernels.h:1284-1292), midpoints 0.5 (a + b).
+// Written as three selects per coordinate: an indexed node table with a runtime
+// child index compiles to hundreds of compare/select instructions.
 __device__ __forceinline__ void bm_child(const float* v, int c, float* o) {
 #pragma clang fp contract(off)
-  float n[18];
+  const bool c0 = c == 0, c1 = c == 1, c3 = c == 3;
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
-    n[d] = v[d];
-    n[3 + d] = 0.5f * (v[d] + v[3 + d]);
-    n[6 + d] = v[3 + d];
-    n[9 + d] = 0.5f * (v[d] + v[6 + d]);
-    n[12 + d] = 0.5f * (v[3 + d] + v[6 + d]);
-    n[15 + d] = v[6 + d];
+    const float m01 = 0.5f * (v[d] + v[3 + d]);
+    const float m02 = 0.5f * (v[d] + v[6 + d]);
+    const float m12 = 0.5f * (v[3 + d] + v[6 + d]);
+    o[d] = c0 ? v[d] : (c3 ? m02 : m01);            // {0, 1, 1, 3}
+    o[3 + d] = c0 ? m01 : (c1 ? v[3 + d] : m12);    // {1, 2, 4, 4}
+    o[6 + d] = c1 ? m12 : (c3 ? v[6 + d] : m02);    // {3, 4, 3, 5}
   }
-  const int a = c == 0 ? 0 : c == 3 ? 3 : 1;
-  const int b = c == 0 ? 1 : c == 1 ? 2 : 4;
-  const int e = c == 0 ? 3 : c == 1 ? 4 : c == 2 ? 3 : 5;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) { o[d] = n[3 * a + d]; o[3 + d] = n[3 * b + d]; o[6 + d] = n[3 * e + d]; }
 }
 
 // s / 3 correctly rounded: q = s RN(1/3), r = fma(-3, q, s) (exact), q + r RN(1/3)
@@ -366,12 +368,19 @@ __device__ __forceinline__ void bm_corners(const float4* r, float* v) {
 // child c/4, target child c%4) and evaluates it if separated.  Level 2: the 16
 // children of each open level-1 pair are spread over the wave (item k: the
 // (k/16)-th open pair, child k%16); a separated one is a leaf, an open one below
-// maxlev is walked by its lane (level 3 all leaves when maxlev is 3).  The tree
+// maxlev is walked by its lane (level 3 all leaves when maxlev is 3).  The
+// level-1 triangles of the open pairs pass through LDS.  The tree
 // decisions are the reference's float arithmetic; leaves below a decision level
 // take FP64 centroids of the float nodes.
+constexpr int BM_L1 = 20;   // floats per level-1 pair in LDS (18 used; rows 16-byte aligned)
+
 template <int MAXLEV>
 __global__ __launch_bounds__(BLOCK) void k_bm_near(BmNearArgs a) {
+  // each wave's 16 level-1 triangle pairs, shared with its items through LDS; a
+  // wave reads only its own rows, and LDS operations of one wave complete in order
+  __shared__ float4 l1s[(BLOCK / 64) * 16 * BM_L1 / 4];
   const int lane = threadIdx.x & 63;
+  float* l1 = (float*)l1s + (threadIdx.x >> 6) * 16 * BM_L1;
   const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
   const int64_t listed = (int64_t)(*a.count < a.cap ? *a.count : a.cap);
@@ -397,31 +406,41 @@ __global__ __launch_bounds__(BLOCK) void k_bm_near(BmNearArgs a) {
     if constexpr (maxlev == 1) {
       if (lane == 0) bm_leaves16(s0, t0, 0.0625, G);
     } else {
-      // level 1 on lanes 0..15; their triangles stay in registers for the items
+      // level 1 on lanes 0..15; the open pairs' triangles go to LDS for the items
       bool open1 = false;
-      float sv1[9], tv1[9];
-      bm_child(s0, (lane >> 2) & 3, sv1);
-      bm_child(t0, lane & 3, tv1);
       if (lane < 16) {
+        float sv1[9], tv1[9];
+        bm_child(s0, lane >> 2, sv1);
+        bm_child(t0, lane & 3, tv1);
         float ex, ey, ez, d2;
-        if (bm_sep(sv1, tv1, t41, ex, ey, ez, d2)) bm_leaf_d(ex, ey, ez, d2, 0.0625, G);
-        else open1 = true;
+        if (bm_sep(sv1, tv1, t41, ex, ey, ez, d2)) {
+          bm_leaf_d(ex, ey, ez, d2, 0.0625, G);
+        } else {
+          open1 = true;
+          float4* row = (float4*)(l1 + lane * BM_L1);
+          row[0] = make_float4(sv1[0], sv1[1], sv1[2], sv1[3]);
+          row[1] = make_float4(sv1[4], sv1[5], sv1[6], sv1[7]);
+          row[2] = make_float4(sv1[8], tv1[0], tv1[1], tv1[2]);
+          row[3] = make_float4(tv1[3], tv1[4], tv1[5], tv1[6]);
+          row[4] = make_float4(tv1[7], tv1[8], 0.f, 0.f);
+        }
       }
       const unsigned long long m1 = __builtin_amdgcn_ballot_w64(open1);
+      __builtin_amdgcn_wave_barrier();
       const int items = 16 * __builtin_popcountll(m1);
-      // every lane runs every pass, so the shuffles read lanes that are active
-      for (int base = 0; base < items; base += 64) {
-        const int k = base + lane;
+      for (int k = lane; k < items; k += 64) {
         unsigned long long m = m1;
-        for (int t = 0; t < ((k < items ? k : 0) >> 4); ++t) m &= m - 1;
+        for (int t = 0; t < (k >> 4); ++t) m &= m - 1;
         const int c1 = __builtin_ctzll(m), c2 = k & 15;
         float s1v[9], t1v[9];
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-          s1v[q] = __shfl(sv1[q], c1, 64);
-          t1v[q] = __shfl(tv1[q], c1, 64);
+        {
+          const float4* row = (const float4*)(l1 + c1 * BM_L1);
+          const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4];
+          s1v[0] = r0.x; s1v[1] = r0.y; s1v[2] = r0.z; s1v[3] = r0.w;
+          s1v[4] = r1.x; s1v[5] = r1.y; s1v[6] = r1.z; s1v[7] = r1.w; s1v[8] = r2.x;
+          t1v[0] = r2.y; t1v[1] = r2.z; t1v[2] = r2.w; t1v[3] = r3.x; t1v[4] = r3.y;
+          t1v[5] = r3.z; t1v[6] = r3.w; t1v[7] = r4.x; t1v[8] = r4.y;
         }
-        if (k >= items) continue;
         float s2v[9], t2v[9];
         bm_child(s1v, c2 >> 2, s2v);
         bm_child(t1v, c2 & 3, t2v);
@@ -452,6 +471,7 @@ __global__ __launch_bounds__(BLOCK) void k_bm_near(BmNearArgs a) {
         }
       }
     }
+    __builtin_amdgcn_wave_barrier();
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
 #pragma unroll
@@ -525,6 +545,7 @@ int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
   std::lock_guard<std::mutex> lk(stream_ws_mutex());
   void* ws = nullptr;
   if ((rc = stream_workspace(st, recb + 256, &ws)) != O3D_OK) return rc;
+  // ev: start, counted, one-leaf pass done, recursed pairs done
   hipEvent_t ev[4] = {};
   if (prof) {
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
@@ -557,7 +578,7 @@ int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
   fa.schunk = ((ntiles + ny - 1) / ny) * BM_TILE;
   ny = (ns + fa.schunk - 1) / fa.schunk;
   const dim3 grid((unsigned)nbx, (unsigned)ny);
-  hipLaunchKernelGGL(k_bm_far<true>, grid, dim3(BLOCK), 0, st, fa);
+  hipLaunchKernelGGL(k_bm_far<BM_COUNT>, grid, dim3(BLOCK), 0, st, fa);
   HIPCHK(hipGetLastError());
   unsigned long long nnear = 0;
   HIPCHK(hipMemcpyAsync(&nnear, cnt, sizeof(nnear), hipMemcpyDeviceToHost, st));
@@ -570,7 +591,7 @@ int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
   launch_geom(targ, trec, st);
   hipLaunchKernelGGL(k_bm_zero, dim3(1), dim3(64), 0, st, cnt);
   fa.srec = srec; fa.trec = trec; fa.count = cnt; fa.list = list; fa.cap = nnear;
-  hipLaunchKernelGGL(k_bm_far<false>, grid, dim3(BLOCK), 0, st, fa);
+  hipLaunchKernelGGL(k_bm_far<BM_WRITE>, grid, dim3(BLOCK), 0, st, fa);
   HIPCHK(hipGetLastError());
   if (prof) HIPCHK(hipEventRecord(ev[2], st));
   if (nnear > 0 && maxlev > 0) {
