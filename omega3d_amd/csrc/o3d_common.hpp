@@ -55,6 +55,13 @@ __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(
 
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
+// sum over the 64 lanes of a wave by a fixed butterfly (the same order on every run)
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
 // error-free transformation a + b = s + e (Knuth's TwoSum)
 __device__ __forceinline__ void two_sum(float& hi, float& lo, float b) {
   const float s = hi + b;

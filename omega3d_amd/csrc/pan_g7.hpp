@@ -378,11 +378,6 @@ __global__ __launch_bounds__(BLOCK) void k_pan_g7_geom(PanArgs a, float4* geom) 
   geom[p] = g7_geometry(v);
 }
 
-__device__ __forceinline__ double wave_sum(double x) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
-}
 
 // near pairs of the targets k_pan_g7 listed, in FP64, added to the outputs after it.
 // One wave per listed target: lane l takes the decision for panels l, l + 64, ...

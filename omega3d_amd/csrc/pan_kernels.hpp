@@ -8,10 +8,15 @@
 // target or at depth maxlev; a leaf is a singular (sigma = 0) vortex + source
 // point at the centroid carrying the node's share of the panel strength.
 //
-// GPU form: one lane per target; panels are staged per tile in LDS; each lane
-// walks the tree for its target in depth-first order with an explicit (level,
-// path) cursor -- no recursion, no stack -- recomputing a node's vertices from
-// the panel's corners along its path.  Every quantity that decides the tree
+// GPU form: k_pan takes one lane per target with the panels staged per tile in
+// LDS, decides every root pair and evaluates the pairs that are leaves at the
+// root, and lists the targets that have a pair needing the recursion (those near
+// the surface).  k_pan_near then gives each listed target one wave whose lanes
+// take the panels in turn and walk the near ones: a lane walks the tree in
+// depth-first order with an explicit (level, path) cursor -- no recursion, no
+// stack -- recomputing a node's vertices from the panel's corners along its path.
+// Walking inside k_pan instead held a whole wave of targets for the deepest walk
+// of any of its lanes.  Every quantity that decides the tree
 // shape (midpoints, centroid, sqrt(area), distance) is computed in correctly
 // rounded IEEE arithmetic with FMA contraction off, in the reference's order,
 // so the GPU subdivides exactly where the reference does.  At the root the
@@ -139,9 +144,8 @@ __device__ __forceinline__ void grad_of(const R* a, R* g) {
 //   f4 = {cx, cy, cz, 0} (root centroid)   f5 = {tsx, tsy, tsz, ss} (raw inputs)
 // The reference passes the sheet strength ts/area and multiplies it back by the
 // area at level 0 (Influence.h:737, Kernels.h:1136-1146); that round trip is kept.
-__device__ __forceinline__ void stage_panels(const PanArgs& a, int64_t p0, int cnt, float4* sh) {
-  for (int j = threadIdx.x; j < cnt; j += BLOCK) {
-    const int64_t p = p0 + j;
+__device__ __forceinline__ void panel_record(const PanArgs& a, int64_t p, float4* s) {
+  {
     const uint32_t i0 = a.idx[3 * p], i1 = a.idx[3 * p + 1], i2 = a.idx[3 * p + 2];
     const float x0 = a.nx[i0], y0 = a.ny[i0], z0 = a.nz[i0];
     const float x1 = a.nx[i1], y1 = a.ny[i1], z1 = a.nz[i1];
@@ -155,7 +159,6 @@ __device__ __forceinline__ void stage_panels(const PanArgs& a, int64_t p0, int c
 #pragma clang fp contract(off)
       gx = (tx / ar) * ar; gy = (ty / ar) * ar; gz = (tz / ar) * ar; gs = sg * ar;
     }
-    float4* s = sh + PAN_F4 * j;
     s[0] = make_float4(x0, y0, z0, x1);
     s[1] = make_float4(y1, z1, x2, y2);
     s[2] = make_float4(z2, gx, gy, gz);
@@ -165,13 +168,23 @@ __device__ __forceinline__ void stage_panels(const PanArgs& a, int64_t p0, int c
   }
 }
 
-// one (target, panel) tree walk, accumulating into acc
-template <class R, bool GRAD>
-__device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, int maxlev, bool active, R* acc) {
+__device__ __forceinline__ void stage_panels(const PanArgs& a, int64_t p0, int cnt, float4* sh) {
+  for (int j = threadIdx.x; j < cnt; j += BLOCK) panel_record(a, p0 + j, sh + PAN_F4 * j);
+}
+
+// the records of all panels in global memory, for the near kernel
+static __global__ __launch_bounds__(BLOCK) void k_pan_geom(PanArgs a, float4* rec) {
+  const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (p < a.np) panel_record(a, p, rec + PAN_F4 * p);
+}
+
+// a panel's root quantities: level-0 absolute strengths (Kernels.h:1136-1146),
+// root centroid and 4 sqrt(area)
+template <class R>
+__device__ __forceinline__ void panel_root(const float4* s, R& gx, R& gy, R& gz, R& gs, R& cx0, R& cy0, R& cz0,
+                                           R& lim) {
   const float4 f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
   const R ar = f3.y;
-  // level-0 absolute strengths (Kernels.h:1136-1146)
-  R gx, gy, gz, gs, cx0, cy0, cz0, lim;
   if constexpr (sizeof(R) == 4) {
     gx = f2.y; gy = f2.z; gz = f2.w; gs = f3.x;
     cx0 = f4.x; cy0 = f4.y; cz0 = f4.z;
@@ -185,6 +198,14 @@ __device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, in
     centroid_exact<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, cx0, cy0, cz0);
     lim = sqrt(ar) * R(4);
   }
+}
+
+// the root decision of one (target, panel) pair: the reference's float test
+// (dist > 4 sqrt(area)), taken from a fast distance estimate when unambiguous and
+// from the exact expression otherwise; every caller takes it through here, so
+// the far and near kernels agree on it
+template <class R>
+__device__ __forceinline__ bool root_leaf(R tx, R ty, R tz, R cx0, R cy0, R cz0, R lim, int maxlev, bool active) {
   bool leaf0;
   if constexpr (sizeof(R) == 4) {
     const float ex = tx - cx0, ey = ty - cy0, ez = tz - cz0;
@@ -200,7 +221,17 @@ __device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, in
   } else {
     leaf0 = dist_exact<double>(tx, ty, tz, cx0, cy0, cz0) > lim;
   }
-  leaf0 = leaf0 || maxlev == 0;
+  return leaf0 || maxlev == 0;
+}
+
+// one (target, panel) tree walk, accumulating into acc
+template <class R, bool GRAD>
+__device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, int maxlev, bool active, R* acc) {
+  const float4 f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3];
+  const R ar = f3.y;
+  R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+  panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
+  const bool leaf0 = root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, maxlev, active);
   if (!wave_any(!leaf0 && active)) {
     if (active) leaf_acc<R, GRAD>(tx, ty, tz, cx0, cy0, cz0, gx, gy, gz, gs, acc);
     return;
@@ -259,6 +290,7 @@ __global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
   R acc[16];
   float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
   double gd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool has_near = false;
 #pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = R(0);
   for (int64_t p0 = 0; p0 < a.np; p0 += a.tile) {
@@ -266,7 +298,15 @@ __global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
     __syncthreads();
     stage_panels(a, p0, cnt, shp);
     __syncthreads();
-    for (int j = 0; j < cnt; ++j) walk_panel<R, GRAD>(tx, ty, tz, shp + PAN_F4 * j, a.maxlev, active, acc);
+    for (int j = 0; j < cnt; ++j) {
+      R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+      panel_root<R>(shp + PAN_F4 * j, gx, gy, gz, gs, cx0, cy0, cz0, lim);
+      if (root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, a.maxlev, active)) {
+        if (active) leaf_acc<R, GRAD>(tx, ty, tz, cx0, cy0, cz0, gx, gy, gz, gs, acc);
+      } else {
+        has_near = true;
+      }
+    }
     if (PREC == PREC_COMP) {
       two_sum(hi[0], lo[0], (float)acc[0]);
       two_sum(hi[1], lo[1], (float)acc[1]);
@@ -282,6 +322,7 @@ __global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
     }
   }
   if (!active) return;
+  if (has_near) a.near_list[atomicAdd(a.near_count, 1u)] = (int32_t)i;
   if (PREC == PREC_FP64) {
     double* u = (double*)a.u; double* v = (double*)a.v; double* w = (double*)a.w;
     u[i] += acc[0]; v[i] += acc[1]; w[i] += acc[2];
@@ -308,6 +349,62 @@ __global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
     if (GRAD) {
 #pragma unroll
       for (int k = 0; k < 9; ++k) ((float*)a.g[k])[i] = (float)((double)((float*)a.g[k])[i] + gd[k]);
+    }
+  }
+}
+
+// The pairs of the listed targets that the root decision sends into the
+// recursion, added to the outputs after k_pan.  One wave per listed target: lane l
+// takes panels l, l + 64, ... from the records, decides each root pair as k_pan
+// did and walks the near ones (Kernels.h:1028-1211); a fixed butterfly adds the
+// lanes' sums in FP64, so results do not depend on scheduling.  Grid-stride over
+// the list, whose length k_pan left in near_count.
+template <bool GRAD, int PREC>
+__global__ __launch_bounds__(BLOCK) void k_pan_near(PanArgs a, const float4* rec) {
+  using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
+  const int64_t listed = *a.near_count;
+  for (int64_t e = wave; e < listed; e += nwaves) {
+    const int64_t i = a.near_list[e];
+    const R tx = a.tx[i], ty = a.ty[i], tz = a.tz[i];
+    R acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = R(0);
+    for (int64_t p = lane; p < a.np; p += 64) {
+      const float4* s = rec + PAN_F4 * p;
+      R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+      panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
+      if (!root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, a.maxlev, true))
+        walk_panel<R, GRAD>(tx, ty, tz, s, a.maxlev, true, acc);
+    }
+    double u[3], g[9];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] = wave_sum((double)acc[d]);
+    if (GRAD) {
+      R gl[9];
+      grad_of<R>(acc, gl);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) g[k] = wave_sum((double)gl[k]);
+    }
+    if (lane != 0) continue;
+    if (PREC == PREC_FP64) {
+      double* uu = (double*)a.u; double* vv = (double*)a.v; double* ww = (double*)a.w;
+      uu[i] += u[0]; vv[i] += u[1]; ww[i] += u[2];
+      if (GRAD) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) ((double*)a.g[k])[i] += g[k];
+      }
+    } else {
+      float* uu = (float*)a.u; float* vv = (float*)a.v; float* ww = (float*)a.w;
+      uu[i] = (float)((double)uu[i] + u[0]);
+      vv[i] = (float)((double)vv[i] + u[1]);
+      ww[i] = (float)((double)ww[i] + u[2]);
+      if (GRAD) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) ((float*)a.g[k])[i] = (float)((double)((float*)a.g[k])[i] + g[k]);
+      }
     }
   }
 }

@@ -18,16 +18,32 @@ __global__ void k_ff_zero_u64_pan(unsigned long long* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
 }
 
-static hipError_t launch_pan(const PanArgs& a, bool grad, int prec, hipStream_t st) {
+// root pairs (k_pan), then the near pairs of the targets it listed (k_pan_near);
+// a.near_list / a.near_count must point at scratch for nt entries and a counter
+// (zeroed here), rec at np panel records
+static hipError_t launch_pan(const PanArgs& a, bool grad, int prec, hipStream_t st, float4* rec) {
   if (a.nt <= 0 || a.np <= 0) return hipSuccess;
+  if (!a.near_list || !a.near_count || !rec) return hipErrorInvalidValue;
   const size_t lds = (size_t)a.tile * PAN_F4 * sizeof(float4);
   const dim3 grid((unsigned)((a.nt + BLOCK - 1) / BLOCK));
-#define O3D_PL(G, P) \
-  if (grad == G && prec == P) { hipLaunchKernelGGL((k_pan<G, P>), grid, dim3(BLOCK), lds, st, a); return hipGetLastError(); }
+  // one wave per listed target, grid-stride: at most 16 waves per CU
+  const dim3 grid_near((unsigned)std::min<int64_t>((a.nt + 3) / 4, 4 * (int64_t)num_cus()));
+  hipLaunchKernelGGL(k_pan_g7_zero, dim3(1), dim3(64), 0, st, a.near_count);
+  hipLaunchKernelGGL(k_pan_geom, dim3((unsigned)((a.np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, a, rec);
+  hipError_t e = hipErrorInvalidValue;
+#define O3D_PL(G, P)                                                                      \
+  if (grad == G && prec == P) {                                                           \
+    hipLaunchKernelGGL((k_pan<G, P>), grid, dim3(BLOCK), lds, st, a);                     \
+    e = hipGetLastError();                                                                \
+    if (e == hipSuccess) {                                                                \
+      hipLaunchKernelGGL((k_pan_near<G, P>), grid_near, dim3(BLOCK), 0, st, a, (const float4*)rec); \
+      e = hipGetLastError();                                                              \
+    }                                                                                     \
+  }
   O3D_PL(true, PREC_FP32) O3D_PL(true, PREC_COMP) O3D_PL(true, PREC_FP64)
   O3D_PL(false, PREC_FP32) O3D_PL(false, PREC_COMP) O3D_PL(false, PREC_FP64)
 #undef O3D_PL
-  return hipErrorInvalidValue;
+  return e;
 }
 
 // far kernel, then (near) the near kernel over the targets the far kernel listed;
@@ -62,13 +78,14 @@ static hipError_t launch_pan_g7(PanArgs a, bool grad, int prec, bool near, hipSt
   return e;
 }
 
-// the near-target list, its counter and the panels' decision geometry in the
-// stream's workspace (caller holds stream_ws_mutex() until the launches are enqueued)
-static int g7_scratch(PanArgs* a, hipStream_t st, float4** geom) {
-  if (a->nt > INT32_MAX) return fail(O3D_EINVAL, "the 7-point rule takes at most %d targets", INT32_MAX);
+// the near-target list, its counter and f4_per_panel float4 of geometry per panel
+// in the stream's workspace (caller holds stream_ws_mutex() until the launches are
+// enqueued)
+static int g7_scratch(PanArgs* a, hipStream_t st, float4** geom, int f4_per_panel = 1) {
+  if (a->nt > INT32_MAX) return fail(O3D_EINVAL, "the panel kernels take at most %d targets", INT32_MAX);
   void* ws = nullptr;
   const size_t list = (sizeof(int32_t) * (size_t)a->nt + 255) & ~(size_t)255;
-  const int rc = stream_workspace(st, 256 + list + sizeof(float4) * (size_t)a->np, &ws);
+  const int rc = stream_workspace(st, 256 + list + sizeof(float4) * f4_per_panel * (size_t)a->np, &ws);
   if (rc != O3D_OK) return rc;
   a->near_count = (unsigned int*)ws;
   a->near_list = (int32_t*)((char*)ws + 256);
@@ -161,7 +178,10 @@ static int pan_dev(const o3d_panels* p, o3d_points* t, const o3d_opts* o, int pr
     if (e != hipSuccess) return fail(O3D_EDEVICE, "panel launch failed: %s", hipGetErrorString(e));
     return O3D_OK;
   }
-  const hipError_t e = launch_pan(a, grad, prec, st);
+  std::lock_guard<std::mutex> lk(stream_ws_mutex());
+  float4* rec = nullptr;
+  if ((rc = g7_scratch(&a, st, &rec, PAN_F4)) != O3D_OK) return rc;
+  const hipError_t e = launch_pan(a, grad, prec, st, rec);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "panel launch failed: %s", hipGetErrorString(e));
   return O3D_OK;
 }
