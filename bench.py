@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true", help="skip the served RK2 workload")
     ap.add_argument("--no-panels", action="store_true", help="skip the icosphere panel workload")
     ap.add_argument("--no-bem", action="store_true", help="skip the BEM influence-matrix workload")
-    ap.add_argument("--bem-only", action="store_true", help="run only the BEM influence-matrix workload")
+    ap.add_argument("--bem-only", action="store_true", help="run only the BEM workloads (matrix, right-hand side)")
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--sweep", action="store_true",
                     help="time every (precision, tile, tpt) on the N=1M workload, print them and exit")
@@ -105,6 +105,42 @@ def bem_leg(args, log):
     return bem
 
 
+def bem_rhs_leg(args, log):
+    """The BEM right-hand side of the icosphere (omega3d_amd.workloads.bem_rhs_run),
+    with its CPU baseline and check on the host unless --no-cpu: the sampled panels
+    through the reference's own rkernel_2vs_0p used backwards (oracle/_ref,
+    S = float, A = double) and the oracle's finalize and projection."""
+    from omega3d_amd import workloads
+    res, smp = workloads.bem_rhs_run(log=log)
+    if not args.no_cpu:
+        import oracle
+        from omega3d_amd.metrics import rel_errors
+        js = smp["panels"]
+        idx = np.ascontiguousarray(smp["idx"][js])
+        try:
+            t = time.perf_counter()
+            pu = oracle.ref_ptpan("rm", smp["pos"], smp["str"], smp["nodes"], idx, smp["area"][js], mode="f32d")
+            secs = time.perf_counter() - t
+            kind = "reference"
+        except FileNotFoundError:
+            t = time.perf_counter()
+            pu = oracle.ptpan("rm", smp["pos"], smp["str"], smp["nodes"], idx, smp["area"][js], mode="f32d")
+            secs = time.perf_counter() - t
+            kind = "port"
+        pu_fin, rhs = oracle.bem_rhs(smp["nodes"], idx, pu, smp["fs"], nunk=smp["nunk"])
+        P = smp["idx"].shape[0]
+        res["errors_vs_reference"] = {"pu_max_rel": rel_errors(smp["pu"], pu_fin)[0],
+                                      "rhs_max_rel": rel_errors(smp["rhs"].T, rhs.reshape(-1, smp["nunk"]).T)[0],
+                                      "panels": int(len(js))}
+        res["cpu"] = {"kind": kind, "cores": oracle.lib().orc_max_threads(),
+                      "what": "points_affect_panels' loop over the reference's rkernel_2vs_0p used backwards "
+                              "(Influence.h:1190-1210; S = float, A = double), OpenMP over panels",
+                      "sample": f"{len(js)} panels x all {smp['pos'].shape[1]} particles",
+                      "sample_s": round(secs, 3), "full_s_estimate": round(secs * P / len(js), 2)}
+        log(f"bench: bem rhs check {res['errors_vs_reference']} cpu {res['cpu']}")
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -129,7 +165,7 @@ def main():
         return
 
     if args.bem_only:
-        print(json.dumps({"bem_matrix": bem_leg(args, log)}))
+        print(json.dumps({"bem_matrix": bem_leg(args, log), "bem_rhs": bem_rhs_leg(args, log)}))
         return
 
     t0 = time.perf_counter()
@@ -310,6 +346,7 @@ def main():
     served = None if args.no_serve else workloads.served(duration=args.serve_seconds, log=log)
     panels = None if args.no_panels else workloads.panels_run(log=log)
     bem = None if args.no_bem else bem_leg(args, log)
+    bem_rhs = None if args.no_bem else bem_rhs_leg(args, log)
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "ff_far_exp_n1m_traffic.json" if factored else "p2p_exp_n1m_traffic.json")
@@ -373,6 +410,7 @@ def main():
         "served": served,
         "panels": panels,
         "bem_matrix": bem,
+        "bem_rhs": bem_rhs,
         "device": dev["name"],
     }
     print(json.dumps(out), flush=True)

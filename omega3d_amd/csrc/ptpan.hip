@@ -8,10 +8,15 @@
 // "target", and SUBTRACTS the sum from the panel's velocity (Influence.h:1196-1210).
 // Velocity only: the reference asserts on gradient requests (:1111).
 //
-// GPU form: one lane per target panel (its corners, area, sqrt(area) and exact
-// centroid in registers), particles staged per tile in LDS and broadcast, the
-// same depth-first tree walk as panels_affect_points (pan_kernels.hpp), so the
-// subdivision decisions are the reference's.
+// GPU form: k_ptpan takes one lane per target panel (its corners, area,
+// sqrt(area) and exact centroid in registers) and a slice of the particles per
+// workgroup row, staged per tile in LDS and broadcast (a boundary has too few
+// panels to fill the GPU otherwise); it takes the root decision of every pair and
+// evaluates the one-leaf pairs.  k_ptpan_combine adds the slices in order and
+// lists the panels that have particles near enough for the recursion.
+// k_ptpan_near gives each group of 4 listed panels one wave whose lanes walk
+// those pairs with the depth-first tree walk of panels_affect_points
+// (pan_kernels.hpp), so the subdivision decisions are the reference's.
 #include "o3d_api_internal.hpp"
 #include "pan_kernels.hpp"
 
@@ -32,8 +37,47 @@ struct PtPanArgs {
   void* u; void* v; void* w;   // float (decremented in place) or double (FP64: zeroed scratch)
   int32_t tile;
   int32_t maxlev;
+  int32_t* near_list;          // panels with a pair that needs the recursion
+  unsigned int* near_count;
+  double* part;                // per slice: 3 sums and a near flag per panel
+  int64_t slen;                // particles per slice
+  int32_t nslice;
 };
 
+// this lane's panel in the staged-panel layout of pan_kernels.hpp (the strength
+// fields are filled per particle)
+__device__ __forceinline__ void ptpan_record(const PtPanArgs& a, int64_t i, float4* s) {
+  const uint32_t i0 = a.idx[3 * i], i1 = a.idx[3 * i + 1], i2 = a.idx[3 * i + 2];
+  const float x0 = a.nx[i0], y0 = a.ny[i0], z0 = a.nz[i0];
+  const float x1 = a.nx[i1], y1 = a.ny[i1], z1 = a.nz[i1];
+  const float x2 = a.nx[i2], y2 = a.ny[i2], z2 = a.nz[i2];
+  const float ar = a.area[i];
+  float cx, cy, cz;
+  centroid_exact(x0, y0, z0, x1, y1, z1, x2, y2, z2, cx, cy, cz);
+  s[0] = make_float4(x0, y0, z0, x1);
+  s[1] = make_float4(y1, z1, x2, y2);
+  s[2] = make_float4(z2, 0.0f, 0.0f, 0.0f);
+  s[3] = make_float4(0.0f, ar, sqrtf(ar), 0.0f);
+  s[4] = make_float4(cx, cy, cz, 0.0f);
+  s[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+
+// particle j's strength on panel record s at level 0.  FP64: the walk forms
+// (alpha / area) * area in double from s[5].  FP32 and compensated: alpha itself;
+// the reference's float round trip (alpha / area) * area (Influence.h:1198,
+// Kernels.h:1136-1146) moves it by at most one float ulp and costs three IEEE
+// divisions per pair.
+template <int PREC>
+__device__ __forceinline__ void ptpan_strength(float4* s, float4 A) {
+  if (PREC == PREC_FP64) s[5] = make_float4(A.x, A.y, A.z, 0.0f);
+  else { s[2].y = A.x; s[2].z = A.y; s[2].w = A.z; }
+}
+
+// One lane per target panel, particles [y*slen, (y+1)*slen) of slice y =
+// blockIdx.y staged per tile in LDS: the root decision of every (panel, particle)
+// pair -- the reference's, through root_leaf -- and the pairs that are leaves at
+// the root.  Each slice leaves its sum (FP64) and whether a pair needs the
+// recursion in part[y][0..3][panel]; k_ptpan_combine adds the slices in order.
 template <int PREC>
 __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
   using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
@@ -41,29 +85,21 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   const bool active = i < a.np;
   const int64_t ic = active ? i : a.np - 1;
-  const uint32_t i0 = a.idx[3 * ic], i1 = a.idx[3 * ic + 1], i2 = a.idx[3 * ic + 2];
-  const float x0 = a.nx[i0], y0 = a.ny[i0], z0 = a.nz[i0];
-  const float x1 = a.nx[i1], y1 = a.ny[i1], z1 = a.nz[i1];
-  const float x2 = a.nx[i2], y2 = a.ny[i2], z2 = a.nz[i2];
-  const float ar = a.area[ic];
-  float cx, cy, cz;
-  centroid_exact(x0, y0, z0, x1, y1, z1, x2, y2, z2, cx, cy, cz);
-  // this lane's panel in the staged-panel layout of pan_kernels.hpp; the strength
-  // fields are filled per particle
   float4 s[PAN_F4];
-  s[0] = make_float4(x0, y0, z0, x1);
-  s[1] = make_float4(y1, z1, x2, y2);
-  s[2] = make_float4(z2, 0.0f, 0.0f, 0.0f);
-  s[3] = make_float4(0.0f, ar, sqrtf(ar), 0.0f);
-  s[4] = make_float4(cx, cy, cz, 0.0f);
-  s[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  ptpan_record(a, ic, s);
+  const float ar = s[3].y;
   R acc[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = R(0);
   float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
+  bool has_near = false;
+  R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+  panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
+  const int64_t q0 = (int64_t)blockIdx.y * a.slen;
+  const int64_t q1 = q0 + a.slen < a.ns ? q0 + a.slen : a.ns;
 
-  for (int64_t p0 = 0; p0 < a.ns; p0 += a.tile) {
-    const int cnt = (int)((a.ns - p0) < a.tile ? (a.ns - p0) : a.tile);
+  for (int64_t p0 = q0; p0 < q1; p0 += a.tile) {
+    const int cnt = (int)((q1 - p0) < a.tile ? (q1 - p0) : a.tile);
     __syncthreads();
     for (int j = threadIdx.x; j < cnt; j += BLOCK) {
       const int64_t g = p0 + j;
@@ -73,15 +109,19 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
     __syncthreads();
     for (int j = 0; j < cnt; ++j) {
       const float4 P = shs[2 * j], A = shs[2 * j + 1];
-      if (PREC == PREC_FP64) {
-        s[5] = make_float4(A.x, A.y, A.z, 0.0f);   // the walk forms (alpha / area) * area in double
+      if (root_leaf<R>(P.x, P.y, P.z, cx0, cy0, cz0, lim, a.maxlev, active)) {
+        if (!active) continue;
+        R ax, ay, az;
+        if (PREC == PREC_FP64) {
+#pragma clang fp contract(off)
+          ax = (R(A.x) / R(ar)) * R(ar); ay = (R(A.y) / R(ar)) * R(ar); az = (R(A.z) / R(ar)) * R(ar);
+        } else {
+          ax = A.x; ay = A.y; az = A.z;
+        }
+        leaf_acc<R, false>(P.x, P.y, P.z, cx0, cy0, cz0, ax, ay, az, R(0), acc);
       } else {
-        // alpha / area in float (Influence.h:1198), times the area at level 0
-        s[2].y = __fmul_rn(__fdiv_rn(A.x, ar), ar);
-        s[2].z = __fmul_rn(__fdiv_rn(A.y, ar), ar);
-        s[2].w = __fmul_rn(__fdiv_rn(A.z, ar), ar);
+        has_near = true;
       }
-      walk_panel<R, false>(P.x, P.y, P.z, s, a.maxlev, active, acc);
     }
     if (PREC == PREC_COMP) {
       two_sum(hi[0], lo[0], (float)acc[0]);
@@ -91,30 +131,168 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
     }
   }
   if (!active) return;
-  if (PREC == PREC_FP64) {
-    ((double*)a.u)[i] = -(double)acc[0];
-    ((double*)a.v)[i] = -(double)acc[1];
-    ((double*)a.w)[i] = -(double)acc[2];
-  } else if (PREC == PREC_FP32) {
-    ((float*)a.u)[i] -= (float)acc[0];
-    ((float*)a.v)[i] -= (float)acc[1];
-    ((float*)a.w)[i] -= (float)acc[2];
+  double* pt = a.part + (int64_t)blockIdx.y * 4 * a.np + i;
+  if (PREC == PREC_COMP) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pt[d * a.np] = (double)hi[d] + (double)lo[d];
   } else {
-    float* u = (float*)a.u; float* v = (float*)a.v; float* w = (float*)a.w;
-    u[i] = (float)((double)u[i] - ((double)hi[0] + (double)lo[0]));
-    v[i] = (float)((double)v[i] - ((double)hi[1] + (double)lo[1]));
-    w[i] = (float)((double)w[i] - ((double)hi[2] + (double)lo[2]));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) pt[d * a.np] = (double)acc[d];
+  }
+  pt[3 * a.np] = has_near ? 1.0 : 0.0;
+}
+
+// per panel: the slices' sums added in slice order and subtracted from the
+// panel's velocity (Influence.h:1206-1210); a panel with a pair that needs the
+// recursion in any slice is listed for k_ptpan_near
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ptpan_combine(PtPanArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= a.np) return;
+  double u[3] = {0.0, 0.0, 0.0};
+  bool near = false;
+  for (int y = 0; y < a.nslice; ++y) {
+    const double* pt = a.part + (int64_t)y * 4 * a.np + i;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) u[d] += pt[d * a.np];
+    near = near || pt[3 * a.np] != 0.0;
+  }
+  if (near) a.near_list[atomicAdd(a.near_count, 1u)] = (int32_t)i;
+  if (PREC == PREC_FP64) {
+    ((double*)a.u)[i] = -u[0];
+    ((double*)a.v)[i] = -u[1];
+    ((double*)a.w)[i] = -u[2];
+  } else {
+    float* uu = (float*)a.u; float* vv = (float*)a.v; float* ww = (float*)a.w;
+    uu[i] = (float)((double)uu[i] - u[0]);
+    vv[i] = (float)((double)vv[i] - u[1]);
+    ww[i] = (float)((double)ww[i] - u[2]);
   }
 }
 
-static hipError_t launch_ptpan(const PtPanArgs& a, int prec, hipStream_t st) {
-  if (a.np <= 0 || a.ns <= 0) return hipSuccess;
+// The near pairs of the listed panels, after k_ptpan_combine.  One wave per 4
+// listed panels, their records in LDS (a wave reads only its own, and LDS
+// operations of one wave complete in order): lane l takes particles l, l + 64, ...,
+// loads each once and tests it against the 4 panels -- a squared-distance test
+// with a 1% margin passes only candidates to the exact root decision -- and walks
+// the pairs that need the recursion (Kernels.h:1028-1211).  A fixed butterfly
+// adds the lanes' sums in FP64, subtracted from each panel's velocity.
+constexpr int PTPAN_GROUP = 4;
+
+template <int PREC>
+__global__ __launch_bounds__(BLOCK) void k_ptpan_near(PtPanArgs a) {
+  using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
+  __shared__ float4 recs[(BLOCK / 64) * PTPAN_GROUP * PAN_F4];
+  const int lane = threadIdx.x & 63;
+  float4* rw = recs + (threadIdx.x >> 6) * PTPAN_GROUP * PAN_F4;
+  const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
+  const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
+  const int64_t listed = *a.near_count;
+  const int64_t groups = (listed + PTPAN_GROUP - 1) / PTPAN_GROUP;
+  for (int64_t e = wave; e < groups; e += nwaves) {
+    const int64_t e0 = e * PTPAN_GROUP;
+    const int ng = (int)(listed - e0 < PTPAN_GROUP ? listed - e0 : PTPAN_GROUP);
+    __builtin_amdgcn_wave_barrier();
+    if (lane < ng) {
+      float4 s[PAN_F4];
+      ptpan_record(a, a.near_list[e0 + lane], s);
+#pragma unroll
+      for (int q = 0; q < PAN_F4; ++q) rw[lane * PAN_F4 + q] = s[q];
+    }
+    __builtin_amdgcn_wave_barrier();
+    float fx[PTPAN_GROUP], fy[PTPAN_GROUP], fz[PTPAN_GROUP], reach2[PTPAN_GROUP];
+#pragma unroll
+    for (int k = 0; k < PTPAN_GROUP; ++k) {
+      const float4 c = rw[k * PAN_F4 + 4], f3 = rw[k * PAN_F4 + 3];
+      fx[k] = c.x; fy[k] = c.y; fz[k] = c.z;
+      // an empty slot of the last group: a reach no particle has
+      reach2[k] = k < ng ? (f3.z * 4.0f) * (f3.z * 4.0f) * 1.01f : -1.0f;
+    }
+    R acc[PTPAN_GROUP][3];
+#pragma unroll
+    for (int k = 0; k < PTPAN_GROUP; ++k) acc[k][0] = acc[k][1] = acc[k][2] = R(0);
+    for (int64_t p = lane; p < a.ns; p += 64) {
+      const float px = a.sx[p], py = a.sy[p], pz = a.sz[p];
+#pragma unroll
+      for (int k = 0; k < PTPAN_GROUP; ++k) {
+        const float ex = px - fx[k], ey = py - fy[k], ez = pz - fz[k];
+        if (!(fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= reach2[k])) continue;
+        float4 s[PAN_F4];
+#pragma unroll
+        for (int q = 0; q < PAN_F4; ++q) s[q] = rw[k * PAN_F4 + q];
+        R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+        panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
+        if (root_leaf<R>(px, py, pz, cx0, cy0, cz0, lim, a.maxlev, true)) continue;
+        ptpan_strength<PREC>(s, make_float4(a.ax[p], a.ay[p], a.az[p], 0.0f));
+        R t16[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) t16[c] = R(0);
+        walk_panel<R, false>(px, py, pz, s, a.maxlev, true, t16);
+        acc[k][0] += t16[0]; acc[k][1] += t16[1]; acc[k][2] += t16[2];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < PTPAN_GROUP; ++k) {
+      double u[3];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) u[d] = wave_sum((double)acc[k][d]);
+      if (lane != 0 || k >= ng) continue;
+      const int64_t i = a.near_list[e0 + k];
+      if (PREC == PREC_FP64) {
+        ((double*)a.u)[i] -= u[0];
+        ((double*)a.v)[i] -= u[1];
+        ((double*)a.w)[i] -= u[2];
+      } else {
+        float* uu = (float*)a.u; float* vv = (float*)a.v; float* ww = (float*)a.w;
+        uu[i] = (float)((double)uu[i] - u[0]);
+        vv[i] = (float)((double)vv[i] - u[1]);
+        ww[i] = (float)((double)ww[i] - u[2]);
+      }
+    }
+  }
+}
+
+__global__ void k_ptpan_zero(unsigned int* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0u;
+}
+
+// root pairs over particle slices, their combination, then the near pairs of the
+// listed panels; the partial sums, the list and its counter in the stream's
+// workspace.  Enough slices of whole tiles for about 16 workgroups per CU.
+static int launch_ptpan(PtPanArgs a, int prec, hipStream_t st) {
+  if (a.np <= 0 || a.ns <= 0) return O3D_OK;
+  if (a.np > INT32_MAX) return fail(O3D_EINVAL, "points_affect_panels takes at most %d panels", INT32_MAX);
+  const int64_t nbx = (a.np + BLOCK - 1) / BLOCK;
+  const int64_t ntiles = (a.ns + a.tile - 1) / a.tile;
+  int64_t nsl = (16LL * num_cus() + nbx - 1) / nbx;
+  nsl = std::max<int64_t>(1, std::min<int64_t>({nsl, ntiles, 65535}));
+  a.slen = ((ntiles + nsl - 1) / nsl) * a.tile;
+  a.nslice = (int32_t)((a.ns + a.slen - 1) / a.slen);
+  std::lock_guard<std::mutex> lk(stream_ws_mutex());
+  void* ws = nullptr;
+  const size_t list = (sizeof(int32_t) * (size_t)a.np + 255) & ~(size_t)255;
+  const size_t part = sizeof(double) * 4 * (size_t)a.nslice * (size_t)a.np;
+  int rc = stream_workspace(st, 256 + list + part, &ws);
+  if (rc != O3D_OK) return rc;
+  a.near_count = (unsigned int*)ws;
+  a.near_list = (int32_t*)((char*)ws + 256);
+  a.part = (double*)((char*)ws + 256 + list);
   const size_t lds = (size_t)a.tile * 2 * sizeof(float4);
-  const dim3 grid((unsigned)((a.np + BLOCK - 1) / BLOCK));
-  if (prec == PREC_FP32) hipLaunchKernelGGL((k_ptpan<PREC_FP32>), grid, dim3(BLOCK), lds, st, a);
-  else if (prec == PREC_COMP) hipLaunchKernelGGL((k_ptpan<PREC_COMP>), grid, dim3(BLOCK), lds, st, a);
-  else hipLaunchKernelGGL((k_ptpan<PREC_FP64>), grid, dim3(BLOCK), lds, st, a);
-  return hipGetLastError();
+  const dim3 grid((unsigned)nbx, (unsigned)a.nslice);
+  const dim3 grid1((unsigned)nbx);
+  const dim3 grid_near((unsigned)std::min<int64_t>((a.np + 15) / 16, 4 * (int64_t)num_cus()));
+  hipLaunchKernelGGL(k_ptpan_zero, dim3(1), dim3(64), 0, st, a.near_count);
+#define O3D_PP(P)                                                                  \
+  if (prec == P) {                                                                 \
+    hipLaunchKernelGGL((k_ptpan<P>), grid, dim3(BLOCK), lds, st, a);               \
+    hipLaunchKernelGGL((k_ptpan_combine<P>), grid1, dim3(BLOCK), 0, st, a);        \
+    hipLaunchKernelGGL((k_ptpan_near<P>), grid_near, dim3(BLOCK), 0, st, a);       \
+  }
+  O3D_PP(PREC_FP32) O3D_PP(PREC_COMP) O3D_PP(PREC_FP64)
+#undef O3D_PP
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
+  return O3D_OK;
 }
 
 static int check_ptpan(const o3d_points* src, const o3d_panels* targ, float* const* vel, const o3d_opts* o) {
@@ -318,8 +496,7 @@ int o3d_points_affect_panels(const o3d_points* src, const o3d_panels* targ, floa
     if (!f64)
       for (int k = 0; k < 3; ++k)
         HIPCHK(hipMemcpyAsync(vb + vsz * np * k, vel[k], sizeof(float) * np, hipMemcpyHostToDevice, st));
-    hipError_t e = launch_ptpan(a, o->precision, st);
-    if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
+    if ((rc = launch_ptpan(a, o->precision, st)) != O3D_OK) return rc;
     if (f64) {
       std::vector<double> h((size_t)3 * np);
       HIPCHK(hipMemcpyAsync(h.data(), vb, vbytes, hipMemcpyDeviceToHost, st));
@@ -349,8 +526,7 @@ int o3d_points_affect_panels_dev(const o3d_points* src, const o3d_panels* targ, 
   if (src->n > 0 && targ->np > 0) {
     PtPanArgs a = ptpan_args(src, targ, o);
     a.u = vel[0]; a.v = vel[1]; a.w = vel[2];
-    hipError_t e = launch_ptpan(a, o->precision, (hipStream_t)o->stream);
-    if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
+    if ((rc = launch_ptpan(a, o->precision, (hipStream_t)o->stream)) != O3D_OK) return rc;
   }
   fill_stats(stats, 0.0, ptpan_flops(targ->np, src->n), (double)targ->np * (double)src->n);
   return O3D_OK;
@@ -396,8 +572,7 @@ int o3d_bem_rhs_dev(const o3d_points* vort, int32_t nvort, const o3d_panels* bdr
     if (vort[k].n == 0) continue;
     PtPanArgs pa = ptpan_args(&vort[k], bdry, opts);
     pa.u = pu[0]; pa.v = pu[1]; pa.w = pu[2];
-    hipError_t e = launch_ptpan(pa, opts->precision, st);
-    if (e != hipSuccess) return fail(O3D_EDEVICE, "points_affect_panels launch failed: %s", hipGetErrorString(e));
+    if ((rc = launch_ptpan(pa, opts->precision, st)) != O3D_OK) return rc;
   }
   hipLaunchKernelGGL(k_bem_rhs, dim3((unsigned)((a.np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, a);
   HIPCHK(hipGetLastError());

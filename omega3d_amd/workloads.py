@@ -12,6 +12,8 @@
   panels()         20,480-panel icosphere -> 524,288 shell targets, recursive
                    centroid rule, compensated FP32.
   bem_matrix_run() the BEM influence matrix of the same icosphere on itself.
+  bem_rhs_run()    the BEM right-hand side of the icosphere from the shell points as
+                   vortex particles.
 
     python -m omega3d_amd.workloads batched|served|panels
 """
@@ -266,6 +268,63 @@ def bem_matrix_run(levels=5, nunk=3, reps=3, sample_sources=32, log=print):
            "far_write_gbs": round(gb / best.far_ms * 1e3, 1)}
     log("bem_matrix: " + json.dumps(res))
     sample = {"nodes": nodes, "idx": idx, "area": area, "sources": js, "columns": cols, "nunk": nunk}
+    return res, sample
+
+
+def bem_rhs_run(n_particles=524288, levels=5, nunk=3, reps=5, sample_panels=128, log=print):
+    """The BEM right-hand side of the 20,480-panel icosphere (BEMHelper.h:83-117) from
+    the 524,288 shell points of the panel workload taken as vortex particles
+    (strengths uniform in [-1, 1]^3 / N), compensated FP32, a unit freestream along x
+    and no body motion: zero the panel velocities, points_affect_panels, finalize,
+    project.  Returns (results, sample) with the velocities and rhs rows of
+    sample_panels spread panels, for a check on the host."""
+    L = _lib.lib
+    pnl = scenes.icosphere_panels(levels)
+    P = pnl.area.shape[0]
+    pos = scenes.shell_points(n_particles)
+    st = (scenes.Rng(scenes.SEED_BASE + 5, 1).uniform(3 * n_particles).reshape(3, -1) * 2.0 - 1.0) / n_particles
+    st = st.astype(np.float32)
+    D = _lib.DeviceArray.from_host
+    dn, di, da = D(pnl.nodes), D(pnl.idx.view(np.float32)), D(pnl.area)
+    bd = _lib.Panels()
+    bd.np, bd.nn = P, pnl.nodes.shape[1]
+    for d in range(3):
+        bd.x[d] = dn.row(d)
+    bd.idx, bd.area = di.ptr, da.ptr
+    dp, ds = D(pos), D(st)
+    pts = (_lib.Points * 1)()
+    pts[0].n = n_particles
+    for d in range(3):
+        pts[0].x[d], pts[0].s[d] = dp.row(d), ds.row(d)
+    dpu, drhs = _lib.DeviceArray((3, P)), _lib.DeviceArray((nunk * P,))
+    pu = (C.c_void_p * 3)(*[dpu.row(d) for d in range(3)])
+    o = _lib.make_opts("rm", False, "comp")
+    fs = (1.0, 0.0, 0.0)
+    b = _lib.BemRhsOpts()
+    for d in range(3):
+        b.fs[d] = fs[d]
+    b.nunk = nunk
+    call = lambda: _lib.check(L.o3d_bem_rhs_dev(pts, 1, C.byref(bd), C.byref(b), C.byref(o), pu,
+                                                C.c_void_p(drhs.ptr)))
+    call()
+    _lib.synchronize()
+    e0, e1 = _lib.Event(), _lib.Event()
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    _lib.synchronize()
+    ms = e0.elapsed_ms(e1) / reps
+    pairs = float(P) * n_particles
+    res = {"workload": f"BEM right-hand side of the {P}-panel icosphere from {n_particles} vortex particles in "
+                       f"1.02<r<1.5 (points_affect_panels, recursive rule maxlev 3, compensated FP32; finalize "
+                       f"with a unit freestream; {nunk} unknowns per panel)",
+           "ms": round(ms, 3), "pairs": pairs, "g_pairs_per_s": round(pairs / ms / 1e6, 2)}
+    log("bem_rhs: " + json.dumps(res))
+    js = np.linspace(0, P - 1, sample_panels).astype(np.int64)
+    vel, rhs = dpu.to_host(), drhs.to_host().reshape(P, nunk)
+    sample = {"nodes": pnl.nodes, "idx": pnl.idx.reshape(-1, 3), "area": pnl.area, "pos": pos, "str": st,
+              "panels": js, "pu": vel[:, js], "rhs": rhs[js], "fs": fs, "nunk": nunk}
     return res, sample
 
 
