@@ -141,6 +141,38 @@ def bem_rhs_leg(args, log):
     return res
 
 
+def scene_io_leg(c, log):
+    """Row f3's data formats on the headline workload: the 1,048,576 particles of
+    the two rings written as the reference's VTU file (omega3d_amd.scene), timed,
+    and beside it the reference's own writer (VtkXmlWriter.h through
+    oracle/_ref/libo3dref_vtk.so) on the same arrays, with a byte comparison."""
+    import tempfile
+    from omega3d_amd import scene
+    vel = np.zeros((3, c.n), np.float32)
+    t = time.perf_counter()
+    data = scene.points_vtu_bytes(c.pos, 0.0, c.str, c.rad, vel)
+    secs = time.perf_counter() - t
+    out = {"workload": f"the {c.n} particles of the two rings written as a VTU file (position, circulation, "
+                       "radius, velocity; base64 binary)", "bytes": len(data), "seconds": round(secs, 3),
+           "mb_per_s": round(len(data) / secs / 1e6, 1)}
+    ref = os.path.join(os.path.dirname(os.path.abspath(__file__)), "oracle", "_ref", "libo3dref_vtk.so")
+    if os.path.exists(ref):
+        L = C.CDLL(ref)
+        L.ref_write_points_vtk.argtypes = [C.c_char_p, C.c_int64] + [C.c_void_p] * 10 + [C.c_double, C.c_int]
+        L.ref_write_points_vtk.restype = C.c_int
+        arrs = [np.ascontiguousarray(x, np.float32) for x in (c.pos[0], c.pos[1], c.pos[2], c.str[0], c.str[1],
+                                                             c.str[2], c.rad, vel[0], vel[1], vel[2])]
+        with tempfile.TemporaryDirectory() as d:
+            fn = os.path.join(d, "ref.vtu")
+            t = time.perf_counter()
+            L.ref_write_points_vtk(fn.encode(), c.n, *[x.ctypes.data_as(C.c_void_p) for x in arrs], 0.0, 0)
+            rsecs = time.perf_counter() - t
+            same = open(fn, "rb").read() == data
+        out["reference_writer"] = {"kind": "reference", "seconds": round(rsecs, 3), "byte_identical": same}
+    log("scene_io: " + json.dumps(out))
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -345,6 +377,7 @@ def main():
     batched = None if args.no_batch else workloads.batched(log=log)
     served = None if args.no_serve else workloads.served(duration=args.serve_seconds, log=log)
     panels = None if args.no_panels else workloads.panels_run(log=log)
+    scene_io = None if args.no_cpu else scene_io_leg(c, log)
     bem = None if args.no_bem else bem_leg(args, log)
     bem_rhs = None if args.no_bem else bem_rhs_leg(args, log)
 
@@ -409,6 +442,7 @@ def main():
         "batched": batched,
         "served": served,
         "panels": panels,
+        "scene_io": scene_io,
         "bem_matrix": bem,
         "bem_rhs": bem_rhs,
         "device": dev["name"],
