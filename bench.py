@@ -377,6 +377,25 @@ def main():
     batched = None if args.no_batch else workloads.batched(log=log)
     served = None if args.no_serve else workloads.served(duration=args.serve_seconds, log=log)
     panels = None if args.no_panels else workloads.panels_run(log=log)
+    if panels is not None and not args.no_cpu:
+        # the c2 workload's recursive rule through the reference's own rkernel_2vs_0pg
+        # (oracle/_ref, S = float, A = double) on a spread sample of the targets
+        import oracle
+        pan = scenes.icosphere_panels()
+        tp = scenes.shell_points(524288)
+        sel = np.linspace(0, tp.shape[1] - 1, 1024).astype(np.int64)
+        try:
+            t = time.perf_counter()
+            oracle.ref_pan("rm", pan.nodes, pan.idx.reshape(-1, 3), pan.ts, pan.area, pan.ss, tp[:, sel], True, "f32d")
+            secs = time.perf_counter() - t
+            panels["cpu"] = {"kind": "reference", "cores": oracle.lib().orc_max_threads(),
+                             "what": "panels_affect_points' loop over the reference's rkernel_2vs_0pg "
+                                     "(Influence.h:729-760; S = float, A = double), OpenMP over targets",
+                             "sample": f"{len(sel)} spread targets x all {pan.np} panels", "sample_s": round(secs, 3),
+                             "full_s_estimate": round(secs * tp.shape[1] / len(sel), 2)}
+        except FileNotFoundError:
+            panels["cpu"] = None
+        log(f"bench: panels cpu {panels['cpu']}")
     scene_io = None if args.no_cpu else scene_io_leg(c, log)
     bem = None if args.no_bem else bem_leg(args, log)
     bem_rhs = None if args.no_bem else bem_rhs_leg(args, log)
