@@ -176,8 +176,8 @@ def scene_io_leg(c, log):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sharding over several GPUs is designed, not implemented)", file=sys.stderr)
         sys.exit(2)
     from omega3d_amd import _lib, scenes
     from omega3d_amd.metrics import field_errors
