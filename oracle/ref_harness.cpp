@@ -218,7 +218,7 @@ int ref_pan_d(int grads, int maxlev, int64_t np, const double* x, const double* 
   return 0;
 }
 
-// the reference's per-interaction flop counts for this build's core
+// the reference's per-interaction flop counts for the core this library was compiled with
 int ref_flops(int which) {
   switch (which) {
     case 0: return (int) flops_0v_0bg<float>();
