@@ -351,6 +351,19 @@ def main():
                    "sample": f"Vc-style AVX2 restatement of Influence.h:388-442 (8 float lanes, rsqrt/rcp "
                              f"estimates, OpenMP over targets), first {ntc} targets x all {N} sources, "
                              f"{dt:.1f} s"}
+            # beside it, the reference's own scalar kernels (oracle/_ref: Kernels.h and
+            # CoreFunc.h compiled in place, EXP core, S = float, A = double: the
+            # non-Vc build) on a few targets of the same workload
+            try:
+                nr = 16 * nth
+                t = time.perf_counter()
+                oracle.ref_p2p("exp", c.pos, c.str, c.rad, c.pos[:, :nr], c.rad[:nr], True, "f32d")
+                rdt = time.perf_counter() - t
+                cpu["reference_scalar"] = {
+                    "value": round(nr * float(N) / rdt / 1e9, 4), "unit": "G interactions/s", "cores": nth,
+                    "kind": "reference", "sample": f"first {nr} targets x all {N} sources, {rdt:.1f} s"}
+            except FileNotFoundError:
+                cpu["reference_scalar"] = None
             log(f"bench: cpu baseline {cpu}")
 
     from omega3d_amd import workloads
