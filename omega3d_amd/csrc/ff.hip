@@ -260,10 +260,25 @@ std::mutex& stream_ws_mutex() { return g_ws_mu; }
 
 // caller holds stream_ws_mutex() for the whole enqueue, so that no other thread
 // frees or grows this stream's workspace between its sizing and the last launch
-// using it; a larger request waits for the stream to drain, then reallocates
+// using it; a larger request waits for the stream to drain, then reallocates.  A
+// stream being captured into a graph cannot do that: the workspace must already
+// be large enough (a call of the same size outside the capture sizes it).
+bool stream_capturing(hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
+bool stream_workspace_fits(hipStream_t st, size_t bytes) {
+  auto it = g_ws.find(st);
+  return it != g_ws.end() && it->second.bytes >= bytes;
+}
+
 int stream_workspace(hipStream_t st, size_t bytes, void** out) {
   Work& w = g_ws[st];
   if (w.bytes < bytes) {
+    if (stream_capturing(st))
+      return fail(O3D_EINVAL, "the stream is being captured and its device workspace would have to grow to %zu "
+                              "bytes; make the same call once outside the capture first", bytes);
     if (w.p) {
       HIPCHK(hipStreamSynchronize(st));
       HIPCHK(hipFree(w.p));

@@ -266,12 +266,15 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   const int nsl = auto_cfg && tpt == 1 ? split_plan(a.nt, a.ns, tile, grad, o->precision, &slen, &pbytes) : 1;
   if (nsl > 1) {
     std::lock_guard<std::mutex> lk(stream_ws_mutex());
-    void* part = nullptr;
-    int rc = stream_workspace(st, pbytes, &part);
-    if (rc != O3D_OK) return rc;
-    hipError_t e = dispatch_p2p_split(o->core, a, thick, grad, o->precision, nsl, slen, part, st);
-    if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
-    return O3D_OK;
+    // under graph capture the workspace cannot grow: the one-pass kernel then
+    if (!stream_capturing(st) || stream_workspace_fits(st, pbytes)) {
+      void* part = nullptr;
+      int rc = stream_workspace(st, pbytes, &part);
+      if (rc != O3D_OK) return rc;
+      hipError_t e = dispatch_p2p_split(o->core, a, thick, grad, o->precision, nsl, slen, part, st);
+      if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
+      return O3D_OK;
+    }
   }
   hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));

@@ -37,6 +37,10 @@ size_t ff_workspace_bytes(int64_t ns, int64_t nt, bool same, int tile);
 // share it); hold stream_ws_mutex() from stream_workspace until the last enqueue
 std::mutex& stream_ws_mutex();
 int stream_workspace(hipStream_t st, size_t bytes, void** out);
+// whether st is being captured into a graph; whether its workspace holds bytes
+// already (callers hold stream_ws_mutex())
+bool stream_capturing(hipStream_t st);
+bool stream_workspace_fits(hipStream_t st, size_t bytes);
 // a collection large enough for the factored method to pay when the caller leaves the choice open
 constexpr int64_t FF_AUTO_MIN_N = 65536;
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);

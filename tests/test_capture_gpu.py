@@ -1,0 +1,117 @@
+"""Device entry points under HIP graph capture.
+
+A small direct call splits its sources over workgroups and takes its partial sums
+from the stream's device workspace, which cannot grow while the stream is being
+captured.  Captured after a call of the same size on that stream, the call replays
+bitwise equal to its eager result, on every replay, with eager work in between.
+Captured on a stream whose workspace is too small, it takes the one-pass kernel
+instead (the same sum in another order).
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from omega3d_amd import scenes
+from omega3d_amd.metrics import rel_errors
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def env():
+    from omega3d_amd import _lib
+    assert _lib.lib.o3d_init(-1) == 0, _lib.last_error()
+    hip = C.CDLL("libamdhip64.so")
+    for f in ("hipStreamBeginCapture", "hipStreamEndCapture", "hipGraphInstantiate", "hipGraphLaunch",
+              "hipGraphExecDestroy", "hipGraphDestroy"):
+        getattr(hip, f).restype = C.c_int
+    return _lib, hip
+
+
+def _setup(_lib, n, stream):
+    p = scenes.ring(n, sigma=0.05)
+    D = _lib.DeviceArray.from_host
+    keep = [D(p.pos), D(p.str), D(p.rad)]
+    dv, dg = _lib.DeviceArray((3, n)), _lib.DeviceArray((9, n))
+    s = _lib.Points()
+    s.n = n
+    for d in range(3):
+        s.x[d], s.s[d], s.u[d] = keep[0].row(d), keep[1].row(d), dv.row(d)
+    s.r = keep[2].ptr
+    for k in range(9):
+        s.ug[k] = dg.row(k)
+    o = _lib.make_opts("rm", True, "comp", self_term=1, stream=stream)
+    return keep, dv, dg, s, o
+
+
+def _capture(_lib, hip, stream, fn):
+    assert hip.hipStreamBeginCapture(C.c_void_p(stream), 0) == 0   # hipStreamCaptureModeGlobal
+    rc = fn()
+    graph = C.c_void_p()
+    end = hip.hipStreamEndCapture(C.c_void_p(stream), C.byref(graph))
+    assert rc == 0, _lib.last_error()
+    assert end == 0
+    exe = C.c_void_p()
+    assert hip.hipGraphInstantiate(C.byref(exe), graph, None, None, C.c_size_t(0)) == 0
+    return graph, exe
+
+
+def test_captured_split_call_replays_bitwise(env):
+    _lib, hip = env
+    from omega3d_amd.batch import stream_create, stream_destroy
+    st = stream_create()
+    try:
+        keep, dv, dg, s, o = _setup(_lib, 8192, st)
+        call = lambda: _lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None)
+        # eager, which also sizes the stream's workspace
+        dv.zero(); dg.zero(); _lib.synchronize()
+        _lib.check(call())
+        _lib.check(_lib.lib.o3d_stream_synchronize(st))
+        v0, g0 = dv.to_host(), dg.to_host()
+        graph, exe = _capture(_lib, hip, st, call)
+        try:
+            other = _setup(_lib, 3000, None)
+            for rep in range(3):
+                dv.zero(); dg.zero(); _lib.synchronize()
+                assert hip.hipGraphLaunch(exe, C.c_void_p(st)) == 0
+                _lib.check(_lib.lib.o3d_stream_synchronize(st))
+                np.testing.assert_array_equal(dv.to_host(), v0)
+                np.testing.assert_array_equal(dg.to_host(), g0)
+                # eager work on the library's own stream between replays
+                _lib.check(_lib.lib.o3d_points_affect_points_dev(C.byref(other[3]), C.byref(other[3]),
+                                                                 C.byref(other[4]), None))
+                _lib.synchronize()
+        finally:
+            hip.hipGraphExecDestroy(exe)
+            hip.hipGraphDestroy(graph)
+    finally:
+        stream_destroy(st)
+
+
+def test_capture_on_a_fresh_stream_falls_back_to_one_pass(env):
+    _lib, hip = env
+    from omega3d_amd.batch import stream_create, stream_destroy
+    st = stream_create()
+    try:
+        # 16,384 targets: a partial buffer four times that of the 8,192 call above,
+        # so the capture cannot use a workspace left on a reused stream handle
+        keep, dv, dg, s, o = _setup(_lib, 16384, st)
+        call = lambda: _lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None)
+        dv.zero(); dg.zero(); _lib.synchronize()
+        _lib.check(_lib.lib.o3d_stream_synchronize(st))
+        graph, exe = _capture(_lib, hip, st, call)
+        try:
+            assert hip.hipGraphLaunch(exe, C.c_void_p(st)) == 0
+            _lib.check(_lib.lib.o3d_stream_synchronize(st))
+            v1, g1 = dv.to_host(), dg.to_host()
+        finally:
+            hip.hipGraphExecDestroy(exe)
+            hip.hipGraphDestroy(graph)
+        dv.zero(); dg.zero(); _lib.synchronize()
+        _lib.check(call())
+        _lib.check(_lib.lib.o3d_stream_synchronize(st))
+        v0, g0 = dv.to_host(), dg.to_host()
+        assert rel_errors(v1, v0)[0] <= 2e-6 and rel_errors(g1, g0)[0] <= 2e-6
+    finally:
+        stream_destroy(st)
