@@ -455,6 +455,8 @@ int o3d_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 int o3d_memset_dev(void* dst, int value, size_t bytes);
 int o3d_device_synchronize(void);
 int o3d_stream_create(void** stream);
+/* Waits for the stream, frees the device workspace the library keeps for it (the
+ * scratch of factored, split and panel calls on that stream), then destroys it. */
 int o3d_stream_destroy(void* stream);
 int o3d_stream_synchronize(void* stream);
 int o3d_event_create(void** ev);

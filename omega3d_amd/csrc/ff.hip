@@ -273,6 +273,17 @@ bool stream_workspace_fits(hipStream_t st, size_t bytes) {
   return it != g_ws.end() && it->second.bytes >= bytes;
 }
 
+void release_stream_workspace(hipStream_t st) {
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  auto it = g_ws.find(st);
+  if (it == g_ws.end()) return;
+  if (it->second.p) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(it->second.p);
+  }
+  g_ws.erase(it);
+}
+
 int stream_workspace(hipStream_t st, size_t bytes, void** out) {
   Work& w = g_ws[st];
   if (w.bytes < bytes) {

@@ -610,7 +610,12 @@ int o3d_stream_create(void** s) {
   HIPCHK(hipStreamCreateWithFlags((hipStream_t*)s, hipStreamNonBlocking));
   return O3D_OK;
 }
-int o3d_stream_destroy(void* s) { HIPCHK(hipStreamDestroy((hipStream_t)s)); return O3D_OK; }
+int o3d_stream_destroy(void* s) {
+  // the stream's device workspace goes with it (a later stream may reuse the handle)
+  release_stream_workspace((hipStream_t)s);
+  HIPCHK(hipStreamDestroy((hipStream_t)s));
+  return O3D_OK;
+}
 int o3d_stream_synchronize(void* s) { HIPCHK(hipStreamSynchronize((hipStream_t)s)); return O3D_OK; }
 int o3d_event_create(void** e) {
   int rc = ensure_init();

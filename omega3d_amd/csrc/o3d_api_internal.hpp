@@ -40,6 +40,8 @@ int stream_workspace(hipStream_t st, size_t bytes, void** out);
 // whether st is being captured into a graph; whether its workspace holds bytes
 // already (callers hold stream_ws_mutex())
 bool stream_capturing(hipStream_t st);
+// frees st's workspace (before the stream is destroyed); takes stream_ws_mutex()
+void release_stream_workspace(hipStream_t st);
 bool stream_workspace_fits(hipStream_t st, size_t bytes);
 // a collection large enough for the factored method to pay when the caller leaves the choice open
 constexpr int64_t FF_AUTO_MIN_N = 65536;
