@@ -132,6 +132,7 @@ struct BmFarArgs {
   uint2* list;               // near pairs (target, source)
   unsigned long long* count; // near pairs found
   unsigned long long cap;    // list capacity
+  int32_t vec4;              // coeffs and every column 16-byte aligned
 };
 
 constexpr int BM_TILE = BLOCK;   // sources staged per LDS pass
@@ -144,8 +145,15 @@ enum { BM_COUNT = 0, BM_WRITE = 1 };
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_bm_far(BmFarArgs a) {
   __shared__ float4 sh[BM_TILE * 3];
+  // per wave: its 3 column segments of 64 rows x 3 floats (float4 stores)
+  __shared__ __align__(16) float wbs[(BLOCK / 64) * 3 * 64 * 3];
   const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  float* wb = wbs + (tid >> 6) * 3 * 64 * 3;
   const int64_t i = (int64_t)blockIdx.x * BLOCK + tid;
+  const int64_t i0 = i - lane;   // the wave's first row
+  // float4 stores: every row of the wave valid and the columns 16-byte aligned
+  const bool vec = MODE == BM_WRITE && a.vec4 && i0 + 64 <= a.nt;
   const bool valid = i < a.nt;
   const int64_t ic = valid ? i : a.nt - 1;
   const float4 t0 = a.trec[ic * BM_F4];
@@ -189,7 +197,6 @@ __global__ __launch_bounds__(BLOCK) void k_bm_far(BmFarArgs a) {
       }
       const unsigned long long m = __builtin_amdgcn_ballot_w64(nearp);
       if (m) {
-        const int lane = tid & 63;
         const int first = __builtin_ctzll(m);
         unsigned long long base = 0;
         if (lane == first) base = atomicAdd(a.count, (unsigned long long)__builtin_popcountll(m));
@@ -198,35 +205,64 @@ __global__ __launch_bounds__(BLOCK) void k_bm_far(BmFarArgs a) {
                                                                       __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
         if (nearp && slot < a.cap) a.list[slot] = make_uint2((unsigned)i, (unsigned)j);
       }
-      if (!valid || nearp) continue;
-      float* col = a.coeffs + (j * nunk) * R + i * nunk;
+      if (!vec && (!valid || nearp)) continue;
+      // this pair's coefficients [unknown k][component m]
+      float cv[3][3];
       if (diag) {
         // Coefficients.h:399-421, then the scaling of :428-433
-        col[0] = 0.f; col[1] = tp; if (nunk == 3) col[2] = 0.f;
-        col += R;
-        col[0] = -tp; col[1] = 0.f; if (nunk == 3) col[2] = 0.f;
-        if (nunk == 3) { col += R; col[0] = 0.f; col[1] = 0.f; col[2] = tp; }
+        cv[0][0] = 0.f; cv[0][1] = tp; cv[0][2] = 0.f;
+        cv[1][0] = -tp; cv[1][1] = 0.f; cv[1][2] = 0.f;
+        cv[2][0] = 0.f; cv[2][1] = 0.f; cv[2][2] = tp;
+      } else {
+        // one leaf: kernel_0vs_0p at the centroids (Kernels.h:115-133), sigma = 0
+        const float rr = frsq(d2);
+        const float w = rr * rr * rr * s1.x * fac;     // K sa / (4 pi)
+        const float gx = w * ex, gy = w * ey, gz = w * ez;
+        const float sb[2][3] = {{s1.y, s1.z, s1.w}, {s2.x, s2.y, s2.z}};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          float ux, uy, uz;
+          if (k < 2) {   // (b x G): the reference's s x d order, Kernels.h:128-130
+            ux = gz * sb[k][1] - gy * sb[k][2];
+            uy = gx * sb[k][2] - gz * sb[k][0];
+            uz = gy * sb[k][0] - gx * sb[k][1];
+          } else {
+            ux = gx; uy = gy; uz = gz;
+          }
+          cv[k][0] = ux * tb[0][0] + uy * tb[0][1] + uz * tb[0][2];
+          cv[k][1] = ux * tb[1][0] + uy * tb[1][1] + uz * tb[1][2];
+          cv[k][2] = ux * tb[2][0] + uy * tb[2][1] + uz * tb[2][2];
+        }
+      }
+      if (vec) {
+        // the wave's column segments (64 rows x nunk floats each) through LDS, out as
+        // float4 stores; the entries of a listed pair are written too and replaced
+        // by k_bm_near, which runs after this kernel
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (k == nunk) break;
+#pragma unroll
+          for (int m = 0; m < 3; ++m)
+            if (m < nunk) wb[k * 64 * 3 + lane * nunk + m] = cv[k][m];
+        }
+        __builtin_amdgcn_wave_barrier();
+        float* col0 = a.coeffs + (j * nunk) * R + i0 * nunk;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if (k == nunk) break;
+          if (lane < 16 * nunk)
+            reinterpret_cast<float4*>(col0 + k * R)[lane] = reinterpret_cast<const float4*>(wb + k * 64 * 3)[lane];
+        }
+        __builtin_amdgcn_wave_barrier();
         continue;
       }
-      // one leaf: kernel_0vs_0p at the centroids (Kernels.h:115-133), sigma = 0
-      const float rr = frsq(d2);
-      const float w = rr * rr * rr * s1.x * fac;     // K sa / (4 pi)
-      const float gx = w * ex, gy = w * ey, gz = w * ez;
-      const float sb[2][3] = {{s1.y, s1.z, s1.w}, {s2.x, s2.y, s2.z}};
+      float* col = a.coeffs + (j * nunk) * R + i * nunk;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         if (k == nunk) break;
-        float ux, uy, uz;
-        if (k < 2) {   // (b x G): the reference's s x d order, Kernels.h:128-130
-          ux = gz * sb[k][1] - gy * sb[k][2];
-          uy = gx * sb[k][2] - gz * sb[k][0];
-          uz = gy * sb[k][0] - gx * sb[k][1];
-        } else {
-          ux = gx; uy = gy; uz = gz;
-        }
-        col[0] = ux * tb[0][0] + uy * tb[0][1] + uz * tb[0][2];
-        col[1] = ux * tb[1][0] + uy * tb[1][1] + uz * tb[1][2];
-        if (nunk == 3) col[2] = ux * tb[2][0] + uy * tb[2][1] + uz * tb[2][2];
+        col[0] = cv[k][0];
+        col[1] = cv[k][1];
+        if (nunk == 3) col[2] = cv[k][2];
         col += R;
       }
     }
@@ -568,6 +604,7 @@ int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
   fa.srec = srec; fa.trec = trec; fa.ns = ns; fa.nt = nt;
   fa.nunk = nunk; fa.same = same ? 1 : 0; fa.maxlev = maxlev;
   fa.coeffs = coeffs; fa.count = cnt;
+  fa.vec4 = ((uintptr_t)coeffs % 16 == 0) && ((int64_t)nunk * nt) % 4 == 0;
   // about 8 workgroups per CU: target blocks x source chunks of whole LDS tiles
   const int64_t nbx = (nt + BLOCK - 1) / BLOCK;
   const int64_t ntiles = (ns + BM_TILE - 1) / BM_TILE;
