@@ -577,14 +577,24 @@ int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
   if (ns == 0 || nt == 0) return O3D_OK;
   if (!coeffs) return fail(O3D_EINVAL, "coeffs is NULL");
   if ((rc = ensure_init()) != O3D_OK) return rc;
+  // the pair list is sized from a count read back to the host between the passes
+  if (stream_capturing(st))
+    return fail(O3D_EINVAL, "the BEM matrix reads its pair count back between passes and cannot be captured");
   const size_t recb = ((sizeof(float4) * BM_F4 * (size_t)(ns + nt)) + 255) & ~(size_t)255;
   std::lock_guard<std::mutex> lk(stream_ws_mutex());
   void* ws = nullptr;
   if ((rc = stream_workspace(st, recb + 256, &ws)) != O3D_OK) return rc;
   // ev: start, counted, one-leaf pass done, recursed pairs done
-  hipEvent_t ev[4] = {};
+  struct Events {
+    hipEvent_t e[4] = {};
+    ~Events() {
+      for (auto x : e)
+        if (x) (void)hipEventDestroy(x);
+    }
+  } evs;
+  hipEvent_t* ev = evs.e;
   if (prof) {
-    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    for (int k = 0; k < 4; ++k) HIPCHK(hipEventCreate(&ev[k]));
     HIPCHK(hipEventRecord(ev[0], st));
   }
   auto layout = [&](void* base, float4** srec, float4** trec, unsigned long long** cnt, uint2** list) {
@@ -655,7 +665,6 @@ int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
     prof->near_ms = c;
     prof->total_ms = (double)a + b + c;
     prof->near_pairs = (double)nnear;
-    for (auto& e : ev) (void)hipEventDestroy(e);
   }
   return O3D_OK;
 }
@@ -690,6 +699,8 @@ int o3d_bem_matrix(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, 
     return fail(O3D_EINVAL, "unknowns per panel %d: must be 2 or 3 (Surfaces.h:261)", nunk);
   if (src->np == 0 || targ->np == 0) return O3D_OK;
   if (!coeffs) return fail(O3D_EINVAL, "coeffs is NULL");
+  if ((rc = check_node_indices(src, "src")) != O3D_OK) return rc;
+  if ((rc = check_node_indices(targ, "targ")) != O3D_OK) return rc;
   if ((rc = ensure_init()) != O3D_OK) return rc;
   const size_t nel = (size_t)nunk * nunk * (size_t)src->np * (size_t)targ->np;
   // device copies of both surfaces (nodes 3*nn floats, areas np floats, indices

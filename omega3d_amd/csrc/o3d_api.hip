@@ -193,6 +193,15 @@ int check_opts(const o3d_opts* o) {
   return O3D_OK;
 }
 
+int check_node_indices(const o3d_panels* p, const char* what) {
+  const uint32_t nn = (uint32_t)p->nn;
+  for (int64_t i = 0; i < 3 * p->np; ++i)
+    if (p->idx[i] >= nn)
+      return fail(O3D_EINVAL, "%s: panel %lld names node %u of %lld", what, (long long)(i / 3), p->idx[i],
+                  (long long)p->nn);
+  return O3D_OK;
+}
+
 template <class P>
 int check_points(const P* p, bool as_src, bool as_tgt, bool grad, const char* what) {
   if (!p) return fail(O3D_EINVAL, "%s is NULL", what);

@@ -26,6 +26,8 @@ void choose_config(const Tuning& req, int64_t nt, int64_t ns, bool grad, int pre
 hipError_t dispatch_p2p(int core, const P2PArgs& a, bool thick, bool grad, int prec, int tpt, hipStream_t st);
 double flops_of(int core, bool grad, bool thick, int64_t nt, int64_t ns);
 int check_opts(const o3d_opts* o);
+// host-memory panels: every node index below nn (device panels are the caller's to keep in range)
+int check_node_indices(const o3d_panels* p, const char* what);
 void fill_stats(o3d_stats* stats, double secs, double flops, double inter);
 bool ff_supported(int core, bool grad, int prec);
 int p2p_factored(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st,

@@ -144,6 +144,17 @@ __device__ __forceinline__ void grad_of(const R* a, R* g) {
 //   f4 = {cx, cy, cz, 0} (root centroid)   f5 = {tsx, tsy, tsz, ss} (raw inputs)
 // The reference passes the sheet strength ts/area and multiplies it back by the
 // area at level 0 (Influence.h:737, Kernels.h:1136-1146); that round trip is kept.
+// squared-distance bands of the root decision (the reference's float test
+// dist > 4 sqrt(area)): d2 > far2 is surely a leaf, d2 < near2 surely not.
+// The bands are 1e-6 of the distance either side of 4 sqrt(area), about five
+// times the rounding of the fast d2 and of the reference's distance together.
+__device__ __forceinline__ void root_bands(float sqa, float& far2, float& near2) {
+  const float lim = sqa * 4.0f;
+  const float lh = lim * 1.000001f, ll = lim * 0.999999f;
+  far2 = lh * lh;
+  near2 = ll * ll;
+}
+
 __device__ __forceinline__ void panel_record(const PanArgs& a, int64_t p, float4* s) {
   {
     const uint32_t i0 = a.idx[3 * p], i1 = a.idx[3 * p + 1], i2 = a.idx[3 * p + 2];
@@ -162,8 +173,10 @@ __device__ __forceinline__ void panel_record(const PanArgs& a, int64_t p, float4
     s[0] = make_float4(x0, y0, z0, x1);
     s[1] = make_float4(y1, z1, x2, y2);
     s[2] = make_float4(z2, gx, gy, gz);
-    s[3] = make_float4(gs, ar, sqrtf(ar), 0.0f);
-    s[4] = make_float4(cx, cy, cz, 0.0f);
+    float far2, near2;
+    root_bands(sqrtf(ar), far2, near2);
+    s[3] = make_float4(gs, ar, sqrtf(ar), far2);
+    s[4] = make_float4(cx, cy, cz, near2);
     s[5] = make_float4(tx, ty, tz, sg);
   }
 }
@@ -179,17 +192,22 @@ static __global__ __launch_bounds__(BLOCK) void k_pan_geom(PanArgs a, float4* re
 }
 
 // a panel's root quantities: level-0 absolute strengths (Kernels.h:1136-1146),
-// root centroid and 4 sqrt(area)
+// root centroid and 4 sqrt(area) with its squared bands (FP32 only)
+struct RootLim {
+  float lim, far2, near2;
+};
 template <class R>
 __device__ __forceinline__ void panel_root(const float4* s, R& gx, R& gy, R& gz, R& gs, R& cx0, R& cy0, R& cz0,
-                                           R& lim) {
+                                           R& lim, RootLim& rl) {
   const float4 f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
   const R ar = f3.y;
   if constexpr (sizeof(R) == 4) {
     gx = f2.y; gy = f2.z; gz = f2.w; gs = f3.x;
     cx0 = f4.x; cy0 = f4.y; cz0 = f4.z;
     lim = f3.z * 4.0f;
+    rl = RootLim{lim, f3.w, f4.w};
   } else {
+    rl = RootLim{0.0f, 0.0f, 0.0f};
     const float4 f5 = s[5];
     {
 #pragma clang fp contract(off)
@@ -201,22 +219,26 @@ __device__ __forceinline__ void panel_root(const float4* s, R& gx, R& gy, R& gz,
 }
 
 // the root decision of one (target, panel) pair: the reference's float test
-// (dist > 4 sqrt(area)), taken from a fast distance estimate when unambiguous and
-// from the exact expression otherwise; every caller takes it through here, so
-// the far and near kernels agree on it
+// (dist > 4 sqrt(area)), taken from the fused d2 against the bands of root_bands
+// when unambiguous and from the exact expression otherwise (both start from the
+// same float differences); every caller takes it through here, so the far and
+// near kernels agree on it.  The empty volatile asm keeps the exact expression
+// behind its branch: without it the compiler evaluates the IEEE square root for
+// every pair and selects.
 template <class R>
-__device__ __forceinline__ bool root_leaf(R tx, R ty, R tz, R cx0, R cy0, R cz0, R lim, int maxlev, bool active) {
+__device__ __forceinline__ bool root_leaf(R tx, R ty, R tz, R cx0, R cy0, R cz0, R lim, const RootLim& rl,
+                                          int maxlev, bool active) {
   bool leaf0;
   if constexpr (sizeof(R) == 4) {
     const float ex = tx - cx0, ey = ty - cy0, ez = tz - cz0;
     const float d2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-    const float dfast = d2 * frsq(d2);
-    const bool sure_far = dfast > lim * 1.0000010f;
-    const bool sure_near = dfast < lim * 0.9999990f;
+    const bool sure_far = d2 > rl.far2;
+    const bool sure_near = d2 < rl.near2;
     leaf0 = sure_far;
     const bool unsure = !sure_far && !sure_near;
     if (wave_any(unsure && active)) {
-      if (unsure) leaf0 = dist_exact<float>(tx, ty, tz, cx0, cy0, cz0) > lim;
+      asm volatile("" ::: "memory");
+      if (unsure) leaf0 = dist_exact<float>(tx, ty, tz, cx0, cy0, cz0) > rl.lim;
     }
   } else {
     leaf0 = dist_exact<double>(tx, ty, tz, cx0, cy0, cz0) > lim;
@@ -230,8 +252,9 @@ __device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, in
   const float4 f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3];
   const R ar = f3.y;
   R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-  panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
-  const bool leaf0 = root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, maxlev, active);
+  RootLim rl;
+  panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
+  const bool leaf0 = root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, rl, maxlev, active);
   if (!wave_any(!leaf0 && active)) {
     if (active) leaf_acc<R, GRAD>(tx, ty, tz, cx0, cy0, cz0, gx, gy, gz, gs, acc);
     return;
@@ -300,8 +323,9 @@ __global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
     __syncthreads();
     for (int j = 0; j < cnt; ++j) {
       R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-      panel_root<R>(shp + PAN_F4 * j, gx, gy, gz, gs, cx0, cy0, cz0, lim);
-      if (root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, a.maxlev, active)) {
+      RootLim rl;
+      panel_root<R>(shp + PAN_F4 * j, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
+      if (root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, rl, a.maxlev, active)) {
         if (active) leaf_acc<R, GRAD>(tx, ty, tz, cx0, cy0, cz0, gx, gy, gz, gs, acc);
       } else {
         has_near = true;
@@ -375,8 +399,9 @@ __global__ __launch_bounds__(BLOCK) void k_pan_near(PanArgs a, const float4* rec
     for (int64_t p = lane; p < a.np; p += 64) {
       const float4* s = rec + PAN_F4 * p;
       R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-      panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
-      if (!root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, a.maxlev, true))
+      RootLim rl;
+      panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
+      if (!root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, rl, a.maxlev, true))
         walk_panel<R, GRAD>(tx, ty, tz, s, a.maxlev, true, acc);
     }
     double u[3], g[9];

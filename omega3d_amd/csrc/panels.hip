@@ -202,6 +202,7 @@ int o3d_panels_affect_points(const o3d_panels* src, o3d_points* targ, const o3d_
   const bool grad = o->results == O3D_VELANDGRAD;
   if ((rc = check_panels(src)) != O3D_OK) return rc;
   if ((rc = check_targets(targ, grad)) != O3D_OK) return rc;
+  if (src->np > 0 && (rc = check_node_indices(src, "src")) != O3D_OK) return rc;
   int ml;
   if ((rc = maxlev_of(o, &ml)) != O3D_OK) return rc;
   if ((rc = ensure_init()) != O3D_OK) return rc;

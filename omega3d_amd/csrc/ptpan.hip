@@ -57,8 +57,10 @@ __device__ __forceinline__ void ptpan_record(const PtPanArgs& a, int64_t i, floa
   s[0] = make_float4(x0, y0, z0, x1);
   s[1] = make_float4(y1, z1, x2, y2);
   s[2] = make_float4(z2, 0.0f, 0.0f, 0.0f);
-  s[3] = make_float4(0.0f, ar, sqrtf(ar), 0.0f);
-  s[4] = make_float4(cx, cy, cz, 0.0f);
+  float far2, near2;
+  root_bands(sqrtf(ar), far2, near2);
+  s[3] = make_float4(0.0f, ar, sqrtf(ar), far2);
+  s[4] = make_float4(cx, cy, cz, near2);
   s[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -94,7 +96,8 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
   float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
   bool has_near = false;
   R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-  panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
+  RootLim rl;
+  panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
   const int64_t q0 = (int64_t)blockIdx.y * a.slen;
   const int64_t q1 = q0 + a.slen < a.ns ? q0 + a.slen : a.ns;
 
@@ -109,7 +112,7 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
     __syncthreads();
     for (int j = 0; j < cnt; ++j) {
       const float4 P = shs[2 * j], A = shs[2 * j + 1];
-      if (root_leaf<R>(P.x, P.y, P.z, cx0, cy0, cz0, lim, a.maxlev, active)) {
+      if (root_leaf<R>(P.x, P.y, P.z, cx0, cy0, cz0, lim, rl, a.maxlev, active)) {
         if (!active) continue;
         R ax, ay, az;
         if (PREC == PREC_FP64) {
@@ -173,8 +176,8 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan_combine(PtPanArgs a) {
 // The near pairs of the listed panels, after k_ptpan_combine.  One wave per 4
 // listed panels, their records in LDS (a wave reads only its own, and LDS
 // operations of one wave complete in order): lane l takes particles l, l + 64, ...,
-// loads each once and tests it against the 4 panels -- a squared-distance test
-// with a 1% margin passes only candidates to the exact root decision -- and walks
+// loads each once and tests it against the 4 panels -- a particle beyond a panel's
+// far band is a leaf at the root, the rest go to the root decision -- and walks
 // the pairs that need the recursion (Kernels.h:1028-1211).  A fixed butterfly
 // adds the lanes' sums in FP64, subtracted from each panel's velocity.
 constexpr int PTPAN_GROUP = 4;
@@ -205,8 +208,9 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan_near(PtPanArgs a) {
     for (int k = 0; k < PTPAN_GROUP; ++k) {
       const float4 c = rw[k * PAN_F4 + 4], f3 = rw[k * PAN_F4 + 3];
       fx[k] = c.x; fy[k] = c.y; fz[k] = c.z;
-      // an empty slot of the last group: a reach no particle has
-      reach2[k] = k < ng ? (f3.z * 4.0f) * (f3.z * 4.0f) * 1.01f : -1.0f;
+      // beyond the far band (the same fused d2 as root_leaf's) a pair is surely a
+      // leaf at the root; an empty slot of the last group: a reach no particle has
+      reach2[k] = k < ng ? f3.w : -1.0f;
     }
     R acc[PTPAN_GROUP][3];
 #pragma unroll
@@ -221,8 +225,9 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan_near(PtPanArgs a) {
 #pragma unroll
         for (int q = 0; q < PAN_F4; ++q) s[q] = rw[k * PAN_F4 + q];
         R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-        panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim);
-        if (root_leaf<R>(px, py, pz, cx0, cy0, cz0, lim, a.maxlev, true)) continue;
+        RootLim rl;
+        panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
+        if (root_leaf<R>(px, py, pz, cx0, cy0, cz0, lim, rl, a.maxlev, true)) continue;
         ptpan_strength<PREC>(s, make_float4(a.ax[p], a.ay[p], a.az[p], 0.0f));
         R t16[16];
 #pragma unroll
@@ -457,6 +462,7 @@ int o3d_points_affect_panels(const o3d_points* src, const o3d_panels* targ, floa
   int rc;
   if ((rc = check_opts(o)) != O3D_OK) return rc;
   if ((rc = check_ptpan(src, targ, vel, o)) != O3D_OK) return rc;
+  if (targ->np > 0 && (rc = check_node_indices(targ, "targ")) != O3D_OK) return rc;
   if ((rc = ensure_init()) != O3D_OK) return rc;
   const auto t0 = std::chrono::steady_clock::now();
   const int64_t ns = src->n, np = targ->np, nn = targ->nn;

@@ -108,3 +108,32 @@ def test_no_device_is_reported_not_crashed():
     # empty collections are a no-op that still validates, and need no device
     s0, k3 = _pts(0)
     assert _lib.lib.o3d_points_affect_points(C.byref(s0), C.byref(t), C.byref(o), None) in (0, 8)
+
+
+def test_out_of_range_node_index_is_rejected_before_any_launch():
+    """Host-memory panels are checked on the host: a node index at or beyond the
+    node count is a status code, not a device access out of bounds."""
+    from omega3d_amd import _lib
+    L = _lib.lib
+    nodes = np.ascontiguousarray(np.eye(3, 4, dtype=np.float32))
+    idx = np.array([[0, 1, 2], [1, 2, 4]], np.uint32)            # node 4 of 4
+    area = np.ones(2, np.float32)
+    ts = np.zeros((3, 2), np.float32)
+    p = _lib.Panels()
+    p.np, p.nn = 2, 4
+    for d in range(3):
+        p.x[d] = nodes[d].ctypes.data
+        p.ts[d] = ts[d].ctypes.data
+    p.idx, p.area = idx.ctypes.data, area.ctypes.data
+    t, keep = _pts(5)
+    o = _lib.make_opts("rm")
+    assert L.o3d_panels_affect_points(C.byref(p), C.byref(t), C.byref(o), None) == 1
+    assert "panel 1 names node 4 of 4" in _lib.last_error()
+    vel = np.zeros((3, 2), np.float32)
+    pv = (C.c_void_p * 3)(*[vel[d].ctypes.data for d in range(3)])
+    ov = _lib.make_opts("rm", False)
+    assert L.o3d_points_affect_panels(C.byref(t), C.byref(p), pv, C.byref(ov), None) == 1
+    assert "names node 4" in _lib.last_error()
+    m = np.zeros(2 * 2 * 2 * 2, np.float32)
+    assert L.o3d_bem_matrix(C.byref(p), C.byref(p), 2, 1, 3, m.ctypes.data_as(C.POINTER(C.c_float))) == 1
+    assert "names node 4" in _lib.last_error()

@@ -115,3 +115,30 @@ def test_capture_on_a_fresh_stream_falls_back_to_one_pass(env):
         assert rel_errors(v1, v0)[0] <= 2e-6 and rel_errors(g1, g0)[0] <= 2e-6
     finally:
         stream_destroy(st)
+
+
+def test_bem_matrix_refuses_capture(env):
+    """The BEM matrix sizes its pair list from a count read back between its two
+    passes, so under capture it returns a status before it enqueues anything and
+    the capture ends cleanly."""
+    _lib, hip = env
+    from omega3d_amd.batch import stream_create, stream_destroy
+    st = stream_create()
+    try:
+        D = _lib.DeviceArray.from_host
+        keep = [D(np.eye(3, dtype=np.float32)), D(np.zeros(3, np.float32)), D(np.ones(1, np.float32))]
+        p = _lib.Panels()
+        p.np, p.nn = 1, 3
+        for d in range(3):
+            p.x[d] = keep[0].row(d)
+        p.idx, p.area = keep[1].ptr, keep[2].ptr
+        out = _lib.DeviceArray((4,))
+        assert hip.hipStreamBeginCapture(C.c_void_p(st), 0) == 0
+        rc = _lib.lib.o3d_bem_matrix_dev(C.byref(p), C.byref(p), 2, 1, 3, C.c_void_p(out.ptr), C.c_void_p(st))
+        graph = C.c_void_p()
+        end = hip.hipStreamEndCapture(C.c_void_p(st), C.byref(graph))
+        assert rc == 1 and "cannot be captured" in _lib.last_error()
+        assert end == 0
+        hip.hipGraphDestroy(graph)
+    finally:
+        stream_destroy(st)
