@@ -62,6 +62,13 @@ __device__ __forceinline__ double wave_sum(double x) {
   return x;
 }
 
+// sum over each group of 16 lanes (lanes 16q .. 16q+15) by a fixed butterfly
+__device__ __forceinline__ double quarter_sum(double x) {
+#pragma unroll
+  for (int o = 8; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
 // error-free transformation a + b = s + e (Knuth's TwoSum)
 __device__ __forceinline__ void two_sum(float& hi, float& lo, float b) {
   const float s = hi + b;

@@ -185,10 +185,14 @@ __device__ __forceinline__ void stage_panels(const PanArgs& a, int64_t p0, int c
   for (int j = threadIdx.x; j < cnt; j += BLOCK) panel_record(a, p0 + j, sh + PAN_F4 * j);
 }
 
-// the records of all panels in global memory, for the near kernel
-static __global__ __launch_bounds__(BLOCK) void k_pan_geom(PanArgs a, float4* rec) {
+// the records of all panels in global memory, for the near kernel, and (cen, when
+// given) each panel's root centroid with its far band, the near kernel's prefilter
+static __global__ __launch_bounds__(BLOCK) void k_pan_geom(PanArgs a, float4* rec, float4* cen) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (p < a.np) panel_record(a, p, rec + PAN_F4 * p);
+  if (p >= a.np) return;
+  float4* s = rec + PAN_F4 * p;
+  panel_record(a, p, s);
+  if (cen) cen[p] = make_float4(s[4].x, s[4].y, s[4].z, s[3].w);
 }
 
 // a panel's root quantities: level-0 absolute strengths (Kernels.h:1136-1146),
@@ -378,42 +382,69 @@ __global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
 }
 
 // The pairs of the listed targets that the root decision sends into the
-// recursion, added to the outputs after k_pan.  One wave per listed target: lane l
-// takes panels l, l + 64, ... from the records, decides each root pair as k_pan
-// did and walks the near ones (Kernels.h:1028-1211); a fixed butterfly adds the
-// lanes' sums in FP64, so results do not depend on scheduling.  Grid-stride over
-// the list, whose length k_pan left in near_count.
+// recursion, added to the outputs after k_pan.  One wave per 4 listed targets, one
+// per 16-lane quarter: lane l of a quarter takes panels l, l + 16, ..., so the
+// quarters load the same panels in the same instruction.  A panel whose centroid
+// lies beyond its far band (cen: the fused d2 of root_leaf against the same band)
+// is a leaf at the root, which k_pan evaluated; the others are decided as k_pan
+// did and the near ones walked (Kernels.h:1028-1211).  A fixed butterfly adds each
+// quarter's sums in FP64, so results do not depend on scheduling.  Grid-stride
+// over the list, whose length k_pan left in near_count.
 template <bool GRAD, int PREC>
-__global__ __launch_bounds__(BLOCK) void k_pan_near(PanArgs a, const float4* rec) {
+__global__ __launch_bounds__(BLOCK) void k_pan_near(PanArgs a, const float4* rec, const float4* cen) {
   using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
   const int lane = threadIdx.x & 63;
+  const int ql = lane & 15;
   const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
   const int64_t listed = *a.near_count;
-  for (int64_t e = wave; e < listed; e += nwaves) {
-    const int64_t i = a.near_list[e];
+  const int64_t groups = (listed + 3) / 4;
+  for (int64_t e = wave; e < groups; e += nwaves) {
+    const int64_t li = 4 * e + (lane >> 4);
+    const bool has = li < listed;
+    const int64_t i = a.near_list[has ? li : 4 * e];
     const R tx = a.tx[i], ty = a.ty[i], tz = a.tz[i];
     R acc[16];
 #pragma unroll
     for (int c = 0; c < 16; ++c) acc[c] = R(0);
-    for (int64_t p = lane; p < a.np; p += 64) {
-      const float4* s = rec + PAN_F4 * p;
-      R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-      RootLim rl;
-      panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
-      if (!root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, rl, a.maxlev, true))
-        walk_panel<R, GRAD>(tx, ty, tz, s, a.maxlev, true, acc);
+    if (has) {
+      // four panels per step: their prefilter loads are issued together, then the
+      // candidates are decided and walked in ascending order
+      for (int64_t p = ql; p < a.np; p += 64) {
+        unsigned m = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const int64_t pk = p + 16 * k;
+          if (pk >= a.np) continue;
+          if constexpr (sizeof(R) == 4) {
+            const float4 c = cen[pk];
+            const float ex = tx - c.x, ey = ty - c.y, ez = tz - c.z;
+            if (fmaf(ez, ez, fmaf(ey, ey, ex * ex)) > c.w) continue;
+          }
+          m |= 1u << k;
+        }
+        while (m) {
+          const int k = __builtin_ctz(m);
+          m &= m - 1;
+          const float4* s = rec + PAN_F4 * (p + 16 * k);
+          R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+          RootLim rl;
+          panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
+          if (!root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, rl, a.maxlev, true))
+            walk_panel<R, GRAD>(tx, ty, tz, s, a.maxlev, true, acc);
+        }
+      }
     }
     double u[3], g[9];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) u[d] = wave_sum((double)acc[d]);
+    for (int d = 0; d < 3; ++d) u[d] = quarter_sum((double)acc[d]);
     if (GRAD) {
       R gl[9];
       grad_of<R>(acc, gl);
 #pragma unroll
-      for (int k = 0; k < 9; ++k) g[k] = wave_sum((double)gl[k]);
+      for (int k = 0; k < 9; ++k) g[k] = quarter_sum((double)gl[k]);
     }
-    if (lane != 0) continue;
+    if (ql != 0 || !has) continue;
     if (PREC == PREC_FP64) {
       double* uu = (double*)a.u; double* vv = (double*)a.v; double* ww = (double*)a.w;
       uu[i] += u[0]; vv[i] += u[1]; ww[i] += u[2];

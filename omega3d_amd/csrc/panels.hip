@@ -20,23 +20,25 @@ __global__ void k_ff_zero_u64_pan(unsigned long long* p) {
 
 // root pairs (k_pan), then the near pairs of the targets it listed (k_pan_near);
 // a.near_list / a.near_count must point at scratch for nt entries and a counter
-// (zeroed here), rec at np panel records
+// (zeroed here), rec at np panel records followed by np centroids
 static hipError_t launch_pan(const PanArgs& a, bool grad, int prec, hipStream_t st, float4* rec) {
   if (a.nt <= 0 || a.np <= 0) return hipSuccess;
   if (!a.near_list || !a.near_count || !rec) return hipErrorInvalidValue;
   const size_t lds = (size_t)a.tile * PAN_F4 * sizeof(float4);
   const dim3 grid((unsigned)((a.nt + BLOCK - 1) / BLOCK));
-  // one wave per listed target, grid-stride: at most 16 waves per CU
-  const dim3 grid_near((unsigned)std::min<int64_t>((a.nt + 3) / 4, 4 * (int64_t)num_cus()));
+  // one wave per 4 listed targets, grid-stride: at most 16 waves per CU
+  const dim3 grid_near((unsigned)std::min<int64_t>((a.nt + 15) / 16, 4 * (int64_t)num_cus()));
+  float4* cen = rec + PAN_F4 * a.np;
   hipLaunchKernelGGL(k_pan_g7_zero, dim3(1), dim3(64), 0, st, a.near_count);
-  hipLaunchKernelGGL(k_pan_geom, dim3((unsigned)((a.np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, a, rec);
+  hipLaunchKernelGGL(k_pan_geom, dim3((unsigned)((a.np + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, st, a, rec, cen);
   hipError_t e = hipErrorInvalidValue;
 #define O3D_PL(G, P)                                                                      \
   if (grad == G && prec == P) {                                                           \
     hipLaunchKernelGGL((k_pan<G, P>), grid, dim3(BLOCK), lds, st, a);                     \
     e = hipGetLastError();                                                                \
     if (e == hipSuccess) {                                                                \
-      hipLaunchKernelGGL((k_pan_near<G, P>), grid_near, dim3(BLOCK), 0, st, a, (const float4*)rec); \
+      hipLaunchKernelGGL((k_pan_near<G, P>), grid_near, dim3(BLOCK), 0, st, a, (const float4*)rec, \
+                         (const float4*)cen);                                             \
       e = hipGetLastError();                                                              \
     }                                                                                     \
   }
@@ -180,7 +182,7 @@ static int pan_dev(const o3d_panels* p, o3d_points* t, const o3d_opts* o, int pr
   }
   std::lock_guard<std::mutex> lk(stream_ws_mutex());
   float4* rec = nullptr;
-  if ((rc = g7_scratch(&a, st, &rec, PAN_F4)) != O3D_OK) return rc;
+  if ((rc = g7_scratch(&a, st, &rec, PAN_F4 + 1)) != O3D_OK) return rc;
   const hipError_t e = launch_pan(a, grad, prec, st, rec);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "panel launch failed: %s", hipGetErrorString(e));
   return O3D_OK;

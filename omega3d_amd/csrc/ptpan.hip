@@ -13,10 +13,11 @@
 // workgroup row, staged per tile in LDS and broadcast (a boundary has too few
 // panels to fill the GPU otherwise); it takes the root decision of every pair and
 // evaluates the one-leaf pairs.  k_ptpan_combine adds the slices in order and
-// lists the panels that have particles near enough for the recursion.
-// k_ptpan_near gives each group of 4 listed panels one wave whose lanes walk
-// those pairs with the depth-first tree walk of panels_affect_points
-// (pan_kernels.hpp), so the subdivision decisions are the reference's.
+// lists the panels that have particles near enough for the recursion; k_ptpan
+// also keeps the first few such particles of each (slice, panel) in ascending
+// order.  k_ptpan_near gives each listed panel one wave whose lanes walk those
+// pairs with the depth-first tree walk of panels_affect_points (pan_kernels.hpp),
+// so the subdivision decisions are the reference's.
 #include "o3d_api_internal.hpp"
 #include "pan_kernels.hpp"
 
@@ -39,10 +40,16 @@ struct PtPanArgs {
   int32_t maxlev;
   int32_t* near_list;          // panels with a pair that needs the recursion
   unsigned int* near_count;
-  double* part;                // per slice: 3 sums and a near flag per panel
+  double* part;                // per slice: 3 sums per panel
+  int32_t* cnt;                // [panel][slice]: particles that need the recursion
+  int32_t* ent;                // [panel][slice][PTPAN_CAP]: the first of them, ascending
   int64_t slen;                // particles per slice
   int32_t nslice;
 };
+
+// particles kept per (slice, panel) for the near kernel; a slice with more is
+// scanned again there
+constexpr int PTPAN_CAP = 8;
 
 // this lane's panel in the staged-panel layout of pan_kernels.hpp (the strength
 // fields are filled per particle)
@@ -78,8 +85,9 @@ __device__ __forceinline__ void ptpan_strength(float4* s, float4 A) {
 // One lane per target panel, particles [y*slen, (y+1)*slen) of slice y =
 // blockIdx.y staged per tile in LDS: the root decision of every (panel, particle)
 // pair -- the reference's, through root_leaf -- and the pairs that are leaves at
-// the root.  Each slice leaves its sum (FP64) and whether a pair needs the
-// recursion in part[y][0..3][panel]; k_ptpan_combine adds the slices in order.
+// the root.  Each slice leaves its sum (FP64) in part[y][0..2][panel], the count
+// of pairs that need the recursion in cnt and the first PTPAN_CAP of their
+// particles in ent; k_ptpan_combine adds the slices in order.
 template <int PREC>
 __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
   using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
@@ -94,7 +102,8 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
 #pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = R(0);
   float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
-  bool has_near = false;
+  int nnear = 0;
+  int32_t* ent = a.ent + ((int64_t)ic * a.nslice + blockIdx.y) * PTPAN_CAP;
   R gx, gy, gz, gs, cx0, cy0, cz0, lim;
   RootLim rl;
   panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
@@ -122,8 +131,9 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
           ax = A.x; ay = A.y; az = A.z;
         }
         leaf_acc<R, false>(P.x, P.y, P.z, cx0, cy0, cz0, ax, ay, az, R(0), acc);
-      } else {
-        has_near = true;
+      } else if (active) {
+        if (nnear < PTPAN_CAP) ent[nnear] = (int32_t)(p0 + j);
+        ++nnear;
       }
     }
     if (PREC == PREC_COMP) {
@@ -134,7 +144,7 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
     }
   }
   if (!active) return;
-  double* pt = a.part + (int64_t)blockIdx.y * 4 * a.np + i;
+  double* pt = a.part + (int64_t)blockIdx.y * 3 * a.np + i;
   if (PREC == PREC_COMP) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) pt[d * a.np] = (double)hi[d] + (double)lo[d];
@@ -142,7 +152,7 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) pt[d * a.np] = (double)acc[d];
   }
-  pt[3 * a.np] = has_near ? 1.0 : 0.0;
+  a.cnt[i * a.nslice + blockIdx.y] = nnear;
 }
 
 // per panel: the slices' sums added in slice order and subtracted from the
@@ -155,10 +165,10 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan_combine(PtPanArgs a) {
   double u[3] = {0.0, 0.0, 0.0};
   bool near = false;
   for (int y = 0; y < a.nslice; ++y) {
-    const double* pt = a.part + (int64_t)y * 4 * a.np + i;
+    const double* pt = a.part + (int64_t)y * 3 * a.np + i;
 #pragma unroll
     for (int d = 0; d < 3; ++d) u[d] += pt[d * a.np];
-    near = near || pt[3 * a.np] != 0.0;
+    near = near || a.cnt[i * a.nslice + y] != 0;
   }
   if (near) a.near_list[atomicAdd(a.near_count, 1u)] = (int32_t)i;
   if (PREC == PREC_FP64) {
@@ -173,86 +183,71 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan_combine(PtPanArgs a) {
   }
 }
 
-// The near pairs of the listed panels, after k_ptpan_combine.  One wave per 4
-// listed panels, their records in LDS (a wave reads only its own, and LDS
-// operations of one wave complete in order): lane l takes particles l, l + 64, ...,
-// loads each once and tests it against the 4 panels -- a particle beyond a panel's
-// far band is a leaf at the root, the rest go to the root decision -- and walks
-// the pairs that need the recursion (Kernels.h:1028-1211).  A fixed butterfly
-// adds the lanes' sums in FP64, subtracted from each panel's velocity.
-constexpr int PTPAN_GROUP = 4;
-
+// The near pairs of the listed panels, after k_ptpan_combine.  One wave per listed
+// panel: lane l takes slices l, l + 64, ... and walks the particles k_ptpan kept
+// for its slice, in ascending order; a slice with more than PTPAN_CAP such
+// particles is scanned again by the whole wave (lane l takes its particles l,
+// l + 64, ..., decides each root pair as k_ptpan did).  The walk is the depth-first
+// tree walk of panels_affect_points (Kernels.h:1028-1211).  A fixed butterfly
+// adds the lanes' sums in FP64, subtracted from the panel's velocity.
 template <int PREC>
 __global__ __launch_bounds__(BLOCK) void k_ptpan_near(PtPanArgs a) {
   using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
-  __shared__ float4 recs[(BLOCK / 64) * PTPAN_GROUP * PAN_F4];
   const int lane = threadIdx.x & 63;
-  float4* rw = recs + (threadIdx.x >> 6) * PTPAN_GROUP * PAN_F4;
   const int64_t wave = ((int64_t)blockIdx.x * BLOCK + threadIdx.x) >> 6;
   const int64_t nwaves = ((int64_t)gridDim.x * BLOCK) >> 6;
   const int64_t listed = *a.near_count;
-  const int64_t groups = (listed + PTPAN_GROUP - 1) / PTPAN_GROUP;
-  for (int64_t e = wave; e < groups; e += nwaves) {
-    const int64_t e0 = e * PTPAN_GROUP;
-    const int ng = (int)(listed - e0 < PTPAN_GROUP ? listed - e0 : PTPAN_GROUP);
-    __builtin_amdgcn_wave_barrier();
-    if (lane < ng) {
-      float4 s[PAN_F4];
-      ptpan_record(a, a.near_list[e0 + lane], s);
-#pragma unroll
-      for (int q = 0; q < PAN_F4; ++q) rw[lane * PAN_F4 + q] = s[q];
-    }
-    __builtin_amdgcn_wave_barrier();
-    float fx[PTPAN_GROUP], fy[PTPAN_GROUP], fz[PTPAN_GROUP], reach2[PTPAN_GROUP];
-#pragma unroll
-    for (int k = 0; k < PTPAN_GROUP; ++k) {
-      const float4 c = rw[k * PAN_F4 + 4], f3 = rw[k * PAN_F4 + 3];
-      fx[k] = c.x; fy[k] = c.y; fz[k] = c.z;
-      // beyond the far band (the same fused d2 as root_leaf's) a pair is surely a
-      // leaf at the root; an empty slot of the last group: a reach no particle has
-      reach2[k] = k < ng ? f3.w : -1.0f;
-    }
-    R acc[PTPAN_GROUP][3];
-#pragma unroll
-    for (int k = 0; k < PTPAN_GROUP; ++k) acc[k][0] = acc[k][1] = acc[k][2] = R(0);
-    for (int64_t p = lane; p < a.ns; p += 64) {
+  for (int64_t e = wave; e < listed; e += nwaves) {
+    const int64_t i = a.near_list[e];
+    float4 s[PAN_F4];
+    ptpan_record(a, i, s);
+    R acc[3] = {R(0), R(0), R(0)};
+    // one walk of particle p on this panel, added to acc
+    auto walk = [&](int64_t p) {
       const float px = a.sx[p], py = a.sy[p], pz = a.sz[p];
+      ptpan_strength<PREC>(s, make_float4(a.ax[p], a.ay[p], a.az[p], 0.0f));
+      R t16[16];
 #pragma unroll
-      for (int k = 0; k < PTPAN_GROUP; ++k) {
-        const float ex = px - fx[k], ey = py - fy[k], ez = pz - fz[k];
-        if (!(fmaf(ez, ez, fmaf(ey, ey, ex * ex)) <= reach2[k])) continue;
-        float4 s[PAN_F4];
-#pragma unroll
-        for (int q = 0; q < PAN_F4; ++q) s[q] = rw[k * PAN_F4 + q];
-        R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-        RootLim rl;
-        panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
-        if (root_leaf<R>(px, py, pz, cx0, cy0, cz0, lim, rl, a.maxlev, true)) continue;
-        ptpan_strength<PREC>(s, make_float4(a.ax[p], a.ay[p], a.az[p], 0.0f));
-        R t16[16];
-#pragma unroll
-        for (int c = 0; c < 16; ++c) t16[c] = R(0);
-        walk_panel<R, false>(px, py, pz, s, a.maxlev, true, t16);
-        acc[k][0] += t16[0]; acc[k][1] += t16[1]; acc[k][2] += t16[2];
+      for (int c = 0; c < 16; ++c) t16[c] = R(0);
+      walk_panel<R, false>(px, py, pz, s, a.maxlev, true, t16);
+      acc[0] += t16[0]; acc[1] += t16[1]; acc[2] += t16[2];
+    };
+    const int32_t* cnt = a.cnt + i * a.nslice;
+    for (int64_t y0 = 0; y0 < a.nslice; y0 += 64) {
+      const int64_t y = y0 + lane;
+      const int n = y < a.nslice ? cnt[y] : 0;
+      if (n <= PTPAN_CAP) {
+        const int32_t* ent = a.ent + (i * a.nslice + y) * PTPAN_CAP;
+        for (int c = 0; c < n; ++c) walk(ent[c]);
+      }
+      // slices whose particles did not all fit: the whole wave, one slice at a time
+      uint64_t over = __builtin_amdgcn_ballot_w64(n > PTPAN_CAP);
+      while (over) {
+        const int l = __builtin_ctzll(over);
+        over &= over - 1;
+        const int64_t q0 = (y0 + l) * a.slen;
+        const int64_t q1 = q0 + a.slen < a.ns ? q0 + a.slen : a.ns;
+        for (int64_t p = q0 + lane; p < q1; p += 64) {
+          R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+          RootLim rl;
+          panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
+          if (!root_leaf<R>(a.sx[p], a.sy[p], a.sz[p], cx0, cy0, cz0, lim, rl, a.maxlev, true)) walk(p);
+        }
       }
     }
+    double u[3];
 #pragma unroll
-    for (int k = 0; k < PTPAN_GROUP; ++k) {
-      double u[3];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) u[d] = wave_sum((double)acc[k][d]);
-      if (lane != 0 || k >= ng) continue;
-      const int64_t i = a.near_list[e0 + k];
-      if (PREC == PREC_FP64) {
-        ((double*)a.u)[i] -= u[0];
-        ((double*)a.v)[i] -= u[1];
-        ((double*)a.w)[i] -= u[2];
-      } else {
-        float* uu = (float*)a.u; float* vv = (float*)a.v; float* ww = (float*)a.w;
-        uu[i] = (float)((double)uu[i] - u[0]);
-        vv[i] = (float)((double)vv[i] - u[1]);
-        ww[i] = (float)((double)ww[i] - u[2]);
-      }
+    for (int d = 0; d < 3; ++d) u[d] = wave_sum((double)acc[d]);
+    if (lane != 0) continue;
+    if (PREC == PREC_FP64) {
+      ((double*)a.u)[i] -= u[0];
+      ((double*)a.v)[i] -= u[1];
+      ((double*)a.w)[i] -= u[2];
+    } else {
+      float* uu = (float*)a.u; float* vv = (float*)a.v; float* ww = (float*)a.w;
+      uu[i] = (float)((double)uu[i] - u[0]);
+      vv[i] = (float)((double)vv[i] - u[1]);
+      ww[i] = (float)((double)ww[i] - u[2]);
     }
   }
 }
@@ -267,6 +262,7 @@ __global__ void k_ptpan_zero(unsigned int* c) {
 static int launch_ptpan(PtPanArgs a, int prec, hipStream_t st) {
   if (a.np <= 0 || a.ns <= 0) return O3D_OK;
   if (a.np > INT32_MAX) return fail(O3D_EINVAL, "points_affect_panels takes at most %d panels", INT32_MAX);
+  if (a.ns > INT32_MAX) return fail(O3D_EINVAL, "points_affect_panels takes at most %d particles", INT32_MAX);
   const int64_t nbx = (a.np + BLOCK - 1) / BLOCK;
   const int64_t ntiles = (a.ns + a.tile - 1) / a.tile;
   int64_t nsl = (16LL * num_cus() + nbx - 1) / nbx;
@@ -276,16 +272,21 @@ static int launch_ptpan(PtPanArgs a, int prec, hipStream_t st) {
   std::lock_guard<std::mutex> lk(stream_ws_mutex());
   void* ws = nullptr;
   const size_t list = (sizeof(int32_t) * (size_t)a.np + 255) & ~(size_t)255;
-  const size_t part = sizeof(double) * 4 * (size_t)a.nslice * (size_t)a.np;
-  int rc = stream_workspace(st, 256 + list + part, &ws);
+  const size_t part = sizeof(double) * 3 * (size_t)a.nslice * (size_t)a.np;
+  const size_t cnt = (sizeof(int32_t) * (size_t)a.nslice * (size_t)a.np + 255) & ~(size_t)255;
+  const size_t ent = sizeof(int32_t) * PTPAN_CAP * (size_t)a.nslice * (size_t)a.np;
+  int rc = stream_workspace(st, 256 + list + part + cnt + ent, &ws);
   if (rc != O3D_OK) return rc;
   a.near_count = (unsigned int*)ws;
   a.near_list = (int32_t*)((char*)ws + 256);
   a.part = (double*)((char*)ws + 256 + list);
+  a.cnt = (int32_t*)((char*)ws + 256 + list + part);
+  a.ent = (int32_t*)((char*)ws + 256 + list + part + cnt);
   const size_t lds = (size_t)a.tile * 2 * sizeof(float4);
   const dim3 grid((unsigned)nbx, (unsigned)a.nslice);
   const dim3 grid1((unsigned)nbx);
-  const dim3 grid_near((unsigned)std::min<int64_t>((a.np + 15) / 16, 4 * (int64_t)num_cus()));
+  // one wave per listed panel, grid-stride: at most 16 waves per CU
+  const dim3 grid_near((unsigned)std::min<int64_t>((a.np + 3) / 4, 4 * (int64_t)num_cus()));
   hipLaunchKernelGGL(k_ptpan_zero, dim3(1), dim3(64), 0, st, a.near_count);
 #define O3D_PP(P)                                                                  \
   if (prec == P) {                                                                 \

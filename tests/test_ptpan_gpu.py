@@ -96,3 +96,32 @@ def test_particles_on_full_icosphere_subset(infl):
     r32 = oracle.ptpan("rm", sp, ss, pan.nodes, sub_idx, pan.area[sel], "f32d")
     print("full sphere subset", rel_errors(v[:, sel], r64), "reference float", rel_errors(r32, r64))
     assert rel_errors(v[:, sel], r64)[0] <= _band(r32, r64)
+
+
+def test_particles_crowding_a_panel(infl):
+    """3,000 particles within 4 sqrt(area) of one panel, then 3,000 spread over a
+    shell: the first slices hold far more particles that need the recursion than the
+    near kernel keeps per (slice, panel), so it scans those slices again, while the
+    later slices fit."""
+    pan = scenes.icosphere_panels(levels=1)
+    tri = pan.nodes[:, pan.idx.reshape(-1, 3)[0]].astype(np.float64)
+    c = tri.mean(1)
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    n /= np.linalg.norm(n)
+    lim = 4.0 * np.sqrt(float(pan.area[0]))
+    rng = np.random.default_rng(34)
+    d = rng.normal(size=(3, 3000))
+    near = c[:, None] + n[:, None] * 0.3 * lim + d / np.linalg.norm(d, axis=0) * rng.uniform(0.05, 0.6, 3000) * lim
+    d = rng.normal(size=(3, 3000))
+    far = d / np.linalg.norm(d, axis=0) * rng.uniform(1.0, 2.0, 3000)
+    sp = np.ascontiguousarray(np.concatenate([near, far], 1).astype(np.float32))
+    ss = (rng.normal(size=(3, 6000)) / 6000).astype(np.float32)
+    r64 = oracle.ptpan("rm", sp, ss, pan.nodes, pan.idx, pan.area, "f64")
+    r32 = oracle.ptpan("rm", sp, ss, pan.nodes, pan.idx, pan.area, "f32d")
+    for prec in ("fp32", "comp"):
+        v = infl.ptpan(sp, ss, pan.nodes, pan.idx, pan.area, prec)
+        print(prec, rel_errors(v, r64), "reference float", rel_errors(r32, r64))
+        assert rel_errors(v, r64)[0] <= _band(r32, r64)
+        assert rel_errors(v, r32)[0] <= 1e-5
+    v = infl.ptpan(sp, ss, pan.nodes, pan.idx, pan.area, "fp64")
+    assert rel_errors(v, r64)[0] <= 1.2e-7
