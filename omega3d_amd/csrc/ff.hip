@@ -256,6 +256,39 @@ int run_core(const o3d_points* src, o3d_points* targ, bool thick, int prec, bool
 
 }  // namespace
 
+// Morton order of n points (the direct kernel's target order): workspace bytes,
+// and the order itself, written into ws (bounding box, keys, indices and the
+// sort's scratch; *perm points into ws)
+size_t point_order_bytes(int64_t n, hipStream_t st) {
+  size_t tb = 0;
+  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                         (const int*)nullptr, (int*)nullptr, (int)n, 0, 30, st) != hipSuccess)
+    return 0;
+  const size_t a = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
+  return 256 + 4 * a + ((tb + 255) & ~(size_t)255);
+}
+
+int point_order(const float* x, const float* y, const float* z, int64_t n, void* ws, hipStream_t st,
+                const int32_t** perm) {
+  const size_t a = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
+  char* p = (char*)ws;
+  unsigned* bb = (unsigned*)p;
+  unsigned* kin = (unsigned*)(p + 256);
+  unsigned* kout = (unsigned*)(p + 256 + a);
+  int* iin = (int*)(p + 256 + 2 * a);
+  int* iout = (int*)(p + 256 + 3 * a);
+  void* tmp = p + 256 + 4 * a;
+  size_t tb = 0;
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const unsigned*)nullptr, (unsigned*)nullptr,
+                                            (const int*)nullptr, (int*)nullptr, (int)n, 0, 30, st));
+  hipLaunchKernelGGL(k_ff_bbox_init, dim3(1), dim3(64), 0, st, bb);
+  hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(n, 256), 2048u)), dim3(256), 0, st, x, y, z, n, bb);
+  hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(n, 256)), dim3(256), 0, st, x, y, z, n, bb, kin, iin);
+  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, iin, iout, (int)n, 0, 30, st));
+  *perm = iout;
+  return O3D_OK;
+}
+
 std::mutex& stream_ws_mutex() { return g_ws_mu; }
 
 // caller holds stream_ws_mutex() for the whole enqueue, so that no other thread

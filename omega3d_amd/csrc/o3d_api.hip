@@ -285,6 +285,23 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
       return O3D_OK;
     }
   }
+  // EXP core: the kernel takes the exponential branch for a whole wave when any of
+  // its targets needs it for the current source, so large calls visit the targets
+  // in Morton order, where a wave's targets are neighbours.  A target's sums do
+  // not depend on the order: the results are the same bits.  Not under graph
+  // capture (the sort records memory fills).
+  if (o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX && !stream_capturing(st)) {
+    std::lock_guard<std::mutex> lk(stream_ws_mutex());
+    const size_t bytes = point_order_bytes(a.nt, st);
+    if (bytes == 0) return fail(O3D_EDEVICE, "sort sizing failed");
+    void* ws = nullptr;
+    int rc = stream_workspace(st, bytes, &ws);
+    if (rc != O3D_OK) return rc;
+    if ((rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ws, st, &a.tperm)) != O3D_OK) return rc;
+    hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
+    if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
+    return O3D_OK;
+  }
   hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
   return O3D_OK;

@@ -45,8 +45,14 @@ bool stream_capturing(hipStream_t st);
 // frees st's workspace (before the stream is destroyed); takes stream_ws_mutex()
 void release_stream_workspace(hipStream_t st);
 bool stream_workspace_fits(hipStream_t st, size_t bytes);
+// Morton order of n points into a workspace of point_order_bytes(n) (0: error)
+size_t point_order_bytes(int64_t n, hipStream_t st);
+int point_order(const float* x, const float* y, const float* z, int64_t n, void* ws, hipStream_t st,
+                const int32_t** perm);
 // a collection large enough for the factored method to pay when the caller leaves the choice open
 constexpr int64_t FF_AUTO_MIN_N = 65536;
+// direct EXP calls with at least this many targets visit them in Morton order
+constexpr int64_t TARGET_ORDER_MIN_N = 32768;
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);
 int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, int32_t same, int32_t maxlev,
                    float* coeffs, hipStream_t st, o3d_bem_matrix_profile* prof);

@@ -30,6 +30,7 @@ struct P2PArgs {
   int64_t nt;
   int32_t tile;        // sources per LDS tile (multiple of 64, <= 1024)
   int32_t self_excl;   // subtract the j == i pair (src and targ are one collection)
+  const int32_t* tperm;   // k_p2p only: target order (nullptr: index order)
 };
 
 // double-precision views for the FP64 oracle kernel

@@ -308,18 +308,20 @@ struct PartCols {
 template <int CORE, bool THICK, bool GRAD, int PREC, int TPT, int OUT>
 __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int64_t s_lo, int64_t s_hi, int64_t nt,
                                          int64_t t_base, int tile, bool self_excl, float4* sh, void* part,
-                                         int64_t slice) {
+                                         int64_t slice, const int32_t* tperm = nullptr) {
   __shared__ unsigned int s_tmax;
   const int tid = threadIdx.x;
   float tx[TPT], ty[TPT], tz[TPT], tt[TPT];
-  int64_t ti[TPT];
+  int64_t ti[TPT], tix[TPT];
 #pragma unroll
   for (int k = 0; k < TPT; ++k) {
     const int64_t i = t_base + tid + (int64_t)k * BLOCK;
     ti[k] = i;
     const int64_t ic = i < nt ? i : nt - 1;    // idle lanes mirror a real target
-    tx[k] = t.x[ic]; ty[k] = t.y[ic]; tz[k] = t.z[ic];
-    tt[k] = THICK ? core_term<CORE>(t.r[ic]) : 0.0f;
+    const int64_t ix = tperm ? (int64_t)tperm[ic] : ic;   // the target's own index
+    tix[k] = ix;
+    tx[k] = t.x[ix]; ty[k] = t.y[ix]; tz[k] = t.z[ix];
+    tt[k] = THICK ? core_term<CORE>(t.r[ix]) : 0.0f;
   }
   float acc[TPT][NACC];
   float hi[TPT][3], lo[TPT][3];
@@ -385,7 +387,7 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
   for (int k = 0; k < TPT; ++k) {
     ks[k] = 0.0f; sax[k] = say[k] = saz[k] = 0.0f;
     if (GRAD && self_excl) {
-      const int64_t ic = ti[k] < nt ? ti[k] : nt - 1;
+      const int64_t ic = tix[k];
       float Bp;
       kb<CORE>(0.0f, core_term<CORE>(s.r[ic]) + tt[k], 0.0f, ks[k], Bp);
       sax[k] = s.ax[ic]; say[k] = s.ay[ic]; saz[k] = s.az[ic];
@@ -394,8 +396,8 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
 
 #pragma unroll
   for (int k = 0; k < TPT; ++k) {
-    const int64_t i = ti[k];
-    if (i >= nt) continue;
+    if (ti[k] >= nt) continue;
+    const int64_t i = tix[k];
     if (PREC == PREC_FP32) {
       float r[12];
       r[0] = acc[k][0];
@@ -465,11 +467,14 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
                                                                   nullptr, 0);
 }
 
+// a.tperm (optional): slot i of the launch evaluates target a.tperm[i]; a target's
+// sums do not depend on the slot, so any order gives the same bits
 template <int CORE, bool THICK, bool GRAD, int PREC, int TPT>
 __global__ __launch_bounds__(BLOCK) void k_p2p(P2PArgs a) {
   extern __shared__ float4 sh[];
   const int64_t t_base = (int64_t)blockIdx.x * (BLOCK * TPT);
-  p2p_chunk<CORE, THICK, GRAD, PREC, TPT>(a.s, a.t, a.ns, a.nt, t_base, a.tile, a.self_excl != 0, sh);
+  p2p_span<CORE, THICK, GRAD, PREC, TPT, OUT_ADD>(a.s, a.t, 0, a.ns, a.nt, t_base, a.tile, a.self_excl != 0, sh,
+                                                  nullptr, 0, a.tperm);
 }
 
 // ---------------------------------------------------------------------------
