@@ -50,6 +50,7 @@ constexpr int FF_M0 = 16;           // near field: fewest tiles per work item
 constexpr int FF_FAR_THREADS = 256; // far field: 4 waves of 64 targets per block
 constexpr int FF_TP = 264;          // far field: LDS row of the FP16 moment tables (halves, tile <= 256)
 constexpr int FF_SL = 40;           // far field: LDS row of a source's BF16 r2 slots (32 used, 80 bytes)
+constexpr int FF_MERGE = 4;         // far field: most consecutive tiles of equal scales per fold
 
 struct FFArgs {
   const float4* sp;    // sorted sources {x, y, z, core_term}
@@ -407,6 +408,33 @@ __global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
   for (int k = 0; k < 12; ++k) sum[k] = 0.0;
 
   const unsigned* mrow = a.mask + blk * a.mwords;
+  // consecutive far tiles with the same scales share one fold: acc keeps summing
+  // (at most FF_MERGE tiles, which bounds its FP32 sum) and the epilogue runs when
+  // the scales change, the run is full, or the tiles end
+  int run = 0, pew = 0, peg[5] = {0, 0, 0, 0, 0};
+  bf16x8 tb[2][2];
+  // fold: one v_permlane32_swap per register exchanges the upper 32 lanes of acc0[r]
+  // with the lower 32 of acc1[r], after which every lane holds its own target's rows
+  // of half 0 in acc0[r] and of half 1 in acc1[r] (row of register r in half h:
+  // 8 (r / 4) + 4 h + r % 4); the scales are removed and the epilogue adds the run's
+  // velocity and gradient in FP64
+  auto fold = [&]() {
+    double c[32];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]),
+                                                       false, false);
+      const int i0 = 8 * (r / 4) + (r % 4);
+      c[i0] = (double)__uint_as_float(sw[0]) * ldexp(1.0, -(pew + peg[ff_group(i0)]));
+      c[i0 + 4] = (double)__uint_as_float(sw[1]) * ldexp(1.0, -(pew + peg[ff_group(i0 + 4)]));
+      acc0[r] = 0.0f;
+      acc1[r] = 0.0f;
+    }
+    double out[12];
+    ff_epilogue(c, ox, oy, oz, oa, out);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) sum[k] += out[k];
+  };
   for (int ti = 0; ti < a.ntiles; ++ti) {
     if (!((mrow[ti >> 5] >> (ti & 31)) & 1u)) continue;
     const int64_t s0 = (int64_t)ti * T;
@@ -433,10 +461,21 @@ __global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
     float gs[5];
 #pragma unroll
     for (int g = 0; g < 5; ++g) gs[g] = ldexpf(1.0f, eg[g]);
+    bool same = run > 0 && run < FF_MERGE && ew == pew;
+#pragma unroll
+    for (int g = 0; g < 5; ++g) same = same && eg[g] == peg[g];
+    if (run > 0 && !same) {
+      fold();
+      run = 0;
+    }
+    pew = ew;
+#pragma unroll
+    for (int g = 0; g < 5; ++g) peg[g] = eg[g];
+    ++run;
 
     // this lane's target slots for both sets and both r2 products (K slots 8 half + e
-    // and 16 + 8 half + e)
-    bf16x8 tb[2][2];
+    // and 16 + 8 half + e); they depend on the tile only through rsc
+    if (run == 1) {
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
       const float tv[5] = {tx[s], ty[s], tz[s], 1.0f, al[s] * rsc};
@@ -452,6 +491,7 @@ __global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
           const __bf16 v1 = q1 < 30 ? part[ff_tgt_part(q1 / 5)][q1 % 5] : (__bf16)0.0f;
           tb[s][m][e] = half ? v1 : v0;
         }
+    }
     }
 
     __syncthreads();
@@ -530,29 +570,8 @@ __global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
         acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Al, Bh1, acc1, 0, 0, 0);
       }
     }
-    // fold this tile: one v_permlane32_swap per register exchanges the upper 32 lanes
-    // of acc0[r] with the lower 32 of acc1[r], after which every lane holds its own
-    // target's rows of half 0 in acc0[r] and of half 1 in acc1[r] (row of register r
-    // in half h: 8 (r / 4) + 4 h + r % 4); the scales are removed and the epilogue
-    // adds this tile's velocity and gradient in FP64
-    {
-      double c[32];
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const auto sw = __builtin_amdgcn_permlane32_swap(__float_as_uint(acc0[r]), __float_as_uint(acc1[r]),
-                                                         false, false);
-        const int i0 = 8 * (r / 4) + (r % 4);
-        c[i0] = (double)__uint_as_float(sw[0]) * ldexp(1.0, -(ew + eg[ff_group(i0)]));
-        c[i0 + 4] = (double)__uint_as_float(sw[1]) * ldexp(1.0, -(ew + eg[ff_group(i0 + 4)]));
-        acc0[r] = 0.0f;
-        acc1[r] = 0.0f;
-      }
-      double out[12];
-      ff_epilogue(c, ox, oy, oz, oa, out);
-#pragma unroll
-      for (int k = 0; k < 12; ++k) sum[k] += out[k];
-    }
   }
+  if (run > 0) fold();
   const int64_t i = base + lane;
   if (i < a.nt) {
 #pragma unroll
