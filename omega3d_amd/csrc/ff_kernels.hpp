@@ -50,7 +50,7 @@ constexpr int FF_M0 = 16;           // near field: fewest tiles per work item
 constexpr int FF_FAR_THREADS = 256; // far field: 4 waves of 64 targets per block
 constexpr int FF_TP = 264;          // far field: LDS row of the FP16 moment tables (halves, tile <= 256)
 constexpr int FF_SL = 40;           // far field: LDS row of a source's BF16 r2 slots (32 used, 80 bytes)
-constexpr int FF_MERGE = 4;         // far field: most consecutive tiles of equal scales per fold
+constexpr int FF_MERGE = 8;         // far field: most consecutive tiles of equal scales per fold
 
 struct FFArgs {
   const float4* sp;    // sorted sources {x, y, z, core_term}
