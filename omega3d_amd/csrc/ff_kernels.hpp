@@ -320,6 +320,18 @@ __device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
   lo = (_Float16)(x - h);
 }
 
+// the same split for two weights into elements e, e + 1 of packed FP16 vectors:
+// hi by one round-toward-zero pair conversion (the top 11 significant bits, as the
+// mask above, for |x| >= 2^-14), lo = x - hi in one mixed-precision FMA rounded to
+// FP16
+__device__ __forceinline__ void split16x2(float x0, float x1, f16x8& hv, f16x8& lv, int e) {
+  const auto h = __builtin_amdgcn_cvt_pkrtz(x0, x1);
+  hv[e] = (_Float16)h[0];
+  hv[e + 1] = (_Float16)h[1];
+  lv[e] = (_Float16)(x0 - (float)h[0]);
+  lv[e + 1] = (_Float16)(x1 - (float)h[1]);
+}
+
 // BF16 three-way split x = hi + mid + lo (about 24 significant bits): hi and mid
 // take the leading bits, each remainder is exact in FP32
 __device__ __forceinline__ void split_bf3(float x, __bf16& hi, __bf16& mid, __bf16& lo) {
@@ -555,12 +567,9 @@ __global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
         const f16x8 Al = *reinterpret_cast<const f16x8*>(ml + col * TP + pbase);
         f16x8 Bh0, Bl0, Bh1, Bl1;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-          _Float16 hh, ll;
-          split16(bp_of(r20[8 * kb + e]), hh, ll);
-          Bh0[e] = hh; Bl0[e] = ll;
-          split16(bp_of(r21[8 * kb + e]), hh, ll);
-          Bh1[e] = hh; Bl1[e] = ll;
+        for (int e = 0; e < 8; e += 2) {
+          split16x2(bp_of(r20[8 * kb + e]), bp_of(r20[8 * kb + e + 1]), Bh0, Bl0, e);
+          split16x2(bp_of(r21[8 * kb + e]), bp_of(r21[8 * kb + e + 1]), Bh1, Bl1, e);
         }
         acc0 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh0, acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_32x32x16_f16(Ah, Bh1, acc1, 0, 0, 0);
