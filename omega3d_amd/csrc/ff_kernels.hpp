@@ -310,20 +310,11 @@ __device__ __forceinline__ int ff_exp2_above(double x) {
   return e;
 }
 
-// FP16 hi/lo split of a scaled FP32 value (|x| < 2^15): hi keeps the top 11
-// significant bits (truncated, so it converts to FP16 exactly for |x| >= 2^-14),
-// lo = x - hi is exact in FP32 and rounds to FP16 with a relative error of 2^-11,
-// i.e. about 2^-22 of x.
-__device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
-  const float h = __uint_as_float(__float_as_uint(x) & 0xffffe000u);
-  hi = (_Float16)h;
-  lo = (_Float16)(x - h);
-}
-
-// the same split for two weights into elements e, e + 1 of packed FP16 vectors:
-// hi by one round-toward-zero pair conversion (the top 11 significant bits, as the
-// mask above, for |x| >= 2^-14), lo = x - hi in one mixed-precision FMA rounded to
-// FP16
+// FP16 hi/lo split of two scaled FP32 values (|x| < 2^15) into elements e, e + 1
+// of packed FP16 vectors: hi by one round-toward-zero pair conversion (the top 11
+// significant bits, exact in FP16 for |x| >= 2^-14); lo = x - hi, exact in FP32,
+// rounded to FP16 by one mixed-precision FMA (a relative error of 2^-11 of lo,
+// about 2^-22 of x).  The moment columns of the staging take the same split.
 __device__ __forceinline__ void split16x2(float x0, float x1, f16x8& hv, f16x8& lv, int e) {
   const auto h = __builtin_amdgcn_cvt_pkrtz(x0, x1);
   hv[e] = (_Float16)h[0];
@@ -508,11 +499,13 @@ __global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
 
     __syncthreads();
     for (int j = tid; j < T; j += FF_FAR_THREADS) {
-      // each moment column is scaled, split and stored as soon as it is formed
+      // the 32 moment columns, scaled, then split into FP16 hi/lo two at a time
       const int pj = ff_perm(j);
-      auto put = [&](int c, float x) { split16(x * gs[ff_group(c)], mh[c * TP + pj], ml[c * TP + pj]); };
       __bf16* srow = sl + j * FF_SL;
       float sv5[5];
+      float col[32];
+#pragma unroll
+      for (int c = 0; c < 32; ++c) col[c] = 0.0f;
       if (j < cnt) {
         const float4 P = a.sp[s0 + j];
         const float4 A = a.sa[s0 + j];
@@ -526,22 +519,27 @@ __global__ __launch_bounds__(FF_FAR_THREADS, 2) void k_ff_far(FFArgs a) {
         sv5[3] = b * rsc; sv5[4] = 1.0f;
         const float sv[3] = {sx, sy, sz}, av[3] = {A.x, A.y, A.z}, mv[3] = {m0, m1, m2};
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { put(k, av[k]); put(3 + k, mv[k]); }
+        for (int k = 0; k < 3; ++k) { col[k] = av[k]; col[3 + k] = mv[k]; }
 #pragma unroll
         for (int l = 0; l < 3; ++l)
 #pragma unroll
-          for (int k = 0; k < 3; ++k) put(6 + 3 * l + k, sv[l] * av[k]);
+          for (int k = 0; k < 3; ++k) col[6 + 3 * l + k] = sv[l] * av[k];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) put(15 + e, sv[e / 3] * mv[e % 3]);
+        for (int e = 0; e < 8; ++e) col[15 + e] = sv[e / 3] * mv[e % 3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { put(23 + k, b * av[k]); put(26 + k, b * mv[k]); }
-#pragma unroll
-        for (int c = 29; c < 32; ++c) put(c, 0.0f);
+        for (int k = 0; k < 3; ++k) { col[23 + k] = b * av[k]; col[26 + k] = b * mv[k]; }
       } else {
         // padding: zero moments and a large finite distance, so that Bp * moment = 0
         sv5[0] = 0.0f; sv5[1] = 0.0f; sv5[2] = 0.0f; sv5[3] = 1e20f; sv5[4] = 1.0f;
+      }
 #pragma unroll
-        for (int c = 0; c < 32; ++c) put(c, 0.0f);
+      for (int c = 0; c < 32; c += 2) {
+        const float x0 = col[c] * gs[ff_group(c)], x1 = col[c + 1] * gs[ff_group(c + 1)];
+        const auto h = __builtin_amdgcn_cvt_pkrtz(x0, x1);
+        mh[c * TP + pj] = (_Float16)h[0];
+        mh[(c + 1) * TP + pj] = (_Float16)h[1];
+        ml[c * TP + pj] = (_Float16)(x0 - (float)h[0]);
+        ml[(c + 1) * TP + pj] = (_Float16)(x1 - (float)h[1]);
       }
       __bf16 part[3][5];
 #pragma unroll
