@@ -142,3 +142,44 @@ def test_bem_matrix_refuses_capture(env):
         hip.hipGraphDestroy(graph)
     finally:
         stream_destroy(st)
+
+
+def test_captured_exp_call_skips_target_order_with_same_bits(env):
+    """A direct EXP call of 40,000 targets visits them in Morton order when run
+    eagerly and in index order under capture (the sort records memory fills): the
+    replays give the eager result's bits."""
+    _lib, hip = env
+    from omega3d_amd.batch import stream_create, stream_destroy
+    st = stream_create()
+    try:
+        n = 40000
+        p = scenes.ring(n, sigma=0.02)
+        D = _lib.DeviceArray.from_host
+        keep = [D(p.pos), D(p.str), D(p.rad)]
+        dv, dg = _lib.DeviceArray((3, n)), _lib.DeviceArray((9, n))
+        s = _lib.Points()
+        s.n = n
+        for d in range(3):
+            s.x[d], s.s[d], s.u[d] = keep[0].row(d), keep[1].row(d), dv.row(d)
+        s.r = keep[2].ptr
+        for k in range(9):
+            s.ug[k] = dg.row(k)
+        o = _lib.make_opts("exp", True, "comp", tile=256, tpt=1, stream=st, method="direct")
+        call = lambda: _lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None)
+        dv.zero(); dg.zero(); _lib.synchronize()
+        _lib.check(call())
+        _lib.check(_lib.lib.o3d_stream_synchronize(st))
+        v0, g0 = dv.to_host(), dg.to_host()
+        graph, exe = _capture(_lib, hip, st, call)
+        try:
+            for rep in range(2):
+                dv.zero(); dg.zero(); _lib.synchronize()
+                assert hip.hipGraphLaunch(exe, C.c_void_p(st)) == 0
+                _lib.check(_lib.lib.o3d_stream_synchronize(st))
+                np.testing.assert_array_equal(dv.to_host(), v0)
+                np.testing.assert_array_equal(dg.to_host(), g0)
+        finally:
+            hip.hipGraphExecDestroy(exe)
+            hip.hipGraphDestroy(graph)
+    finally:
+        stream_destroy(st)
