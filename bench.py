@@ -1,23 +1,22 @@
 """Headline benchmark: direct Biot-Savart velocity + velocity gradient at N = 1,048,576.
 
-Workload (one "step" = one evaluation of every particle's velocity and 3x3
-velocity gradient from every particle, inputs resident in HBM):
+Workload (one "step" = zeroing the outputs and one evaluation of every particle's
+velocity and 3x3 velocity gradient from every particle, inputs resident in HBM):
   two colliding vortex rings, 524,288 particles each (centres z = +-1, lower ring
   tilted 15 deg about x, circulation +-1, |alpha| jittered +-5%), core radius 0.02,
   compact-exponential core, FP32 pairs with compensated accumulation.
 
-The evaluation uses the library's default method, which at this size is the
-factored one (Morton order; well-separated target-block / source-tile pairs
-summed through a per-source moment table on the matrix cores; the rest pair by
-pair).  Prints ONE JSON line with G interactions/s (value), the plain-FP32 rate
-beside it, the direct (pair-by-pair) method's rate, the roofline of the dominant
-kernel (the far-field matrix-core kernel: the reference's 68 flops per
-interaction times the pairs it takes, over its HIP-event time), max/RMS relative
-error against the FP64 kernel on a target subset, step-latency percentiles and the
-CPU baseline (the oracle's Vc-style loop on this host's cores, on a bounded
-target sample).
+The headline is the direct pair-by-pair sum on the vector ALUs (k_p2p, method
+DIRECT): `value` is G interactions/s over the timed steps, `roofline` its FP32
+VALU rate in the reference's flop count (68 per interaction + 12 per target) over
+the call's HIP-event time, against the 157.3 TFLOP/s FP32 vector peak.  Beside it:
+the plain-FP32 direct rate, max/RMS relative error of both against the FP64
+kernel on a target subset, the factored method (the library's AUTO choice at this
+size; matrix-core arithmetic, reported under its own key), and the CPU baseline:
+the reference's own kernels (oracle/_ref) on this host's cores on a bounded
+target sample, with the GPU result compared against them.
 
-    python bench.py [--gpus 1] [--steps 5] [--warmup 1] [--no-cpu] [--no-error]
+    python bench.py [--gpus 1] [--steps 20] [--warmup 5] [--no-cpu] [--no-error]
 """
 import argparse
 import ctypes as C
@@ -38,8 +37,10 @@ FLOPS_PAIR = 68.0          # reference flop count of one EXP vel+grad interactio
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--plain-steps", type=int, default=5,
+                    help="timed steps of the plain-FP32 and factored legs (each after one warm-up)")
     ap.add_argument("--n-each", type=int, default=524288)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-error", action="store_true")
@@ -47,7 +48,9 @@ def parse():
     ap.add_argument("--error-targets", type=int, default=16384)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--tpt", type=int, default=0)
-    ap.add_argument("--method", default="auto", choices=["auto", "direct", "factored"])
+    ap.add_argument("--method", default="direct", choices=["auto", "direct", "factored"],
+                    help="method of the headline leg (the north-star kernel is the direct one)")
+    ap.add_argument("--no-factored", action="store_true", help="skip the factored-method leg")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="also run the known-traffic copy kernel on this many MiB (counter calibration)")
     ap.add_argument("--no-batch", action="store_true", help="skip the 256-instance batched workload")
@@ -173,6 +176,127 @@ def scene_io_leg(c, log):
     return out
 
 
+def fp64_subset(L, _lib, c, sel):
+    """FP64 kernel (k_p2p64): targets c[:, sel] against all of c, double device arrays."""
+    nt = len(sel)
+    d64 = lambda a: _lib.DeviceArray.from_host(np.ascontiguousarray(a, np.float64))
+    spos64, sstr64, srad64 = d64(c.pos), d64(c.str), d64(c.rad)
+    tpos64, trad64 = d64(c.pos[:, sel]), d64(c.rad[sel])
+    v64 = _lib.DeviceArray.from_host(np.zeros((3, nt)))
+    g64 = _lib.DeviceArray.from_host(np.zeros((9, nt)))
+    s64, t64 = _lib.Points64(), _lib.Points64()
+    s64.n, t64.n = c.n, nt
+    for d in range(3):
+        s64.x[d], s64.s[d] = spos64.row(d), sstr64.row(d)
+        t64.x[d], t64.u[d] = tpos64.row(d), v64.row(d)
+    s64.r, t64.r = srad64.ptr, trad64.ptr
+    for k in range(9):
+        t64.ug[k] = g64.row(k)
+    o = _lib.make_opts("exp", True, "fp64")
+    _lib.check(L.o3d_points_affect_points64_dev(C.byref(s64), C.byref(t64), C.byref(o), None))
+    _lib.synchronize()
+    return v64.to_host(), g64.to_host()
+
+
+def traffic_of(kernel_file):
+    """HBM bytes per launch of a kernel from a committed rocprofv3 --pmc summary
+    (profiles/*.json), with the command, commit and date it records: the counters
+    cannot be read from inside this process."""
+    f = os.path.join(ROOT, "profiles", kernel_file)
+    if not os.path.exists(f):
+        return None, None
+    d = json.load(open(f))
+    src = {"file": "profiles/" + kernel_file, "commit": d.get("commit"), "date": d.get("date"),
+           "command": d.get("command"), "measured_kernel_ms": d.get("kernel_ms")}
+    return d.get("traffic_bytes"), src
+
+
+def cpu_leg(args, c, vel_c, grad_c, log):
+    """CPU baseline of the headline workload on this host's cores: the reference's
+    own EXP kernels (oracle/_ref: Kernels.h / CoreFunc.h compiled in place, the
+    non-Vc build, S = float, A = double, OpenMP over targets; Influence.h:405-440)
+    on a spread sample of targets sized for about --cpu-seconds, with the GPU
+    result on the same targets compared against it; beside it the oracle's
+    Vc-style AVX2 restatement of the Vc loop.  Without oracle/_ref the port is the
+    baseline."""
+    import oracle
+    from omega3d_amd.metrics import field_errors
+    N = c.n
+    nth = oracle.lib().orc_max_threads()
+    out = None
+    if oracle.ref_lib("exp") is not None:
+        nr0 = 8 * nth
+        sel0 = np.linspace(0, N - 1, nr0).astype(np.int64)
+        t = time.perf_counter()
+        oracle.ref_p2p("exp", c.pos, c.str, c.rad, c.pos[:, sel0], c.rad[sel0], True, "f32d")
+        dt0 = time.perf_counter() - t
+        ntc = int(min(N, max(1024, nr0 * args.cpu_seconds / max(dt0, 1e-3))))
+        sel = np.linspace(0, N - 1, ntc).astype(np.int64)
+        t = time.perf_counter()
+        rv, rg = oracle.ref_p2p("exp", c.pos, c.str, c.rad, c.pos[:, sel], c.rad[sel], True, "f32d")
+        dt = time.perf_counter() - t
+        out = {"value": round(ntc * float(N) / dt / 1e9, 4), "unit": "G interactions/s", "cores": nth,
+               "kind": "reference",
+               "sample": f"the reference's kernel_0v_0bg / EXP core_func (oracle/_ref, S = float, A = double, "
+                         f"OpenMP over targets) for {ntc} evenly spread targets x all {N} sources, {dt:.1f} s",
+               "gpu_vs_reference": dict(field_errors(vel_c[:, sel], grad_c[:, sel], rv, rg), targets=ntc,
+                                        what="the headline's compensated direct result against the reference's "
+                                             "own kernels on the same targets (bar 1e-5)")}
+    if oracle.lib().orc_have_avx2():
+        t = time.perf_counter()
+        oracle.p2p_vcstyle("exp", c.pos, c.str, c.rad, c.pos, c.rad, 0, 4 * nth, nth)
+        dt = time.perf_counter() - t
+        ntc = int(min(N, max(4 * nth, 4 * nth * args.cpu_seconds / 2 / max(dt, 1e-3))))
+        t = time.perf_counter()
+        oracle.p2p_vcstyle("exp", c.pos, c.str, c.rad, c.pos, c.rad, 0, ntc, nth)
+        dt = time.perf_counter() - t
+        port = {"value": round(ntc * float(N) / dt / 1e9, 4), "unit": "G interactions/s", "cores": nth,
+                "kind": "port",
+                "sample": f"Vc-style AVX2 restatement of Influence.h:388-442 (8 float lanes, rsqrt/rcp estimates, "
+                          f"OpenMP over targets), first {ntc} targets x all {N} sources, {dt:.1f} s"}
+        if out is None:
+            out = port
+        else:
+            out["vc_style_port"] = port
+    log(f"bench: cpu baseline {out}")
+    return out
+
+
+def single_ring_cpu(single, log):
+    """c0 on the host cores: the reference's own RM kernels (oracle/_ref, S = float,
+    A = double) over the full 8,192 x 8,192 workload, repeated for at least 5 s, with
+    the GPU's self-included FP32 and compensated results compared against them; and
+    the oracle's Vc-style loop."""
+    import oracle
+    from omega3d_amd import influence, scenes
+    from omega3d_amd.metrics import field_errors
+    p0 = scenes.ring(8192, sigma=0.05)
+    nth = oracle.lib().orc_max_threads()
+    out = {"cores": nth}
+    if oracle.ref_lib("rm") is not None:
+        reps, t = 0, time.perf_counter()
+        while True:
+            rv, rg = oracle.ref_p2p("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, True, "f32d")
+            reps += 1
+            if time.perf_counter() - t >= 5.0:
+                break
+        secs = (time.perf_counter() - t) / reps
+        gv, gg = influence.p2p("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, True, "fp32")
+        cv, cg = influence.p2p("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, True, "comp")
+        out["reference"] = {"kind": "reference", "ms": round(secs * 1e3, 3), "reps": reps,
+                            "ginter_s": round(p0.n * float(p0.n) / secs / 1e9, 3),
+                            "what": "the reference's kernel_0v_0bg / RM core_func (oracle/_ref, S = float, "
+                                    "A = double, OpenMP over targets), every target, self pair included",
+                            "gpu_fp32_vs_reference": field_errors(gv, gg, rv, rg),
+                            "gpu_compensated_vs_reference": field_errors(cv, cg, rv, rg)}
+    if oracle.lib().orc_have_avx2():
+        t = time.perf_counter()
+        oracle.p2p_vcstyle("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, 0, p0.n, nth)
+        out["vc_style_port_ms"] = round((time.perf_counter() - t) * 1e3, 3)
+    log(f"bench: single ring on the CPU {out}")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -218,7 +342,9 @@ def main():
     for k in range(9):
         tgt.ug[k] = dgrad.row(k)
 
-    def run(prec, steps, warmup, method=args.method):
+    def run(prec, steps, warmup, method):
+        """warmup + steps calls; each step zeroes the outputs and makes one call, the
+        call bracketed by HIP events on the stream it runs on (the null stream)"""
         o = _lib.make_opts("exp", True, prec, 0, args.tile, args.tpt, method=method)
         evs = [(_lib.Event(), _lib.Event()) for _ in range(steps)]
         for _ in range(warmup):
@@ -265,128 +391,82 @@ def main():
         log(f"bench: copy probe {args.copy_probe_mb} MiB x3 done")
         del a_, b_
 
-    log(f"bench: compensated FP32, {args.warmup} warmup + {args.steps} timed steps")
-    wall, kms = run("comp", args.steps, args.warmup)
-    vel_c, grad_c = dvel.to_host(), dgrad.to_host()
     inter = float(N) * float(N)
-    value = inter * args.steps / wall / 1e9
-    kavg = kms.mean() / 1e3
-    flops = N * (12.0 + FLOPS_PAIR * N)
-    call_tflops = flops / kavg / 1e12
-    log(f"bench: comp {value:.1f} G int/s, call avg {kms.mean():.1f} ms, {call_tflops:.2f} TFLOP/s")
+    flops = N * (12.0 + FLOPS_PAIR * N)       # Influence.h:475 with flops_0v_0bg<EXP> = 68
 
-    # stage times of the factored path, HIP events on the evaluation's stream
-    factored = args.method == "factored" or (args.method == "auto" and args.tile == 0 and args.tpt == 0)
-    prof = None
-    if factored:
-        o = _lib.make_opts("exp", True, "comp", 0, args.tile, 0, method="factored")
+    # ---- headline: the direct kernel, compensated --------------------------------
+    log(f"bench: direct, compensated FP32, {args.warmup} warmup + {args.steps} timed steps")
+    wall, kms = run("comp", args.steps, args.warmup, args.method)
+    vel_c, grad_c = dvel.to_host(), dgrad.to_host()
+    value = inter * args.steps / wall / 1e9
+    kavg = float(kms.mean())
+    achieved = flops / (kavg / 1e3) / 1e12
+    log(f"bench: direct comp {value:.1f} G int/s, call avg {kavg:.2f} ms, {achieved:.2f} TFLOP/s")
+
+    psteps = max(1, min(args.steps, args.plain_steps))
+    pwall, pkms = run("fp32", psteps, 1, args.method)
+    vel_p, grad_p = dvel.to_host(), dgrad.to_host()
+    plain = {"value": round(inter * psteps / pwall / 1e9, 3), "unit": "G interactions/s",
+             "call_ms": round(float(pkms.mean()), 3), "steps": psteps,
+             "tflops": round(flops / (pkms.mean() / 1e3) / 1e12, 3),
+             "kernel": "k_p2p<EXP,thick,grad,fp32>"}
+    log(f"bench: direct plain fp32 {plain}")
+
+    # ---- the factored method (AUTO's choice at this size), its own arithmetic -------
+    factored = None
+    if not args.no_factored:
+        fsteps = max(1, min(args.steps, args.plain_steps))
+        fwall, fkms = run("comp", fsteps, 1, "factored")
+        vel_f, grad_f = dvel.to_host(), dgrad.to_host()
+        o = _lib.make_opts("exp", True, "comp", 0, 0, 0, method="factored")
         recs = []
-        for _ in range(max(1, min(args.steps, 3))):
+        for _ in range(3):
             dvel.zero(); dgrad.zero()
             pr = _lib.FactoredProfile()
             _lib.check(L.o3d_factored_profile_dev(C.byref(src), C.byref(tgt), C.byref(o), C.byref(pr)))
             recs.append({k: getattr(pr, k) for k, _ in _lib.FactoredProfile._fields_})
         prof = {k: float(np.mean([r[k] for r in recs])) for k in recs[0]}
-        prof["steps"] = len(recs)
-        log(f"bench: factored stages {prof}")
-
-    psteps = max(1, min(args.steps, 3))
-    pwall, pkms = run("fp32", psteps, 1)
-    vel_p, grad_p = dvel.to_host(), dgrad.to_host()
-    plain_value = inter * psteps / pwall / 1e9
-    log(f"bench: plain fp32 {plain_value:.1f} G int/s, kernel avg {pkms.mean():.1f} ms")
-
-    direct = None
-    if factored:
-        dwall, dkms = run("comp", 1, 0, method="direct")
-        direct = {"value": round(inter / dwall / 1e9, 3), "unit": "G interactions/s",
-                  "kernel_ms": round(float(dkms.mean()), 3), "kernel": "k_p2p<EXP,thick,grad,compensated>",
-                  "tflops": round(flops / (dkms.mean() / 1e3) / 1e12, 3)}
-        log(f"bench: direct method {direct}")
+        # matrix-core work of the far kernel, from its structure: per 32 sources x 32
+        # targets, two 32x32x16 BF16 products (r^2) and six 32x32x16 FP16 products
+        # (three split products x two 16-column halves of the moment table)
+        mfma_flops = prof["far_pairs"] / 1024.0 * 8 * (32 * 32 * 16 * 2)
+        factored = {"value": round(inter * fsteps / fwall / 1e9, 3), "unit": "G interactions/s",
+                    "call_ms": round(float(fkms.mean()), 3), "steps": fsteps,
+                    "dtype": "far pairs: r^2 as BF16x3 split products and weights x 29 moment columns as FP16x2 "
+                             "split products on the matrix cores, FP32 VALU weights, FP64 epilogue folds; near "
+                             "pairs: FP32 pairs with compensated accumulation, as the direct kernel",
+                    "stages_ms": {k: round(prof[k], 3) for k in ("prep_ms", "far_ms", "near_ms", "total_ms")},
+                    "far_pair_fraction": round(prof["far_tile_fraction"], 5),
+                    "far_kernel": {"kernel": "k_ff_far<EXP,thick,compensated>", "ms": round(prof["far_ms"], 3),
+                                   "matrix_flops_per_launch": mfma_flops,
+                                   "matrix_tflops": round(mfma_flops / (prof["far_ms"] / 1e3) / 1e12, 2),
+                                   "matrix_peak_frac": round(mfma_flops / (prof["far_ms"] / 1e3) / 2.5e15, 4),
+                                   "what": "dense FP16/BF16 matrix-core rate of the far kernel's products "
+                                           "(counted from the kernel's structure) over its HIP-event time, "
+                                           "against the 2.5 PFLOP/s dense peak; unit utilisation from counters "
+                                           "is in profiles/ff_far_exp_n1m_counters.md"}}
+        log(f"bench: factored {factored}")
 
     errors = None
     if not args.no_error:
         nt = min(args.error_targets, N)
         sel = np.linspace(0, N - 1, nt).astype(np.int64)
         t = time.perf_counter()
-        tp = np.ascontiguousarray(c.pos[:, sel], np.float64)
-        tr = np.ascontiguousarray(c.rad[sel], np.float64)
-        d64 = lambda a: _lib.DeviceArray.from_host(np.ascontiguousarray(a, np.float64))
-        spos64, sstr64, srad64 = d64(c.pos), d64(c.str), d64(c.rad)
-        tpos64, trad64 = d64(tp), d64(tr)
-        v64 = _lib.DeviceArray.from_host(np.zeros((3, nt)))
-        g64 = _lib.DeviceArray.from_host(np.zeros((9, nt)))
-        s64, t64 = _lib.Points64(), _lib.Points64()
-        s64.n, t64.n = N, nt
-        for d in range(3):
-            s64.x[d], s64.s[d] = spos64.row(d), sstr64.row(d)
-            t64.x[d], t64.u[d] = tpos64.row(d), v64.row(d)
-        s64.r, t64.r = srad64.ptr, trad64.ptr
-        for k in range(9):
-            t64.ug[k] = g64.row(k)
-        o = _lib.make_opts("exp", True, "fp64")
-        _lib.check(L.o3d_points_affect_points64_dev(C.byref(s64), C.byref(t64), C.byref(o), None))
-        _lib.synchronize()
-        vref, gref = v64.to_host(), g64.to_host()
-        errors = {"reference": "FP64 kernel", "targets": nt,
+        vref, gref = fp64_subset(L, _lib, c, sel)
+        errors = {"reference": "FP64 kernel (k_p2p64, the reference's formulas in double)", "targets": nt,
                   "compensated": field_errors(vel_c[:, sel], grad_c[:, sel], vref, gref),
                   "plain_fp32": field_errors(vel_p[:, sel], grad_p[:, sel], vref, gref)}
+        if factored is not None:
+            factored["errors_vs_fp64"] = dict(field_errors(vel_f[:, sel], grad_f[:, sel], vref, gref), targets=nt)
+        plain["errors_vs_fp64"] = dict(errors["plain_fp32"], targets=nt)
         log(f"bench: FP64 subset ({nt} targets) in {time.perf_counter() - t:.1f}s: {errors}")
 
-    cpu = None
-    if not args.no_cpu:
-        import oracle
-        nth = oracle.lib().orc_max_threads()
-        if oracle.lib().orc_have_avx2():
-            # calibrate on a few targets, then run ~cpu_seconds worth
-            t = time.perf_counter()
-            oracle.p2p_vcstyle("exp", c.pos, c.str, c.rad, c.pos, c.rad, 0, 4 * nth, nth)
-            dt = time.perf_counter() - t
-            ntc = int(min(N, max(4 * nth, 4 * nth * args.cpu_seconds / max(dt, 1e-3))))
-            t = time.perf_counter()
-            oracle.p2p_vcstyle("exp", c.pos, c.str, c.rad, c.pos, c.rad, 0, ntc, nth)
-            dt = time.perf_counter() - t
-            cpu = {"value": round(ntc * float(N) / dt / 1e9, 4), "unit": "G interactions/s", "cores": nth,
-                   "kind": "port",
-                   "sample": f"Vc-style AVX2 restatement of Influence.h:388-442 (8 float lanes, rsqrt/rcp "
-                             f"estimates, OpenMP over targets), first {ntc} targets x all {N} sources, "
-                             f"{dt:.1f} s"}
-            # beside it, the reference's own scalar kernels (oracle/_ref: Kernels.h and
-            # CoreFunc.h compiled in place, EXP core, S = float, A = double: the
-            # non-Vc build) on a few targets of the same workload
-            try:
-                nr = 16 * nth
-                t = time.perf_counter()
-                oracle.ref_p2p("exp", c.pos, c.str, c.rad, c.pos[:, :nr], c.rad[:nr], True, "f32d")
-                rdt = time.perf_counter() - t
-                cpu["reference_scalar"] = {
-                    "value": round(nr * float(N) / rdt / 1e9, 4), "unit": "G interactions/s", "cores": nth,
-                    "kind": "reference", "sample": f"first {nr} targets x all {N} sources, {rdt:.1f} s"}
-            except FileNotFoundError:
-                cpu["reference_scalar"] = None
-            log(f"bench: cpu baseline {cpu}")
+    cpu = None if args.no_cpu else cpu_leg(args, c, vel_c, grad_c, log)
 
     from omega3d_amd import workloads
     single = workloads.single_ring_run(log=log)
     if not args.no_cpu:
-        # the same single ring on the host cores: the reference's scalar loop restated
-        # (float storage and accumulation) and the Vc-style loop
-        import oracle
-        p0 = scenes.ring(8192, sigma=0.05)
-        nth = oracle.lib().orc_max_threads()
-        t = time.perf_counter()
-        oracle.p2p("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, True, "f32f", nthreads=nth)
-        t_exact = time.perf_counter() - t
-        t_vc = None
-        if oracle.lib().orc_have_avx2():
-            t = time.perf_counter()
-            oracle.p2p_vcstyle("rm", p0.pos, p0.str, p0.rad, p0.pos, p0.rad, 0, p0.n, nth)
-            t_vc = time.perf_counter() - t
-        single["cpu"] = {"cores": nth, "scalar_float_ms": round(t_exact * 1e3, 2),
-                         "vc_style_ms": None if t_vc is None else round(t_vc * 1e3, 2),
-                         "kind": "port", "what": "the oracle's restatement of Influence.h:202-545 on the host "
-                                                 "cores (self pair included), full workload"}
-        log(f"bench: single ring on the CPU {single['cpu']}")
+        single["cpu"] = single_ring_cpu(single, log)
     batched = None if args.no_batch else workloads.batched(log=log)
     served = None if args.no_serve else workloads.served(duration=args.serve_seconds, log=log)
     panels = None if args.no_panels else workloads.panels_run(log=log)
@@ -413,34 +493,15 @@ def main():
     bem = None if args.no_bem else bem_leg(args, log)
     bem_rhs = None if args.no_bem else bem_rhs_leg(args, log)
 
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "ff_far_exp_n1m_traffic.json" if factored else "p2p_exp_n1m_traffic.json")
-    if os.path.exists(tfile):
-        traffic = json.load(open(tfile)).get("traffic_bytes")
-    executed = None
-    cfile = os.path.join(ROOT, "profiles", "ff_far_exp_n1m_counters.json")
-    if factored and os.path.exists(cfile):
-        executed = json.load(open(cfile))
-    if factored:
-        far_flops = FLOPS_PAIR * prof["far_pairs"]
-        achieved = far_flops / (prof["far_ms"] / 1e3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_ff_far<EXP,thick,compensated>", "kernel_ms": round(prof["far_ms"], 3),
-                "flops_per_launch": far_flops, "far_pair_fraction": round(prof["far_tile_fraction"], 5),
-                "stages_ms": {k: round(prof[k], 3) for k in ("prep_ms", "far_ms", "near_ms", "total_ms")},
-                "whole_call": {"achieved": round(call_tflops, 3), "frac": round(call_tflops / FP32_PEAK_TFLOPS, 4),
-                               "what": "reference flops of all N^2 pairs over the whole evaluation (sort, far, "
-                                       "near)"},
-                "note": "achieved counts the reference's 68 FP32 flops per far pair; the kernel executes an exact "
-                        "factorisation instead (about 8 VALU operations per pair plus BF16/FP16 split matrix products), "
-                        "so it can exceed the FP32 peak; 'executed' gives the utilisation of the units it runs on",
-                "executed": executed}
-    else:
-        roof = {"bound": "valu", "achieved": round(call_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(call_tflops / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
-                "kernel": "k_p2p<EXP,thick,grad,compensated>", "kernel_ms": round(float(kms.mean()), 3),
-                "flops_per_launch": flops}
+    traffic, tsrc = traffic_of("p2p_exp_n1m_traffic.json")
+    roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "traffic_source": tsrc,
+            "kernel": "k_p2p<EXP,thick,grad,compensated>", "kernel_ms": round(kavg, 3),
+            "flops_per_launch": flops,
+            "what": "the reference's count, N*(12 + 68 N) flops per launch (Influence.h:475, flops_0v_0bg of the "
+                    "EXP core), over the mean HIP-event time of the call on its stream (the call also sorts the "
+                    "targets into Morton order, about 0.1 ms); FP32 vector peak with packed FP32",
+            "counters": "profiles/p2p_exp_n1m_counters.md"}
 
     out = {
         "metric": "G interactions/s (vel+grad) at N=1M vs plain FP32; p50/p99 latency, mixed N",
@@ -456,19 +517,19 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": "two colliding vortex rings, N=1,048,576 (524,288 each), compact-exponential core, "
-                               "sigma 0.02, velocity + gradient at every particle, compensated FP32",
+                               "sigma 0.02, velocity + gradient at every particle, FP32 pairs with compensated "
+                               "accumulation, direct pair-by-pair sum",
                    "n": N, "core": "exp", "precision": "compensated", "self_term": "included",
-                   "method": "factored" if factored else "direct"},
-        "plain_fp32": {"value": round(plain_value, 3), "unit": "G interactions/s",
-                       "kernel_ms": round(float(pkms.mean()), 3)},
+                   "method": args.method},
+        "plain_fp32": plain,
         "latency_ms": ({"p50": round(served["p50_s"] * 1e3, 3), "p99": round(served["p99_s"] * 1e3, 3),
                         "what": "served RK2 step latency, mixed N (see served)"} if served else
                        {"p50": round(float(np.percentile(kms, 50)), 3),
                         "p99": round(float(np.percentile(kms, 99)), 3),
-                        "what": "per-step kernel time of this workload"}),
+                        "what": "per-step call time of this workload"}),
         "roofline": roof,
-        "direct_method": direct,
         "errors_vs_fp64": errors,
+        "factored": factored,
         "cpu_baseline": cpu,
         "single_ring": single,
         "batched": batched,

@@ -193,6 +193,20 @@ int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ,
 int o3d_points_affect_panels_dev(const o3d_points* src, const o3d_panels* targ, float* const vel[3],
                                  const o3d_opts* opts, o3d_stats* stats);
 
+/* panels_affect_panels (Influence.h:1225-1245): the velocity that the panels of src
+ * induce at colocation points of the panels of targ -- each panel's centroid
+ * pushed 1e-4 along its normal, as Surfaces::represent_as_particles(0.0001, -1.0)
+ * places them (Surfaces.h:959-1010) -- by panels_affect_points' recursive rule,
+ * velocity only (opts->results must be O3D_VELONLY; the reference asserts
+ * otherwise, :1227), FP32 or compensated.  As in the reference, which copies the
+ * temporary points' velocities over the target's (:1240-1244), vel[0..2][panel]
+ * is OVERWRITTEN, not accumulated.  targ->ts, area and ss are not read.  HOST
+ * arrays; the _dev form takes device pointers and runs on opts->stream. */
+int o3d_panels_affect_panels(const o3d_panels* src, const o3d_panels* targ, float* const vel[3],
+                             const o3d_opts* opts, o3d_stats* stats);
+int o3d_panels_affect_panels_dev(const o3d_panels* src, const o3d_panels* targ, float* const vel[3],
+                                 const o3d_opts* opts, o3d_stats* stats);
+
 /* ---- BEM right-hand side of one boundary surface (BEMHelper.h:83-117) ---- */
 typedef struct o3d_bem_rhs_opts {
   double fs[3];        /* freestream (Surfaces::finalize_vels, Surfaces.h:877-887)          */
@@ -269,6 +283,15 @@ typedef struct o3d_factored_profile {
 int o3d_factored_profile_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* opts,
                              o3d_factored_profile* out);
 
+/* Morton order of n points on DEVICE arrays (no counterpart in the reference: the
+ * order in which the factored method, and direct EXP calls of >= 32,768 targets,
+ * visit particles).  keys[i] receives point i's 30-bit Morton key (10 bits per
+ * axis on the points' cubic bounding box), perm[k] the index of the k-th point in
+ * key order; the sort is stable, so equal keys keep index order.  Enqueued on
+ * `stream` (kernels only); scratch is the stream's library workspace. */
+int o3d_morton_order_dev(const float* x, const float* y, const float* z, int64_t n, uint32_t* keys,
+                         int32_t* perm, void* stream);
+
 /* Double-precision collections (the reference's Points<double>): always run the
  * FP64 kernel, whatever opts->precision says.  Host and device variants. */
 typedef struct o3d_points64 {
@@ -338,11 +361,10 @@ int o3d_batch_run(o3d_batch* b, const o3d_points* all, void* stream);
 /* The same work as one ordinary call per instance, one after another on `stream`
  * (the comparison baseline). */
 int o3d_batch_run_separate(o3d_batch* b, const o3d_points* all, void* stream);
-/* Capture o3d_batch_run's queue part into a HIP graph (zero-fill kernel, queue
- * reset kernel, persistent kernel) and replay it; the graph reads the queue from
- * device memory.  A replay launches the graph, then enqueues the factored
- * instances (if any) on the same stream with the arrays given at capture: their
- * radix sorts record memory fills, which are kept out of graphs. */
+/* Capture all of o3d_batch_run into one HIP graph (zero-fill kernel, queue reset
+ * kernel, persistent kernel, and the factored instances' pipelines on their lanes,
+ * forked and joined by events) and replay it; the graph reads the queue from
+ * device memory and is bound to the arrays given at capture. */
 int o3d_batch_capture(o3d_batch* b, const o3d_points* all, void* stream);
 int o3d_batch_replay(o3d_batch* b, void* stream);
 /* Autotuner: time every (tile, tpt) candidate `reps` times on the real batch's

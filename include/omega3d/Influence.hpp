@@ -4,6 +4,8 @@
 // Same names, argument meaning and accumulate semantics as the reference:
 //   points_affect_points<S,A>(src, targ, restype, env)   src/Influence.h:67-551
 //   panels_affect_points<S,A>(src, targ, restype, env)   src/Influence.h:557-1099
+//   points_affect_panels<S,A>(src, targ, restype, env)   src/Influence.h:1107-1221
+//   panels_affect_panels<S,A>(src, targ, restype, env)   src/Influence.h:1225-1245
 //   InfluenceVisitor<S,A>                                 src/Influence.h:1251-1261
 //   ExecEnv, accel_t (plus gpu_hip)                       src/ExecEnv.h:26-123
 //   ResultsType, results_t                                src/ResultsType.h:27-95
@@ -143,13 +145,16 @@ class Points {
 };
 
 // SoA flat triangular panels: node positions, 3 node indices per panel, absolute
-// vortex strength per panel, area, optional sheet source strength.
+// vortex strength per panel, area, optional sheet source strength, and the
+// panel-centre velocities pu that get_vel() returns (Surfaces.h:239-240).
 template <class S>
 class Surfaces {
  public:
   Surfaces(std::array<std::vector<S>, 3> x, std::vector<uint32_t> idx, std::array<std::vector<S>, 3> ts,
            std::vector<S> area, std::vector<S> ss = {})
-      : m_x(std::move(x)), m_idx(std::move(idx)), m_ts(std::move(ts)), m_area(std::move(area)), m_ss(std::move(ss)) {}
+      : m_x(std::move(x)), m_idx(std::move(idx)), m_ts(std::move(ts)), m_area(std::move(area)), m_ss(std::move(ss)) {
+    for (auto& v : m_pu) v.assign(m_area.size(), S(0));
+  }
   size_t get_npanels() const { return m_area.size(); }
   const std::array<std::vector<S>, 3>& get_pos() const { return m_x; }
   const std::vector<uint32_t>& get_idx() const { return m_idx; }
@@ -157,12 +162,25 @@ class Surfaces {
   const std::vector<S>& get_area() const { return m_area; }
   bool have_src_str() const { return !m_ss.empty(); }
   const std::vector<S>& get_src_str() const { return m_ss; }
+  std::array<std::vector<S>, 3>& get_vel() { return m_pu; }
+  const std::array<std::vector<S>, 3>& get_vel() const { return m_pu; }
+
+  // Surfaces.h:868-887: panel-centre velocities zeroed, then pu = fs + pu/(4 pi)
+  void zero_vels() {
+    for (auto& v : m_pu) std::fill(v.begin(), v.end(), S(0));
+  }
+  void finalize_vels(const std::array<double, 3>& fs) {
+    const double factor = 0.25 / M_PI;
+    for (int d = 0; d < 3; ++d)
+      for (auto& v : m_pu[d]) v = (S)(fs[d] + v * factor);
+  }
 
  private:
   std::array<std::vector<S>, 3> m_x;
   std::vector<uint32_t> m_idx;
   std::array<std::vector<S>, 3> m_ts;
   std::vector<S> m_area, m_ss;
+  std::array<std::vector<S>, 3> m_pu;
 };
 
 namespace detail {
@@ -175,6 +193,20 @@ inline o3d_opts opts_of(const ResultsType& rt, const ExecEnv& env) {
   o.self_term = env.get_self_term();
   o.stream = env.get_stream();
   return o;
+}
+template <class S>
+o3d_panels panels_of(const Surfaces<S>& p) {
+  o3d_panels d{};
+  d.np = (int64_t)p.get_npanels();
+  d.nn = (int64_t)p.get_pos()[0].size();
+  for (int k = 0; k < 3; ++k) {
+    d.x[k] = p.get_pos()[k].data();
+    d.ts[k] = p.get_str()[k].data();
+  }
+  d.idx = p.get_idx().data();
+  d.area = p.get_area().data();
+  d.ss = p.have_src_str() ? p.get_src_str().data() : nullptr;
+  return d;
 }
 inline void check_env(const ResultsType& rt, const ExecEnv& env) {
   if (env.get_instrs() != gpu_hip) throw InfluenceError(O3D_EINVAL, "this library provides accel_t::gpu_hip only");
@@ -216,16 +248,7 @@ void panels_affect_points(const Surfaces<S>& src, Points<S>& targ, const Results
   static_assert(std::is_same<S, float>::value, "libo3dbiot.so stores float");
   detail::check_env(restype, env);
   auto& tug = targ.get_velgrad();
-  o3d_panels p{};
-  p.np = (int64_t)src.get_npanels();
-  p.nn = (int64_t)src.get_pos()[0].size();
-  for (int d = 0; d < 3; ++d) {
-    p.x[d] = src.get_pos()[d].data();
-    p.ts[d] = src.get_str()[d].data();
-  }
-  p.idx = src.get_idx().data();
-  p.area = src.get_area().data();
-  p.ss = src.have_src_str() ? src.get_src_str().data() : nullptr;
+  const o3d_panels p = detail::panels_of(src);
   o3d_points t{};
   t.n = (int64_t)targ.get_n();
   for (int d = 0; d < 3; ++d) {
@@ -239,6 +262,49 @@ void panels_affect_points(const Surfaces<S>& src, Points<S>& targ, const Results
   check(o3d_panels_affect_points(&p, &t, &o, nullptr), "panels_affect_points");
 }
 
+namespace detail {
+inline void check_velonly(const ResultsType& rt, const char* what) {
+  if (rt.compute_grad())
+    throw InfluenceError(O3D_EINVAL, std::string(what) + ": velocity gradients are not computed on panels "
+                                                         "(the reference asserts, Influence.h:1111,1227)");
+}
+}  // namespace detail
+
+// Influence.h:1107-1221: the particles' velocity at each target panel's centre,
+// SUBTRACTED from the panel velocities (the panel kernel run backwards)
+template <class S, class A>
+void points_affect_panels(const Points<S>& src, Surfaces<S>& targ, const ResultsType& restype, const ExecEnv& env) {
+  static_assert(std::is_same<S, float>::value, "libo3dbiot.so stores float");
+  detail::check_env(restype, env);
+  detail::check_velonly(restype, "points_affect_panels");
+  o3d_points s{};
+  s.n = (int64_t)src.get_n();
+  for (int d = 0; d < 3; ++d) {
+    s.x[d] = src.get_pos()[d].data();
+    s.s[d] = src.get_str()[d].data();
+  }
+  s.r = src.is_inert() ? nullptr : src.get_rad().data();
+  const o3d_panels t = detail::panels_of(targ);
+  float* vel[3] = {targ.get_vel()[0].data(), targ.get_vel()[1].data(), targ.get_vel()[2].data()};
+  const o3d_opts o = detail::opts_of(restype, env);
+  check(o3d_points_affect_panels(&s, &t, vel, &o, nullptr), "points_affect_panels");
+}
+
+// Influence.h:1225-1245: src's velocity at targ's colocation points (centroids
+// pushed 1e-4 along the normals); like the reference, the result REPLACES the
+// target's panel velocities
+template <class S, class A>
+void panels_affect_panels(const Surfaces<S>& src, Surfaces<S>& targ, const ResultsType& restype, const ExecEnv& env) {
+  static_assert(std::is_same<S, float>::value, "libo3dbiot.so stores float");
+  detail::check_env(restype, env);
+  detail::check_velonly(restype, "panels_affect_panels");
+  const o3d_panels s = detail::panels_of(src);
+  const o3d_panels t = detail::panels_of(targ);
+  float* vel[3] = {targ.get_vel()[0].data(), targ.get_vel()[1].data(), targ.get_vel()[2].data()};
+  const o3d_opts o = detail::opts_of(restype, env);
+  check(o3d_panels_affect_panels(&s, &t, vel, &o, nullptr), "panels_affect_panels");
+}
+
 template <class S>
 using Collection = std::variant<Points<S>, Surfaces<S>>;
 
@@ -249,12 +315,8 @@ struct InfluenceVisitor {
   ExecEnv env;
   void operator()(const Points<S>& src, Points<S>& targ) { points_affect_points<S, A>(src, targ, restype, env); }
   void operator()(const Surfaces<S>& src, Points<S>& targ) { panels_affect_points<S, A>(src, targ, restype, env); }
-  void operator()(const Points<S>&, Surfaces<S>&) {
-    throw InfluenceError(O3D_EINVAL, "points_affect_panels is not part of this path");
-  }
-  void operator()(const Surfaces<S>&, Surfaces<S>&) {
-    throw InfluenceError(O3D_EINVAL, "panels_affect_panels is not part of this path");
-  }
+  void operator()(const Points<S>& src, Surfaces<S>& targ) { points_affect_panels<S, A>(src, targ, restype, env); }
+  void operator()(const Surfaces<S>& src, Surfaces<S>& targ) { panels_affect_panels<S, A>(src, targ, restype, env); }
 };
 
 }  // namespace o3d

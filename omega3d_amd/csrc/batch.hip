@@ -13,9 +13,9 @@
 // least-loaded lane), each lane with scratch of its own.  The lanes fork from the
 // caller's stream after the zero fill and join it at the end, so they overlap
 // one another and the persistent kernel: an instance of 65,536 particles has
-// only 256 far-field workgroups, half the GPU's slots.  The pipeline's sorts
-// record memory fills, which a graph must not hold, so a captured graph covers
-// the queue part and a replay enqueues the factored instances after it.
+// only 256 far-field workgroups, half the GPU's slots.  Every step of a run is a
+// kernel launch or an event (no memory fills, no allocation), so the whole run,
+// lanes included, is captured into one HIP graph.
 #include "o3d_api_internal.hpp"
 #include "p2p_kernels.hpp"
 
@@ -506,7 +506,7 @@ int o3d_batch_capture(o3d_batch* b, const o3d_points* all, void* stream) {
   drop_graph(b);
   hipStream_t st = (hipStream_t)stream;
   HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-  rc = enqueue_queue(b, all, st);
+  rc = enqueue_run(b, all, st);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(st, &g);
   if (rc != O3D_OK) { if (g) hipGraphDestroy(g); return rc; }
@@ -522,8 +522,7 @@ int o3d_batch_replay(o3d_batch* b, void* stream) {
   clear_error();
   if (!b || !b->exec) return fail(O3D_EINVAL, "no captured graph");
   HIPCHK(hipGraphLaunch(b->exec, (hipStream_t)stream));
-  const int rc = fork_factored(b, &b->cap_all, (hipStream_t)stream);
-  return rc != O3D_OK ? rc : join_factored(b, (hipStream_t)stream);
+  return O3D_OK;
 }
 
 int o3d_batch_tune(o3d_batch* b, const o3d_points* all, void* stream, const int32_t* tiles, int32_t ntiles,

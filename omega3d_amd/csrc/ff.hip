@@ -1,7 +1,7 @@
 // ff.hip -- host side of the factored far-field evaluation (method "factored").
 //
 // One call enqueues, on the caller's stream and without synchronising:
-//   bounding box -> Morton keys -> radix sort (hipcub) of sources (and of targets
+//   bounding box -> Morton keys -> radix sort (sort.hip) of sources (and of targets
 //   when they are a different collection) -> gather into sorted float4 rows ->
 //   bounding spheres of source tiles and target blocks -> far/near bit per
 //   (block, tile) -> far-field kernel (matrix cores) -> near-field kernel, which
@@ -10,8 +10,6 @@
 // ordered, so they can share it); it grows when a larger call comes, after the
 // stream has drained.  A caller that owns its scratch (a batch) passes it in
 // instead, sized by ff_workspace_bytes.
-#include <hipcub/hipcub.hpp>
-
 #include "ff_kernels.hpp"
 #include "o3d_api_internal.hpp"
 
@@ -64,19 +62,9 @@ bool same_points(const o3d_points* src, const o3d_points* targ) {
   return src->n == targ->n && src->x[0] == targ->x[0] && src->x[1] == targ->x[1] && src->x[2] == targ->x[2];
 }
 
-// hipcub scratch for the sorts and scans of one call
-size_t lib_tmp_bytes(int64_t ns, int64_t nt, bool same, hipStream_t st) {
-  const int64_t nb = (nt + FF_TB - 1) / FF_TB;
-  size_t tb_s = 0, tb_t = 0, tb_scan = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb_s, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                         (const int*)nullptr, (int*)nullptr, (int)ns, 0, 30, st) != hipSuccess)
-    return 0;
-  if (!same && hipcub::DeviceRadixSort::SortPairs(nullptr, tb_t, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                                  (const int*)nullptr, (int*)nullptr, (int)nt, 0, 30, st) != hipSuccess)
-    return 0;
-  if (hipcub::DeviceScan::ExclusiveSum(nullptr, tb_scan, (const int*)nullptr, (int*)nullptr, (int)nb, st) != hipSuccess)
-    return 0;
-  return std::max(std::max(tb_s, tb_t), tb_scan);
+// scratch of the radix sorts of one call (the scans need none)
+size_t lib_tmp_bytes(int64_t ns, int64_t nt, bool same) {
+  return radix_tmp_bytes(same ? ns : std::max(ns, nt));
 }
 
 // the workspace's sub-arrays in order; with base == nullptr only .bytes is meaningful
@@ -140,8 +128,7 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
   const int ntiles = (int)((ns + T - 1) / T);
   const int mwords = (ntiles + 31) / 32;
 
-  const size_t tmp_bytes = lib_tmp_bytes(ns, nt, same, st);
-  if (tmp_bytes == 0) return fail(O3D_EDEVICE, "factored path: sort/scan scratch query failed");
+  const size_t tmp_bytes = lib_tmp_bytes(ns, nt, same);
 
   Layout L = layout(nullptr, ns, nt, same, T, tmp_bytes);
   char* base = static_cast<char*>(ws);
@@ -173,17 +160,21 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
                        L.bb);
   hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(ns, 256)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, L.bb, L.ks_in, L.is_in);
   HIPCHK(hipGetLastError());
-  size_t tb = tmp_bytes;
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(L.tmp, tb, L.ks_in, L.ks_out, L.is_in, L.is_out, (int)ns, 0, 30, st));
-  const int* tperm = L.is_out;
+  unsigned* skeys = nullptr;
+  int* sperm = nullptr;
+  int rc = radix_sort_pairs(L.ks_in, L.is_in, L.ks_out, L.is_out, ns, 30, L.tmp, st, &skeys, &sperm);
+  if (rc != O3D_OK) return rc;
+  const int* tperm = sperm;
   if (!same) {
     hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(nt, 256)), dim3(256), 0, st, tv.x, tv.y, tv.z, nt, L.bb, L.kt_in, L.it_in);
     HIPCHK(hipGetLastError());
-    tb = tmp_bytes;
-    HIPCHK(hipcub::DeviceRadixSort::SortPairs(L.tmp, tb, L.kt_in, L.kt_out, L.it_in, L.it_out, (int)nt, 0, 30, st));
-    tperm = L.it_out;
+    unsigned* tkeys = nullptr;
+    int* tp = nullptr;
+    if ((rc = radix_sort_pairs(L.kt_in, L.it_in, L.kt_out, L.it_out, nt, 30, L.tmp, st, &tkeys, &tp)) != O3D_OK)
+      return rc;
+    tperm = tp;
   }
-  hipLaunchKernelGGL((k_ff_gather_src<CORE>), dim3(grid_of(ns, 256)), dim3(256), 0, st, sv, L.is_out, ns, L.sp, L.sa);
+  hipLaunchKernelGGL((k_ff_gather_src<CORE>), dim3(grid_of(ns, 256)), dim3(256), 0, st, sv, sperm, ns, L.sp, L.sa);
   hipLaunchKernelGGL((k_ff_gather_tgt<CORE, THICK>), dim3(grid_of(nt, 256)), dim3(256), 0, st, tv, tperm, nt, L.tq);
   hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)ntiles), dim3(256), 0, st, L.sp, L.sa, ns, T, L.tsph, L.tcs, L.tam);
   hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)nb), dim3(256), 0, st, L.tq, (const float4*)nullptr, nt, FF_TB, L.bsph,
@@ -192,12 +183,10 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
                      L.tcs, ntiles, mwords, L.mask);
   hipLaunchKernelGGL(k_ff_plan_count, dim3(grid_of(nb, 256)), dim3(256), 0, st, L.mask, nb, mwords, ntiles, L.nn);
   HIPCHK(hipGetLastError());
-  tb = tmp_bytes;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(L.tmp, tb, L.nn, L.noff, (int)nb, st));
+  if ((rc = exclusive_scan_i32(L.nn, L.noff, nb, st)) != O3D_OK) return rc;
   hipLaunchKernelGGL(k_ff_plan_items, dim3(grid_of(nb, 256)), dim3(256), 0, st, L.nn, L.noff, nb, L.items, L.mfold);
   HIPCHK(hipGetLastError());
-  tb = tmp_bytes;
-  HIPCHK(hipcub::DeviceScan::ExclusiveSum(L.tmp, tb, L.items, L.ioff, (int)nb, st));
+  if ((rc = exclusive_scan_i32(L.items, L.ioff, nb, st)) != O3D_OK) return rc;
   hipLaunchKernelGGL(k_ff_zero_u32, dim3(1), dim3(64), 0, st, L.ctr);
   HIPCHK(hipGetLastError());
 
@@ -259,17 +248,13 @@ int run_core(const o3d_points* src, o3d_points* targ, bool thick, int prec, bool
 // Morton order of n points (the direct kernel's target order): workspace bytes,
 // and the order itself, written into ws (bounding box, keys, indices and the
 // sort's scratch; *perm points into ws)
-size_t point_order_bytes(int64_t n, hipStream_t st) {
-  size_t tb = 0;
-  if (hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                         (const int*)nullptr, (int*)nullptr, (int)n, 0, 30, st) != hipSuccess)
-    return 0;
+size_t point_order_bytes(int64_t n) {
   const size_t a = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
-  return 256 + 4 * a + ((tb + 255) & ~(size_t)255);
+  return 256 + 4 * a + ((radix_tmp_bytes(n) + 255) & ~(size_t)255);
 }
 
 int point_order(const float* x, const float* y, const float* z, int64_t n, void* ws, hipStream_t st,
-                const int32_t** perm) {
+                const int32_t** perm, const unsigned** keys) {
   const size_t a = (sizeof(int32_t) * (size_t)n + 255) & ~(size_t)255;
   char* p = (char*)ws;
   unsigned* bb = (unsigned*)p;
@@ -278,16 +263,20 @@ int point_order(const float* x, const float* y, const float* z, int64_t n, void*
   int* iin = (int*)(p + 256 + 2 * a);
   int* iout = (int*)(p + 256 + 3 * a);
   void* tmp = p + 256 + 4 * a;
-  size_t tb = 0;
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (const unsigned*)nullptr, (unsigned*)nullptr,
-                                            (const int*)nullptr, (int*)nullptr, (int)n, 0, 30, st));
   hipLaunchKernelGGL(k_ff_bbox_init, dim3(1), dim3(64), 0, st, bb);
   hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(n, 256), 2048u)), dim3(256), 0, st, x, y, z, n, bb);
   hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(n, 256)), dim3(256), 0, st, x, y, z, n, bb, kin, iin);
-  HIPCHK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, iin, iout, (int)n, 0, 30, st));
-  *perm = iout;
+  HIPCHK(hipGetLastError());
+  unsigned* sk = nullptr;
+  int* sp = nullptr;
+  const int rc = radix_sort_pairs(kin, iin, kout, iout, n, 30, tmp, st, &sk, &sp);
+  if (rc != O3D_OK) return rc;
+  *perm = sp;
+  if (keys) *keys = sk;
   return O3D_OK;
 }
+
+bool ff_same_points(const o3d_points* src, const o3d_points* targ) { return same_points(src, targ); }
 
 std::mutex& stream_ws_mutex() { return g_ws_mu; }
 
@@ -387,9 +376,30 @@ int p2p_factored_ws(const o3d_points* src, o3d_points* targ, const o3d_opts* o, 
 
 size_t ff_workspace_bytes(int64_t ns, int64_t nt, bool same, int tile) {
   const int T = tile > 0 ? tile : 256;
-  const size_t tmp = lib_tmp_bytes(ns, nt, same, nullptr);
-  if (tmp == 0) return 0;
-  return layout(nullptr, ns, nt, same, T, tmp).bytes;
+  return layout(nullptr, ns, nt, same, T, lib_tmp_bytes(ns, nt, same)).bytes;
 }
 
 }  // namespace o3d
+
+extern "C" int o3d_morton_order_dev(const float* x, const float* y, const float* z, int64_t n, uint32_t* keys,
+                                    int32_t* perm, void* stream) {
+  using namespace o3d;
+  clear_error();
+  int rc = ensure_init();
+  if (rc != O3D_OK) return rc;
+  if (n < 0 || n > INT32_MAX) return fail(O3D_EINVAL, "morton order: n %lld out of range", (long long)n);
+  if (n == 0) return O3D_OK;
+  if (!x || !y || !z || !keys || !perm) return fail(O3D_EINVAL, "morton order: NULL array");
+  hipStream_t st = (hipStream_t)stream;
+  std::lock_guard<std::mutex> lk(g_ws_mu);
+  void* ws = nullptr;
+  if ((rc = stream_workspace(st, point_order_bytes(n), &ws)) != O3D_OK) return rc;
+  const int32_t* p = nullptr;
+  if ((rc = point_order(x, y, z, n, ws, st, &p)) != O3D_OK) return rc;
+  HIPCHK(hipMemcpyAsync(perm, p, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToDevice, st));
+  // the keys in index order, from the same box (point_order left it at the workspace's start)
+  hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(n, 256)), dim3(256), 0, st, x, y, z, n, (const unsigned*)ws,
+                     (unsigned*)keys, (int*)nullptr);
+  HIPCHK(hipGetLastError());
+  return O3D_OK;
+}

@@ -145,7 +145,7 @@ __global__ __launch_bounds__(256) void k_ff_keys(const float* x, const float* y,
     k |= spread10((unsigned)c) << d;
   }
   keys[i] = k;
-  idx[i] = (int)i;
+  if (idx) idx[i] = (int)i;
 }
 
 template <int CORE>

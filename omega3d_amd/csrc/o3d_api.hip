@@ -247,8 +247,16 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   if (o->precision == PREC_FP64) return fail(O3D_EINVAL, "internal: FP64 routed to p2p_dev");
   if (o->method == O3D_METHOD_FACTORED) return p2p_factored(src, targ, o, st);
   if (o->method == O3D_METHOD_AUTO && !tun && o->tile == 0 && o->tpt == 0 && ff_supported(o->core, grad, o->precision) &&
-      src->n >= FF_AUTO_MIN_N && targ->n >= FF_AUTO_MIN_N && src->n <= INT32_MAX && targ->n <= INT32_MAX)
-    return p2p_factored(src, targ, o, st);
+      src->n >= FF_AUTO_MIN_N && targ->n >= FF_AUTO_MIN_N && src->n <= INT32_MAX && targ->n <= INT32_MAX) {
+    // under graph capture the stream's workspace cannot grow: when it is too small
+    // for the factored method, the direct kernel (which needs none) takes the call
+    bool ff = true;
+    if (stream_capturing(st)) {
+      std::lock_guard<std::mutex> lk(stream_ws_mutex());
+      ff = stream_workspace_fits(st, ff_workspace_bytes(src->n, targ->n, ff_same_points(src, targ), 0));
+    }
+    if (ff) return p2p_factored(src, targ, o, st);
+  }
   P2PArgs a{};
   a.s = SrcView{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
   a.t.x = targ->x[0]; a.t.y = targ->x[1]; a.t.z = targ->x[2]; a.t.r = targ->r;
@@ -288,19 +296,21 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   // EXP core: the kernel takes the exponential branch for a whole wave when any of
   // its targets needs it for the current source, so large calls visit the targets
   // in Morton order, where a wave's targets are neighbours.  A target's sums do
-  // not depend on the order: the results are the same bits.  Not under graph
-  // capture (the sort records memory fills).
-  if (o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX && !stream_capturing(st)) {
+  // not depend on the order: the results are the same bits.  Under graph capture
+  // the order needs the stream's workspace to be large enough already; otherwise
+  // the targets are visited in index order.
+  if (o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX) {
     std::lock_guard<std::mutex> lk(stream_ws_mutex());
-    const size_t bytes = point_order_bytes(a.nt, st);
-    if (bytes == 0) return fail(O3D_EDEVICE, "sort sizing failed");
-    void* ws = nullptr;
-    int rc = stream_workspace(st, bytes, &ws);
-    if (rc != O3D_OK) return rc;
-    if ((rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ws, st, &a.tperm)) != O3D_OK) return rc;
-    hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
-    if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
-    return O3D_OK;
+    const size_t bytes = point_order_bytes(a.nt);
+    if (!stream_capturing(st) || stream_workspace_fits(st, bytes)) {
+      void* ws = nullptr;
+      int rc = stream_workspace(st, bytes, &ws);
+      if (rc != O3D_OK) return rc;
+      if ((rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ws, st, &a.tperm)) != O3D_OK) return rc;
+      hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
+      if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
+      return O3D_OK;
+    }
   }
   hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));

@@ -35,6 +35,8 @@ int p2p_factored(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hip
 int p2p_factored_ws(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, void* ws,
                     size_t ws_bytes);
 size_t ff_workspace_bytes(int64_t ns, int64_t nt, bool same, int tile);
+// sources and targets are one collection (the factored method sorts it once)
+bool ff_same_points(const o3d_points* src, const o3d_points* targ);
 // one growable device workspace per stream (calls on a stream are ordered, so they
 // share it); hold stream_ws_mutex() from stream_workspace until the last enqueue
 std::mutex& stream_ws_mutex();
@@ -45,10 +47,20 @@ bool stream_capturing(hipStream_t st);
 // frees st's workspace (before the stream is destroyed); takes stream_ws_mutex()
 void release_stream_workspace(hipStream_t st);
 bool stream_workspace_fits(hipStream_t st, size_t bytes);
-// Morton order of n points into a workspace of point_order_bytes(n) (0: error)
-size_t point_order_bytes(int64_t n, hipStream_t st);
+// Morton order of n points into a workspace of point_order_bytes(n): *perm gets the
+// permutation (slot -> point index), *keys (optional) the sorted 30-bit keys
+size_t point_order_bytes(int64_t n);
 int point_order(const float* x, const float* y, const float* z, int64_t n, void* ws, hipStream_t st,
-                const int32_t** perm);
+                const int32_t** perm, const unsigned** keys = nullptr);
+// stable radix sort of (key, value) pairs by the low `bits` bits of the keys
+// (sort.hip): (ka, va) and (kb, vb) are ping-pong buffers of n entries, both
+// overwritten; *keys / *vals name the one that holds the result.  tmp holds
+// radix_tmp_bytes(n).  Kernels only: capturable into a graph.
+size_t radix_tmp_bytes(int64_t n);
+int radix_sort_pairs(unsigned* ka, int* va, unsigned* kb, int* vb, int64_t n, int bits, void* tmp, hipStream_t st,
+                     unsigned** keys, int** vals);
+// exclusive prefix sum of n int32 values (one workgroup; meant for small n)
+int exclusive_scan_i32(const int* in, int* out, int64_t n, hipStream_t st);
 // a collection large enough for the factored method to pay when the caller leaves the choice open
 constexpr int64_t FF_AUTO_MIN_N = 65536;
 // direct EXP calls with at least this many targets visit them in Morton order

@@ -31,6 +31,30 @@
 
 namespace o3d {
 
+// Surfaces::compute_bases (Surfaces.h:766-816) for the panel with nodes id0, id1,
+// id2 of xs: x1 along edge 0-1, x2 toward node 2 and orthogonal to x1, nm = x1 x x2;
+// the scalings by 1/base and 1/height in double, stored float; contraction off.
+__device__ __forceinline__ void panel_basis_f(const float* const xs[3], uint32_t id0, uint32_t id1, uint32_t id2,
+                                              float* x1, float* x2, float* nm) {
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x1[j] = xs[j][id1] - xs[j][id0];
+  const float base = sqrtf(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x1[j] = (float)((double)x1[j] * (1.0 / (double)base));
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x2[j] = xs[j][id2] - xs[j][id0];
+  const float dp = x2[0] * x1[0] + x2[1] * x1[1] + x2[2] * x1[2];
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x2[j] = x2[j] - dp * x1[j];
+  const float height = sqrtf(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) x2[j] = (float)((double)x2[j] * (1.0 / (double)height));
+  nm[0] = x1[1] * x2[2] - x1[2] * x2[1];
+  nm[1] = x1[2] * x2[0] - x1[0] * x2[2];
+  nm[2] = x1[0] * x2[1] - x1[1] * x2[0];
+}
+
 struct PanArgs {
   const float* nx; const float* ny; const float* nz;    // nodes
   const uint32_t* idx;                                  // 3 per panel

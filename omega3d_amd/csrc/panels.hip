@@ -83,15 +83,18 @@ static hipError_t launch_pan_g7(PanArgs a, bool grad, int prec, bool near, hipSt
 // the near-target list, its counter and f4_per_panel float4 of geometry per panel
 // in the stream's workspace (caller holds stream_ws_mutex() until the launches are
 // enqueued)
-static int g7_scratch(PanArgs* a, hipStream_t st, float4** geom, int f4_per_panel = 1) {
+static int g7_scratch(PanArgs* a, hipStream_t st, float4** geom, int f4_per_panel = 1, size_t extra = 0,
+                      void** tail = nullptr) {
   if (a->nt > INT32_MAX) return fail(O3D_EINVAL, "the panel kernels take at most %d targets", INT32_MAX);
   void* ws = nullptr;
   const size_t list = (sizeof(int32_t) * (size_t)a->nt + 255) & ~(size_t)255;
-  const int rc = stream_workspace(st, 256 + list + sizeof(float4) * f4_per_panel * (size_t)a->np, &ws);
+  const size_t gbytes = (sizeof(float4) * f4_per_panel * (size_t)a->np + 255) & ~(size_t)255;
+  const int rc = stream_workspace(st, 256 + list + gbytes + extra, &ws);
   if (rc != O3D_OK) return rc;
   a->near_count = (unsigned int*)ws;
   a->near_list = (int32_t*)((char*)ws + 256);
   *geom = (float4*)((char*)ws + 256 + list);
+  if (tail) *tail = (char*)ws + 256 + list + gbytes;
   return O3D_OK;
 }
 
@@ -185,6 +188,94 @@ static int pan_dev(const o3d_panels* p, o3d_points* t, const o3d_opts* o, int pr
   if ((rc = g7_scratch(&a, st, &rec, PAN_F4 + 1)) != O3D_OK) return rc;
   const hipError_t e = launch_pan(a, grad, prec, st, rec);
   if (e != hipSuccess) return fail(O3D_EDEVICE, "panel launch failed: %s", hipGetErrorString(e));
+  return O3D_OK;
+}
+
+// Colocation points of a surface as Surfaces::represent_as_particles(offset, ips)
+// places them (Surfaces.h:959-1010): the centroid, the float sum of the three nodes
+// times (1./3.) in double, stored float; then pushed out by offset along the
+// panel's normal from compute_bases (Surfaces.h:766-816), in float.
+__global__ __launch_bounds__(BLOCK) void k_colocation(const float* nx, const float* ny, const float* nz,
+                                                      const uint32_t* idx, int64_t np, float offset, float* px,
+                                                      float* py, float* pz) {
+#pragma clang fp contract(off)
+  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
+  if (i >= np) return;
+  const uint32_t id0 = idx[3 * i], id1 = idx[3 * i + 1], id2 = idx[3 * i + 2];
+  const float* xs[3] = {nx, ny, nz};
+  float c[3], x1[3], x2[3], nm[3];
+  panel_basis_f(xs, id0, id1, id2, x1, x2, nm);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    c[j] = (float)((1. / 3.) * (double)(xs[j][id0] + xs[j][id1] + xs[j][id2]));
+    c[j] = c[j] + offset * nm[j];
+  }
+  px[i] = c[0];
+  py[i] = c[1];
+  pz[i] = c[2];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_zero3(float* a, float* b, float* c, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * BLOCK;
+  for (int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x; i < n; i += stride) {
+    a[i] = 0.0f; b[i] = 0.0f; c[i] = 0.0f;
+  }
+}
+
+// the offset Influence.h:1233 asks of represent_as_particles
+constexpr float COLOCATION_OFFSET = 0.0001f;
+
+// panels_affect_panels (Influence.h:1225-1245) on device arrays: the target
+// surface's colocation points, zeroed velocities (the reference copies the
+// temporary points' velocities over the target's, :1240-1244), then the recursive
+// panel rule of panels_affect_points, velocity only.
+static int pap_dev(const o3d_panels* src, const o3d_panels* targ, float* const vel[3], const o3d_opts* o,
+                   hipStream_t st) {
+  const int64_t nt = targ->np;
+  if (nt <= 0) return O3D_OK;
+  const unsigned g = (unsigned)((nt + BLOCK - 1) / BLOCK);
+  hipLaunchKernelGGL(k_zero3, dim3(std::min(g, 4096u)), dim3(BLOCK), 0, st, vel[0], vel[1], vel[2], nt);
+  HIPCHK(hipGetLastError());
+  if (src->np <= 0) return O3D_OK;
+  int ml, rc;
+  if ((rc = maxlev_of(o, &ml)) != O3D_OK) return rc;
+  o3d_points t{};
+  t.n = nt;
+  for (int d = 0; d < 3; ++d) t.u[d] = vel[d];
+  PanArgs a = pan_args(src, &t, false, nullptr);
+  a.tile = o->tile > 0 ? o->tile : 128;
+  a.maxlev = ml;
+  std::lock_guard<std::mutex> lk(stream_ws_mutex());
+  float4* rec = nullptr;
+  void* tail = nullptr;
+  if ((rc = g7_scratch(&a, st, &rec, PAN_F4 + 1, sizeof(float) * 3 * (size_t)nt, &tail)) != O3D_OK) return rc;
+  float* px = (float*)tail;
+  a.tx = px; a.ty = px + nt; a.tz = px + 2 * nt;
+  hipLaunchKernelGGL(k_colocation, dim3(g), dim3(BLOCK), 0, st, targ->x[0], targ->x[1], targ->x[2], targ->idx, nt,
+                     COLOCATION_OFFSET, px, px + nt, px + 2 * nt);
+  HIPCHK(hipGetLastError());
+  const hipError_t e = launch_pan(a, false, o->precision, st, rec);
+  if (e != hipSuccess) return fail(O3D_EDEVICE, "panel launch failed: %s", hipGetErrorString(e));
+  return O3D_OK;
+}
+
+static int check_pap(const o3d_panels* src, const o3d_panels* targ, float* const vel[3], const o3d_opts* o) {
+  int rc;
+  if ((rc = check_panels(src)) != O3D_OK) return rc;
+  if (!targ) return fail(O3D_EINVAL, "target panels are NULL");
+  if (targ->np < 0 || targ->nn < 0) return fail(O3D_EINVAL, "negative target panel or node count");
+  if (targ->np > 0) {
+    if (targ->nn <= 0 || !targ->idx) return fail(O3D_EINVAL, "target panels without nodes or indices");
+    for (int d = 0; d < 3; ++d)
+      if (!targ->x[d]) return fail(O3D_EINVAL, "target node component %d is NULL", d);
+    if (!vel) return fail(O3D_EINVAL, "vel is NULL");
+    for (int d = 0; d < 3; ++d)
+      if (!vel[d]) return fail(O3D_EINVAL, "velocity component %d is NULL", d);
+  }
+  if (o->results != O3D_VELONLY)
+    return fail(O3D_EINVAL, "panels_affect_panels computes velocities only (the reference asserts, Influence.h:1227)");
+  if (o->precision == PREC_FP64) return fail(O3D_EINVAL, "panels_affect_panels runs in FP32 or compensated precision");
+  if (o->method == O3D_PANEL_GAUSS7) return fail(O3D_EINVAL, "panels_affect_panels uses the recursive panel rule");
   return O3D_OK;
 }
 
@@ -360,6 +451,73 @@ int o3d_panels_g7_profile_dev(const o3d_panels* src, o3d_points* targ, const o3d
   out->pairs = (double)nt * (double)np;
   out->near_pairs = (double)nnear;
   out->far_pairs = out->pairs - out->near_pairs;
+  return O3D_OK;
+}
+
+int o3d_panels_affect_panels_dev(const o3d_panels* src, const o3d_panels* targ, float* const vel[3],
+                                 const o3d_opts* opts, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(opts)) != O3D_OK) return rc;
+  if ((rc = check_pap(src, targ, vel, opts)) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  if ((rc = pap_dev(src, targ, vel, opts, (hipStream_t)opts->stream)) != O3D_OK) return rc;
+  fill_stats(stats, 0.0, pan_flops(false, targ->np, src->np, false), (double)targ->np * (double)src->np);
+  return O3D_OK;
+}
+
+int o3d_panels_affect_panels(const o3d_panels* src, const o3d_panels* targ, float* const vel[3],
+                             const o3d_opts* opts, o3d_stats* stats) {
+  clear_error();
+  int rc;
+  if ((rc = check_opts(opts)) != O3D_OK) return rc;
+  if ((rc = check_pap(src, targ, vel, opts)) != O3D_OK) return rc;
+  if (src->np > 0 && (rc = check_node_indices(src, "src")) != O3D_OK) return rc;
+  if (targ->np > 0 && (rc = check_node_indices(targ, "targ")) != O3D_OK) return rc;
+  if ((rc = ensure_init()) != O3D_OK) return rc;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int64_t np = src->np, nn = src->nn, nt = targ->np, tn = targ->nn;
+  if (nt > 0) {
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    hipStream_t st = lib_stream();
+    void *dp, *dt;
+    // source panels: nodes 3*nn, ts 3*np, area np, ss np (floats) + idx 3*np (uint32);
+    // target panels: nodes 3*tn (floats) + idx 3*nt (uint32), velocities 3*nt (floats)
+    if ((rc = pool_buffer(2, sizeof(float) * (3 * nn + 8 * np) + 256, &dp)) != O3D_OK) return rc;
+    if ((rc = pool_buffer(3, sizeof(float) * (3 * tn + 6 * nt) + 256, &dt)) != O3D_OK) return rc;
+    float* f = (float*)dp;
+    o3d_panels d{};
+    d.np = np; d.nn = nn;
+    for (int k = 0; k < 3; ++k) { d.x[k] = f + k * nn; d.ts[k] = f + 3 * nn + k * np; }
+    d.area = f + 3 * nn + 3 * np;
+    d.ss = src->ss ? f + 3 * nn + 4 * np : nullptr;
+    d.idx = (const uint32_t*)(f + 3 * nn + 5 * np);
+    if (np > 0) {
+      for (int k = 0; k < 3; ++k) {
+        HIPCHK(hipMemcpyAsync((void*)d.x[k], src->x[k], sizeof(float) * nn, hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync((void*)d.ts[k], src->ts[k], sizeof(float) * np, hipMemcpyHostToDevice, st));
+      }
+      HIPCHK(hipMemcpyAsync((void*)d.area, src->area, sizeof(float) * np, hipMemcpyHostToDevice, st));
+      if (src->ss) HIPCHK(hipMemcpyAsync((void*)d.ss, src->ss, sizeof(float) * np, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync((void*)d.idx, src->idx, sizeof(uint32_t) * 3 * np, hipMemcpyHostToDevice, st));
+    }
+    float* g = (float*)dt;
+    o3d_panels dtg{};
+    dtg.np = nt; dtg.nn = tn;
+    for (int k = 0; k < 3; ++k) dtg.x[k] = g + k * tn;
+    dtg.idx = (const uint32_t*)(g + 3 * tn);
+    float* dv[3] = {g + 3 * tn + 3 * nt, g + 3 * tn + 4 * nt, g + 3 * tn + 5 * nt};
+    for (int k = 0; k < 3; ++k)
+      HIPCHK(hipMemcpyAsync((void*)dtg.x[k], targ->x[k], sizeof(float) * tn, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync((void*)dtg.idx, targ->idx, sizeof(uint32_t) * 3 * nt, hipMemcpyHostToDevice, st));
+    o3d_opts oc = *opts;
+    oc.stream = st;
+    if ((rc = pap_dev(&d, &dtg, dv, &oc, st)) != O3D_OK) return rc;
+    for (int k = 0; k < 3; ++k) HIPCHK(hipMemcpyAsync(vel[k], dv[k], sizeof(float) * nt, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(stats, secs, pan_flops(false, nt, np, false), (double)nt * (double)np);
   return O3D_OK;
 }
 

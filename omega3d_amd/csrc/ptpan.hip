@@ -380,22 +380,7 @@ __global__ __launch_bounds__(BLOCK) void k_bem_rhs(BemArgs a) {
     u[2] = (float)((double)u[2] + (double)f * (a.rotvel[0] * c[1] - a.rotvel[1] * c[0]));
   }
   float x1[3], x2[3], nm[3];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) x1[j] = xs[j][id1] - xs[j][id0];
-  const float base = sqrtf(x1[0] * x1[0] + x1[1] * x1[1] + x1[2] * x1[2]);
-#pragma unroll
-  for (int j = 0; j < 3; ++j) x1[j] = (float)((double)x1[j] * (1.0 / (double)base));
-#pragma unroll
-  for (int j = 0; j < 3; ++j) x2[j] = xs[j][id2] - xs[j][id0];
-  const float dp = x2[0] * x1[0] + x2[1] * x1[1] + x2[2] * x1[2];
-#pragma unroll
-  for (int j = 0; j < 3; ++j) x2[j] = x2[j] - dp * x1[j];
-  const float height = sqrtf(x2[0] * x2[0] + x2[1] * x2[1] + x2[2] * x2[2]);
-#pragma unroll
-  for (int j = 0; j < 3; ++j) x2[j] = (float)((double)x2[j] * (1.0 / (double)height));
-  nm[0] = x1[1] * x2[2] - x1[2] * x2[1];
-  nm[1] = x1[2] * x2[0] - x1[0] * x2[2];
-  nm[2] = x1[0] * x2[1] - x1[1] * x2[0];
+  panel_basis_f(xs, id0, id1, id2, x1, x2, nm);
   const float t1 = -(u[0] * x1[0] + u[1] * x1[1] + u[2] * x1[2]);
   const float t2 = -(u[0] * x2[0] + u[1] * x2[1] + u[2] * x2[2]);
   const float tn = -(u[0] * nm[0] + u[1] * nm[1] + u[2] * nm[2]);

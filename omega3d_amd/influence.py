@@ -272,6 +272,35 @@ def pan(core, nodes, idx, ts, area, ss, tpos, grads=True, precision="comp", maxl
     return (vel, grad, st) if stats else (vel, grad)
 
 
+def pap(core, nodes, idx, ts, area, ss, tnodes, tidx, precision="comp", maxlev=-1, vel=None, stats=False):
+    """Host-array panels -> panels on the GPU (panels_affect_panels, Influence.h:1225-1245):
+    the source panels' velocity at the target surface's colocation points (centroids
+    pushed 1e-4 along the normals), velocity only; vel (3, Pt) is overwritten (new
+    when not given) and returned."""
+    nodes, ts, area, ss, tnodes = _c32(nodes), _c32(ts), _c32(area), _c32(ss), _c32(tnodes)
+    idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    tidx = np.ascontiguousarray(tidx, dtype=np.uint32).reshape(-1)
+    npt = tidx.shape[0] // 3
+    vel = np.zeros((3, npt), np.float32) if vel is None else vel
+    p = _lib.Panels()
+    p.np, p.nn = area.shape[0], nodes.shape[1]
+    for d in range(3):
+        p.x[d] = nodes[d].ctypes.data
+        p.ts[d] = ts[d].ctypes.data
+    p.idx, p.area = idx.ctypes.data, area.ctypes.data
+    p.ss = ss.ctypes.data if ss is not None else None
+    t = _lib.Panels()
+    t.np, t.nn = npt, tnodes.shape[1]
+    for d in range(3):
+        t.x[d] = tnodes[d].ctypes.data
+    t.idx = tidx.ctypes.data
+    vp = (C.c_void_p * 3)(*[vel[d].ctypes.data for d in range(3)])
+    o = make_opts(core, False, precision, 0, 0, 0, maxlev)
+    st = _lib.Stats()
+    check(lib.o3d_panels_affect_panels(C.byref(p), C.byref(t), vp, C.byref(o), C.byref(st)))
+    return (vel, st) if stats else vel
+
+
 def points_affect_points(src: Points, targ: Points, restype: ResultsType, env: ExecEnv):
     """Influence.h:67-551 on the GPU: accumulate src's influence into targ."""
     grads = restype.compute_grad()

@@ -105,24 +105,31 @@ def batched(k=256, lo=2048, hi=262144, reps=2, tune=True, log=print):
     return out
 
 
-def served(duration=20.0, sessions=16, lo=1024, hi=1048576, load=0.5, rate=None, seed=scenes.SEED_BASE + 4,
-           log=print):
-    from .serve import Server
+def served_sessions(sessions=16, lo=1024, hi=1048576, seed=scenes.SEED_BASE + 4):
+    """The served workload's sessions: N log-uniform on [lo, hi]; each a single ring
+    (RM core, sigma 0.05) or a ring pair (EXP core, sigma 0.02) with equal odds.
+    Returns [(particles, core)]."""
     rng = scenes.Rng(seed, 11)
     counts = scenes.log_uniform_counts(sessions, lo, hi, seed)
     kinds = rng.uniform(sessions) < 0.5
+    out = []
+    for i in range(sessions):
+        if kinds[i]:
+            half = max(1, int(counts[i]) // 2)
+            out.append((scenes.colliding_rings(n_each=half, seed=seed + 100 + i), "exp"))
+        else:
+            out.append((scenes.ring(int(counts[i]), sigma=0.05, seed=seed, stream=100 + i), "rm"))
+    return out
+
+
+def served(duration=20.0, sessions=16, lo=1024, hi=1048576, load=0.5, rate=None, seed=scenes.SEED_BASE + 4,
+           log=print):
+    from .serve import Server
     t0 = time.perf_counter()
     srv = Server(hbm_budget=16 << 30, max_active=64)
     try:
         sids, ns = [], []
-        for i in range(sessions):
-            if kinds[i]:
-                half = max(1, int(counts[i]) // 2)
-                p = scenes.colliding_rings(n_each=half, seed=seed + 100 + i)
-                core = "exp"
-            else:
-                p = scenes.ring(int(counts[i]), sigma=0.05, seed=seed, stream=100 + i)
-                core = "rm"
+        for p, core in served_sessions(sessions, lo, hi, seed):
             sids.append(srv.open_session(p, core))
             ns.append(p.n)
         ns = np.array(ns, np.float64)

@@ -44,7 +44,8 @@ def lib():
     if _lib is None:
         _lib = _load(os.path.join(HERE, "liboracle.so"))
         for name in ("orc_p2p_f", "orc_p2p_d", "orc_pan_f", "orc_pan_d", "orc_p2p_vcstyle", "orc_tri_integrals",
-                     "orc_pan_g7_near", "orc_pan_g7_d", "orc_bem_rhs", "orc_bem_bases", "orc_bem_matrix"):
+                     "orc_pan_g7_near", "orc_pan_g7_d", "orc_bem_rhs", "orc_bem_bases", "orc_bem_matrix",
+                     "orc_colocation"):
             getattr(_lib, name).restype = C.c_int
         _lib.orc_flops_pair.restype = C.c_int
         _lib.orc_flops_pair.argtypes = [C.c_int] * 4
@@ -263,6 +264,27 @@ def bem_bases(nodes, idx):
     if rc != 0:
         raise RuntimeError(f"oracle bem_bases failed rc={rc}")
     return b, area
+
+
+def colocation(nodes, idx, offset=1e-4):
+    """Surfaces::represent_as_particles(offset, ...) positions (Surfaces.h:959-1010):
+    each panel's centroid pushed offset along its normal, (3, P) float32"""
+    idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    nodes = _as(nodes, np.float32)
+    P = idx.shape[0] // 3
+    out = np.zeros((3, P), np.float32)
+    rc = lib().orc_colocation(_i64(P), _ptr(nodes[0]), _ptr(nodes[1]), _ptr(nodes[2]), _ptr(idx), C.c_float(offset),
+                              _ptr(out[0]), _ptr(out[1]), _ptr(out[2]))
+    if rc != 0:
+        raise RuntimeError(f"oracle colocation failed rc={rc}")
+    return out
+
+
+def pap(core, nodes, idx, ts, area, ss, tnodes, tidx, mode="f32d", maxlev=RECURSIVE_LEVELS):
+    """panels_affect_panels (Influence.h:1225-1245): the velocity of the source panels
+    at the target surface's colocation points (offset 1e-4), velocity only, (3, Pt)"""
+    v, _ = pan(core, nodes, idx, ts, area, ss, colocation(tnodes, tidx, 1e-4), False, mode, maxlev)
+    return v
 
 
 def surface(nodes, idx):

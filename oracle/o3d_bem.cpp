@@ -10,6 +10,8 @@
 //   compute_bases       Surfaces.h:766-816     x1 = (n1 - n0) * (1.0/base), x2 = ((n2 - n0) - dp x1)
 //                                              * (1.0/height) (the scalings in double), norm = x1 x x2
 //                                              (also exported alone: orc_bem_bases, with the areas)
+//   represent_as_particles  Surfaces.h:959-1010  colocation points: centroid + offset * normal
+//                                              (orc_colocation, for panels_affect_panels)
 //   vels_to_rhs_panels  RHS.h:47-98            rhs = -(pu . x1), -(pu . x2) [, -(pu . norm)], or
 //                                              -(pu . norm) alone for one unknown
 // as BEMHelper.h:83-117 chains them (finalize, then the body motion with factor -1).
@@ -60,6 +62,27 @@ int orc_bem_bases(int64_t np, const float* x, const float* y, const float* z, co
     float x1[3], x2[3], nm[3];
     panel_basis(xs, idx, i, x1, x2, nm, area ? area + i : nullptr);
     for (int d = 0; d < 3; ++d) { b[d][i] = x1[d]; b[3 + d][i] = x2[d]; b[6 + d][i] = nm[d]; }
+  }
+  return 0;
+}
+
+// Surfaces::represent_as_particles (Surfaces.h:959-1010) positions, the colocation
+// points panels_affect_panels evaluates at (Influence.h:1233-1237): centroid as
+// (1./3.) * (float sum of the nodes) in double, stored float, plus offset times
+// the compute_bases normal in float
+int orc_colocation(int64_t np, const float* x, const float* y, const float* z, const uint32_t* idx, float offset,
+                   float* px, float* py, float* pz) {
+  const float* xs[3] = {x, y, z};
+  float* out[3] = {px, py, pz};
+  for (int64_t i = 0; i < np; ++i) {
+    float x1[3], x2[3], nm[3];
+    panel_basis(xs, idx, i, x1, x2, nm, nullptr);
+    const uint32_t id0 = idx[3 * i], id1 = idx[3 * i + 1], id2 = idx[3 * i + 2];
+    for (int j = 0; j < 3; ++j) {
+      float c = (float)((1. / 3.) * (double)(xs[j][id0] + xs[j][id1] + xs[j][id2]));
+      c += offset * nm[j];
+      out[j][i] = c;
+    }
   }
   return 0;
 }

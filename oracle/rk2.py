@@ -13,7 +13,9 @@ Reference (file:line):
 
 The pair sums come from the oracle's points_affect_points restatement (sources =
 targets = the collection, self pair included as in the reference); the O(N)
-updates follow the reference's float/double types operation by operation.
+updates follow the reference's float/double types operation by operation.  A test
+may pass its own pair sums (``pair``) to check a large step whose N^2 sums the
+CPU cannot afford: the O(N) updates are still this restatement's.
 """
 import numpy as np
 
@@ -22,8 +24,11 @@ from . import p2p
 F32, F64 = np.float32, np.float64
 
 
-def _find_vels(core, pos, st, rad, fs, mode):
-    v, g = p2p(core, pos, st, rad, pos, rad, True, mode)
+def _find_vels(core, pos, st, rad, fs, mode, pair=None):
+    if pair is None:
+        v, g = p2p(core, pos, st, rad, pos, rad, True, mode)
+    else:
+        v, g = pair(core, pos, st, rad)
     if mode == "f64":
         factor = 0.25 / np.pi
         return np.asarray(fs, F64)[:, None] + v * factor, g * factor
@@ -38,10 +43,11 @@ def _wdu(s, ug, T):
     return np.stack([(s[0] * ug[i] + s[1] * ug[3 + i]) + s[2] * ug[6 + i] for i in range(3)]).astype(T)
 
 
-def rk2_step(core, pos, st, rad, elong, dt, fs=(0.0, 0.0, 0.0), mode="f32d"):
+def rk2_step(core, pos, st, rad, elong, dt, fs=(0.0, 0.0, 0.0), mode="f32d", pair=None):
     """Returns (pos, str, elong) after one Ralston step.  mode 'f32d' restates the
     reference's float storage with double accumulation; 'f64' does every operation
-    in double on the same inputs."""
+    in double on the same inputs.  pair(core, pos, str, rad) -> (vel, grad), the
+    unscaled sums of the collection on itself, replaces the oracle's own."""
     T = F64 if mode == "f64" else F32
     x0 = np.asarray(pos, T)
     s0 = np.asarray(st, T)
@@ -49,7 +55,7 @@ def rk2_step(core, pos, st, rad, elong, dt, fs=(0.0, 0.0, 0.0), mode="f32d"):
     e0 = np.asarray(elong, T)
     dt23 = 2.0 / 3.0 * dt
     # stage 1
-    u1, ug1 = _find_vels(core, x0, s0, r, fs, mode)
+    u1, ug1 = _find_vels(core, x0, s0, r, fs, mode, pair)
     if mode == "f64":
         x1 = x0 + dt23 * 1.0 * u1
         s1 = s0 + dt23 * 1.0 * _wdu(s0, ug1, F64)
@@ -58,7 +64,7 @@ def rk2_step(core, pos, st, rad, elong, dt, fs=(0.0, 0.0, 0.0), mode="f32d"):
         w1 = _wdu(s0, ug1, F32)
         s1 = (s0.astype(F64) + dt23 * 1.0 * w1.astype(F64)).astype(F32)
     # stage 2 at the interim state
-    u2, ug2 = _find_vels(core, x1, s1, r, fs, mode)
+    u2, ug2 = _find_vels(core, x1, s1, r, fs, mode, pair)
     wdu1 = _wdu(s0, ug1, T)
     wdu2 = _wdu(s0, ug2, T)
     if mode == "f64":

@@ -1,8 +1,12 @@
 // drive_influence.cpp -- drives include/omega3d/Influence.hpp the way Omega3D's
 // Convection::find_vels does (Convection.h:131-182): zero the targets, visit every
-// source collection with InfluenceVisitor, finalize.  Writes inputs and results as
-// raw float32 to the directory given on the command line so that the test can
-// check them against the oracle.  Exit status 0 on success.
+// source collection with InfluenceVisitor, finalize; then the way solve_bem builds
+// its right-hand side (BEMHelper.h:73-95): for every boundary surface, zero its
+// panel velocities, visit every vortex collection with a velocity-only visitor
+// (particles -> panels) and finalize; and one surface on another (panels ->
+// panels).  Writes inputs and results as raw float32 to the directory given on the
+// command line so that the test can check them against the oracle.  Exit status 0
+// on success.
 #include "omega3d/Influence.hpp"
 
 #include <cstdio>
@@ -54,6 +58,10 @@ int main(int argc, char** argv) {
   env.set_core(O3D_CORE_EXP);
   InfluenceVisitor<float, double> vis{ResultsType(velandgrad), env};
   const std::array<double, 3> fs = {0.1, 0.0, -0.05};
+  std::array<std::vector<float>, 3> nx2 = nx;
+  for (auto& z : nx2[2]) z += 0.15f;
+  std::vector<Collection<float>> bdry2;
+  bdry2.emplace_back(Surfaces<float>(nx2, idx, ts, area));
   try {
     for (auto* targs : {&vort, &fld}) {
       for (auto& t : *targs) {
@@ -63,7 +71,16 @@ int main(int argc, char** argv) {
         std::visit([&](auto& e) { if constexpr (std::is_same<std::decay_t<decltype(e)>, Points<float>>::value) e.finalize_vels(fs); }, t);
       }
     }
-    // the error path: a velandgrad request on targets without gradient storage
+    // solve_bem's right-hand-side loop (BEMHelper.h:73-95), through std::visit
+    InfluenceVisitor<float, double> ivis{ResultsType(velonly), env};
+    for (auto& t : bdry) {
+      std::visit([](auto& e) { e.zero_vels(); }, t);
+      for (auto& src : vort) std::visit(ivis, src, t);
+      std::visit([&](auto& e) { e.finalize_vels(fs); }, t);
+    }
+    // one surface on another: a second strip above the first
+    for (auto& src : bdry) std::visit(ivis, src, bdry2[0]);
+
     Points<float> nograd(xf, false);
     bool threw = false;
     try { points_affect_points<float, double>(std::get<Points<float>>(vort[0]), nograd, ResultsType(velandgrad), env); }
@@ -79,6 +96,8 @@ int main(int argc, char** argv) {
   dump(out, "x", flat(x)); dump(out, "s", flat(s)); dump(out, "r", r); dump(out, "xf", flat(xf));
   dump(out, "vort_u", flat(pv.get_vel())); dump(out, "vort_ug", flat(*pv.get_velgrad()));
   dump(out, "fld_u", flat(pf.get_vel())); dump(out, "fld_ug", flat(*pf.get_velgrad()));
+  dump(out, "bdry_pu", flat(std::get<Surfaces<float>>(bdry[0]).get_vel()));
+  dump(out, "bdry2_pu", flat(std::get<Surfaces<float>>(bdry2[0]).get_vel()));
   std::printf("drive_influence: ok (%zu particles, %zu field points, 2 panels)\n", n, nf);
   return 0;
 }

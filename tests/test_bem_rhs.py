@@ -74,3 +74,28 @@ def test_rigid_rotation_and_translation_with_the_stream(sphere):
     assert np.abs(pu - want).max() < 1e-6
     pu, rhs = oracle.bem_rhs(sphere.nodes, sphere.idx, np.zeros((3, P)), fs, body=True, vel=fs, nunk=3)
     assert not pu.any() and not rhs.any()
+
+
+def test_colocation_points():
+    """represent_as_particles' positions (Surfaces.h:959-1010): each panel's centroid
+    pushed 1e-4 along its normal (x1 x x2 with x1 along edge 0-1, i.e. along
+    (v1 - v0) x (v2 - v0)).  Checked in float64 against that geometry: the offset
+    along the normal is 1e-4 and the tangential part vanishes, within the float
+    rounding of positions of size ~1."""
+    pan = scenes.icosphere_panels(levels=2)
+    idx = pan.idx.reshape(-1, 3)
+    c = oracle.colocation(pan.nodes, pan.idx, 1e-4).astype(np.float64)
+    v = pan.nodes.astype(np.float64)[:, idx]          # (3, P, 3 corners)
+    cen = v.mean(axis=2)
+    n = np.cross(v[:, :, 1] - v[:, :, 0], v[:, :, 2] - v[:, :, 0], axis=0)
+    n /= np.linalg.norm(n, axis=0)
+    d = c - cen
+    along = (d * n).sum(0)
+    tang = d - along * n
+    assert np.abs(along - 1e-4).max() <= 3e-7
+    assert np.linalg.norm(tang, axis=0).max() <= 3e-7
+    # outward on the sphere (its panels are wound with the normal pointing out)
+    assert (along * (cen * n).sum(0) > 0).all()
+    # a planted wrong offset sign fails the check
+    c2 = oracle.colocation(pan.nodes, pan.idx, -1e-4).astype(np.float64)
+    assert np.abs(((c2 - cen) * n).sum(0) - 1e-4).max() > 1e-4

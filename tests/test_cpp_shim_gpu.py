@@ -1,6 +1,8 @@
 """The C++ mirror of the reference interface (include/omega3d/Influence.hpp), driven
-like Convection::find_vels (tests/cpp/drive_influence.cpp), checked against the
-oracle: particles and field points, vortex and panel sources, finalize_vels."""
+like Convection::find_vels and like solve_bem's right-hand-side loop
+(tests/cpp/drive_influence.cpp), checked against the oracle: particles and field
+points, vortex and panel sources, finalize_vels; particles -> panels and panels ->
+panels through InfluenceVisitor."""
 import os
 import subprocess
 
@@ -39,3 +41,15 @@ def test_cpp_influence_shim(tmp_path):
         got_g = rd(uname + "_ug", (9, tpos.shape[1]))
         assert rel_errors(got_v, want_v)[0] <= 1e-6, uname
         assert rel_errors(got_g, want_g)[0] <= 1e-6, uname
+    # solve_bem's loop: zero, particles -> panels (subtracted), finalize
+    pu = rd("bdry_pu", (3, 2))
+    want = fs + oracle.ptpan("exp", x, s, nodes, idx, area, "f64") * f
+    assert rel_errors(pu, want)[0] <= 1e-6
+    # panels -> panels: the strip's velocity at the colocation points of the strip
+    # 0.15 above it, overwriting (no finalize)
+    nodes2 = nodes.copy()
+    nodes2[2] += np.float32(0.15)
+    pu2 = rd("bdry2_pu", (3, 2))
+    r64 = oracle.pap("exp", nodes, idx, ts, area, None, nodes2, idx, "f64")
+    r32 = oracle.pap("exp", nodes, idx, ts, area, None, nodes2, idx, "f32d")
+    assert rel_errors(pu2, r32)[0] <= 1e-5 and rel_errors(pu2, r64)[0] <= max(2e-6, 3 * rel_errors(r32, r64)[0])

@@ -371,6 +371,97 @@ def test_rk2_stretching_sign(core):
     assert np.sign(e[-1] - 1.0) == np.sign(dwdz)
 
 
+def _rm_field(src, alpha, sig_s, sig_t, x):
+    """Velocity (finalized: / 4 pi) and its gradient g[3j+i] = du_i/dx_j at x of one
+    Rosenhead-Moore particle, written from the formulas: u = K(d) alpha x d / 4 pi,
+    K = (|d|^2 + ss^2 + st^2)^-3/2, d = x - src, so
+    du_i/dx_j = (dK/dx_j (alpha x d)_i + K eps_ikj alpha_k) / 4 pi."""
+    d = x - src
+    r2 = d @ d + sig_s ** 2 + sig_t ** 2
+    K = r2 ** -1.5
+    c = np.cross(alpha, d)
+    u = K * c / (4 * np.pi)
+    dK = -3.0 * d * r2 ** -2.5
+    eps = np.zeros((3, 3, 3))
+    eps[0, 1, 2] = eps[1, 2, 0] = eps[2, 0, 1] = 1.0
+    eps[0, 2, 1] = eps[2, 1, 0] = eps[1, 0, 2] = -1.0
+    J = np.zeros((3, 3))      # J[i, j] = du_i/dx_j
+    for i in range(3):
+        for j in range(3):
+            J[i, j] = (dK[j] * c[i] + K * sum(eps[i, k, j] * alpha[k] for k in range(3))) / (4 * np.pi)
+    return u, J
+
+
+def _probe_step_expected(J_use, weights, h, src, alpha_s, sig, p0, a0, e0):
+    """One step of the reference's scheme for a weak probe near one stationary source,
+    from the closed-form field: Ralston's position update (x1 = x0 + 2h/3 u(x0),
+    x = x0 + h (w1 u(x0) + w2 u(x1))) and the stretch with the ORIGINAL strength at
+    both stages (Points.h:380-416): wdu_k = (a0 . grad) u at x0 and x1, combined with
+    the same weights; elongation e0 (1 + h a0 . wdu / |a0|^2)."""
+    u1, J1 = _rm_field(src, alpha_s, sig, sig, p0)
+    x1 = p0 + (2.0 / 3.0) * h * u1
+    u2, J2 = _rm_field(src, alpha_s, sig, sig, x1)
+    w1, w2 = weights
+    x = p0 + h * (w1 * u1 + w2 * u2)
+    wdu = w1 * J_use(J1) @ a0 + w2 * J_use(J2) @ a0
+    a = a0 + h * wdu
+    e = e0 * (1.0 + h * (a0 @ wdu) / (a0 @ a0))
+    return x, a, e
+
+
+def test_rk2_stretching_off_axis_probe():
+    """The stretch term and the elongation of one Ralston step, pinned by a case where
+    the velocity gradient is not symmetric: a weak probe particle off the axis of a
+    single Rosenhead-Moore source, with its strength along no eigenvector of the
+    gradient.  The source does not move (its self pair has no velocity) and keeps its
+    strength (the self pair's gradient K(0) skew(alpha) gives (alpha . grad) u = 0);
+    the probe's strength is 1e-12 of the source's, so its influence back is far below
+    the bar.  The expected step comes from the closed-form field of one particle.  A
+    transposed gradient ((grad u)^T a instead of (a . grad) u, the alternative the
+    reference's comment names, Points.h:380-383) or swapped weights 3/4, 1/4 fail it
+    by orders of magnitude more than the bar."""
+    from oracle import rk2
+    src = np.array([0.0, 0.0, 0.0])
+    alpha_s = np.array([0.3, -0.2, 1.0])
+    sig = 0.1
+    p0 = np.array([0.25, 0.12, 0.07])
+    a0 = np.array([0.4, 0.9, -0.3]) * 1e-12
+    h = 0.02
+    pos = np.stack([src, p0], axis=1)
+    st = np.stack([alpha_s, a0], axis=1)
+    rad = np.array([sig, sig])
+    x, s, e = rk2.rk2_step("rm", pos, st, rad, np.ones(2), h, (0.0, 0.0, 0.0), "f64")
+    ident = lambda J: J
+    want_x, want_a, want_e = _probe_step_expected(ident, (0.25, 0.75), h, src, alpha_s, sig, p0, a0, 1.0)
+    da = s[:, 1] - a0
+    dwant = want_a - a0
+    np.testing.assert_allclose(x[:, 1], want_x, rtol=0, atol=1e-12)
+    assert np.linalg.norm(da - dwant) <= 1e-9 * np.linalg.norm(dwant)
+    assert abs((e[1] - 1.0) - (want_e - 1.0)) <= 1e-9 * abs(want_e - 1.0)
+    # the source: neither moved nor stretched beyond the probe's O(1e-12) influence
+    assert np.linalg.norm(x[:, 0] - src) <= 1e-12 and np.linalg.norm(s[:, 0] - alpha_s) <= 1e-12
+    # the case discriminates: the plausible mistakes are far outside the bar
+    for J_use, wts in ((lambda J: J.T, (0.25, 0.75)), (ident, (0.75, 0.25)), (ident, (0.5, 0.5))):
+        _, bad_a, bad_e = _probe_step_expected(J_use, wts, h, src, alpha_s, sig, p0, a0, 1.0)
+        assert np.linalg.norm((bad_a - a0) - dwant) > 1e-3 * np.linalg.norm(dwant)
+    # (a . (grad u) a is the same for the transpose; the weights move the elongation)
+    _, bad_a, bad_e = _probe_step_expected(ident, (0.75, 0.25), h, src, alpha_s, sig, p0, a0, 1.0)
+    assert abs((bad_e - 1.0) - (want_e - 1.0)) > 1e-3 * abs(want_e - 1.0)
+    # float storage with double accumulation (the served step's mode) within FP32 rounding
+    x32, s32, e32 = rk2.rk2_step("rm", pos.astype(np.float32), st.astype(np.float32), rad.astype(np.float32),
+                                 np.ones(2, np.float32), h, (0.0, 0.0, 0.0), "f32d")
+    assert np.linalg.norm(s32[:, 1].astype(np.float64) - want_a) <= 1e-5 * np.linalg.norm(dwant) + 1e-7 * np.linalg.norm(a0)
+
+
+def test_rk2_stretching_planted_transpose_fails(monkeypatch):
+    """Planting the transposed stretch term in the oracle makes the off-axis check fail."""
+    from oracle import rk2
+    T = rk2._wdu
+    monkeypatch.setattr(rk2, "_wdu", lambda s, ug, dt: T(s, ug[[0, 3, 6, 1, 4, 7, 2, 5, 8]], dt))
+    with pytest.raises(AssertionError):
+        test_rk2_stretching_off_axis_probe()
+
+
 def test_rk2_freestream_only():
     """With zero strengths the step moves every particle by dt * U_inf exactly."""
     from oracle import rk2
