@@ -193,6 +193,22 @@ int o3d_panels_affect_points_dev(const o3d_panels* src, o3d_points* targ,
 int o3d_points_affect_panels_dev(const o3d_points* src, const o3d_panels* targ, float* const vel[3],
                                  const o3d_opts* opts, o3d_stats* stats);
 
+/* No counterpart in the reference: o3d_points_affect_points_dev for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc.  The same direct kernels (method AUTO
+ * and DIRECT both run them; FACTORED and FP64 are refused with O3D_EINVAL), with
+ * their scratch (the source split's partial sums, the Morton order of EXP
+ * targets) in one static 66 MiB device array per GPU, held until the process
+ * ends; nothing is allocated and no memory fill is issued, so the call can be
+ * captured.  At most O3D_NOALLOC_MAX_TARGETS targets (O3D_EINVAL beyond).  Where
+ * the _dev call's source split would need more partial sums than the array holds
+ * this call takes fewer slices, so its rounding can differ; otherwise the results
+ * are the _dev call's bits.  Calls that overlap on one GPU (other streams or
+ * threads) share the array and race: keep them ordered. */
+#define O3D_NOALLOC_MAX_TARGETS 4194304LL
+int o3d_points_affect_points_noalloc_dev(const o3d_points* src, o3d_points* targ,
+                                         const o3d_opts* opts, o3d_stats* stats);
+
 /* panels_affect_panels (Influence.h:1225-1245): the velocity that the panels of src
  * induce at colocation points of the panels of targ -- each panel's centroid
  * pushed 1e-4 along its normal, as Surfaces::represent_as_particles(0.0001, -1.0)

@@ -69,7 +69,8 @@ _int_fns = ["o3d_init", "o3d_abi_version", "o3d_points_affect_points", "o3d_poin
             "o3d_panels_g7_profile_dev",
             "o3d_points_affect_panels", "o3d_points_affect_panels_dev", "o3d_bem_rhs_dev",
             "o3d_bem_rhs_from_vels_dev", "o3d_bem_matrix", "o3d_bem_matrix_dev", "o3d_bem_matrix_profile_dev",
-            "o3d_morton_order_dev", "o3d_panels_affect_panels", "o3d_panels_affect_panels_dev"]
+            "o3d_morton_order_dev", "o3d_panels_affect_panels", "o3d_panels_affect_panels_dev",
+            "o3d_points_affect_points_noalloc_dev"]
 for _n in _int_fns:
     if hasattr(lib, _n):
         getattr(lib, _n).restype = C.c_int

@@ -67,6 +67,7 @@ static int init_locked(int device) {
   if (g_ctx.stream == nullptr) HIPCHK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
   g_ctx.device = device;
   g_ctx.ready = true;
+  noalloc_setup(device);   // a failure is kept for the no-allocation entry point alone
   return O3D_OK;
 }
 
@@ -144,10 +145,12 @@ hipError_t dispatch_p2p_split(int core, const P2PArgs& a, bool thick, bool grad,
 // Source split of a call whose targets make fewer workgroups than the GPU holds at
 // once: slices of whole tiles, enough of them for about SPLIT_WGS_PER_CU
 // workgroups per compute unit, and a partial buffer of at most SPLIT_MAX_BYTES.
-// Returns the slice count (1: no split) and the slice length.
+// cap > 0 (a caller's fixed scratch of cap bytes): fewer slices when the partials
+// of the wanted slice count do not fit.  Returns the slice count (1: no split) and
+// the slice length.
 constexpr int SPLIT_WGS_PER_CU = 16;
 constexpr size_t SPLIT_MAX_BYTES = (size_t)256 << 20;
-int split_plan(int64_t nt, int64_t ns, int tile, bool grad, int prec, int64_t* slen, size_t* bytes) {
+int split_plan(int64_t nt, int64_t ns, int tile, bool grad, int prec, int64_t* slen, size_t* bytes, size_t cap) {
   const int64_t nb = (nt + BLOCK - 1) / BLOCK;
   const int64_t want = (int64_t)SPLIT_WGS_PER_CU * num_cus();
   if (nb >= want) return 1;
@@ -155,13 +158,17 @@ int split_plan(int64_t nt, int64_t ns, int tile, bool grad, int prec, int64_t* s
   int64_t nsl = (want + nb - 1) / nb;
   if (nsl > ntiles) nsl = ntiles;
   if (nsl > 65535) nsl = 65535;
+  const size_t cols = prec == PREC_COMP ? (grad ? 12 : 3) * sizeof(double) : (grad ? 14 : 3) * sizeof(float);
+  if (cap > 0) {
+    const int64_t fit = (int64_t)(cap / ((size_t)nt * cols));
+    if (nsl > fit) nsl = fit;
+  }
   if (nsl < 2) return 1;
   const int64_t tiles_per = (ntiles + nsl - 1) / nsl;
   *slen = tiles_per * tile;
   nsl = (ns + *slen - 1) / *slen;
-  const size_t cols = prec == PREC_COMP ? (grad ? 12 : 3) * sizeof(double) : (grad ? 14 : 3) * sizeof(float);
   *bytes = (size_t)nsl * (size_t)nt * cols;
-  if (nsl < 2 || *bytes > SPLIT_MAX_BYTES) return 1;
+  if (nsl < 2 || *bytes > (cap > 0 ? cap : SPLIT_MAX_BYTES)) return 1;
   return (int)nsl;
 }
 
@@ -234,6 +241,16 @@ int check_self(const P* src, const P* targ, const o3d_opts* o) {
   return O3D_OK;
 }
 
+// the argument checks of a float point -> point call
+int check_p2p_args(const o3d_points* src, const o3d_points* targ, const o3d_opts* o) {
+  int rc;
+  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  const bool grad = o->results == O3D_VELANDGRAD;
+  if ((rc = check_points(src, true, false, grad, "src")) != O3D_OK) return rc;
+  if ((rc = check_points(targ, false, true, grad, "targ")) != O3D_OK) return rc;
+  return check_self(src, targ, o);
+}
+
 double flops_of(int core, bool grad, bool thick, int64_t nt, int64_t ns) {
   const int f = o3d_flops_per_interaction(core, grad ? O3D_VELANDGRAD : O3D_VELONLY, thick, 0);
   return (double)nt * ((grad ? 12.0 : 3.0) + (double)f * (double)ns);
@@ -242,7 +259,6 @@ double flops_of(int core, bool grad, bool thick, int64_t nt, int64_t ns) {
 // device-pointer point->point; all pointers device, float
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun) {
   const bool grad = o->results == O3D_VELANDGRAD;
-  const bool thick = targ->r != nullptr;
   if (src->n == 0 || targ->n == 0) return O3D_OK;
   if (o->precision == PREC_FP64) return fail(O3D_EINVAL, "internal: FP64 routed to p2p_dev");
   if (o->method == O3D_METHOD_FACTORED) return p2p_factored(src, targ, o, st);
@@ -257,6 +273,14 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
     }
     if (ff) return p2p_factored(src, targ, o, st);
   }
+  return p2p_direct(src, targ, o, st, tun, nullptr, 0);
+}
+
+int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun,
+               void* ws, size_t ws_bytes) {
+  const bool grad = o->results == O3D_VELANDGRAD;
+  const bool thick = targ->r != nullptr;
+  if (src->n == 0 || targ->n == 0) return O3D_OK;
   P2PArgs a{};
   a.s = SrcView{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
   a.t.x = targ->x[0]; a.t.y = targ->x[1]; a.t.z = targ->x[2]; a.t.r = targ->r;
@@ -280,14 +304,22 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   // batched and served kernels share)
   int64_t slen = 0;
   size_t pbytes = 0;
-  const int nsl = auto_cfg && tpt == 1 ? split_plan(a.nt, a.ns, tile, grad, o->precision, &slen, &pbytes) : 1;
+  const int nsl = auto_cfg && tpt == 1 ? split_plan(a.nt, a.ns, tile, grad, o->precision, &slen, &pbytes,
+                                                    ws ? ws_bytes : 0)
+                                       : 1;
   if (nsl > 1) {
-    std::lock_guard<std::mutex> lk(stream_ws_mutex());
-    // under graph capture the workspace cannot grow: the one-pass kernel then
-    if (!stream_capturing(st) || stream_workspace_fits(st, pbytes)) {
-      void* part = nullptr;
-      int rc = stream_workspace(st, pbytes, &part);
-      if (rc != O3D_OK) return rc;
+    void* part = ws;
+    std::unique_lock<std::mutex> lk(stream_ws_mutex(), std::defer_lock);
+    if (!ws) {
+      lk.lock();
+      // under graph capture the workspace cannot grow: the one-pass kernel then
+      if (stream_capturing(st) && !stream_workspace_fits(st, pbytes)) part = nullptr;
+      else {
+        const int rc = stream_workspace(st, pbytes, &part);
+        if (rc != O3D_OK) return rc;
+      }
+    }
+    if (part) {
       hipError_t e = dispatch_p2p_split(o->core, a, thick, grad, o->precision, nsl, slen, part, st);
       if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
       return O3D_OK;
@@ -300,13 +332,24 @@ int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStrea
   // the order needs the stream's workspace to be large enough already; otherwise
   // the targets are visited in index order.
   if (o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX) {
-    std::lock_guard<std::mutex> lk(stream_ws_mutex());
     const size_t bytes = point_order_bytes(a.nt);
-    if (!stream_capturing(st) || stream_workspace_fits(st, bytes)) {
-      void* ws = nullptr;
-      int rc = stream_workspace(st, bytes, &ws);
+    void* ow = ws;
+    std::unique_lock<std::mutex> lk(stream_ws_mutex(), std::defer_lock);
+    if (ws) {
+      if (bytes > ws_bytes)
+        return fail(O3D_EINVAL, "the target order of %lld targets needs %zu bytes of scratch, %zu given",
+                    (long long)a.nt, bytes, ws_bytes);
+    } else {
+      lk.lock();
+      if (stream_capturing(st) && !stream_workspace_fits(st, bytes)) ow = nullptr;
+      else {
+        const int rc = stream_workspace(st, bytes, &ow);
+        if (rc != O3D_OK) return rc;
+      }
+    }
+    if (ow) {
+      const int rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ow, st, &a.tperm);
       if (rc != O3D_OK) return rc;
-      if ((rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ws, st, &a.tperm)) != O3D_OK) return rc;
       hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
       if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
       return O3D_OK;
@@ -427,6 +470,7 @@ int ext_core() { return g_ext_core; }
 void set_ext_core(int c) { g_ext_core = c; }
 Context& ctx() { return g_ctx; }
 hipStream_t lib_stream() { return g_ctx.stream; }
+int lib_device() { return g_ctx.device; }
 std::mutex& pool_mutex() { return g_pool_mu; }
 int pool_buffer(int slot, size_t bytes, void** out) { return pool_get(slot, bytes, out); }
 
@@ -518,11 +562,8 @@ int o3d_points_affect_points(const o3d_points* src, o3d_points* targ, const o3d_
 int o3d_points_affect_points_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, o3d_stats* stats) {
   clear_error();
   int rc;
-  if ((rc = check_opts(o)) != O3D_OK) return rc;
+  if ((rc = check_p2p_args(src, targ, o)) != O3D_OK) return rc;
   const bool grad = o->results == O3D_VELANDGRAD;
-  if ((rc = check_points(src, true, false, grad, "src")) != O3D_OK) return rc;
-  if ((rc = check_points(targ, false, true, grad, "targ")) != O3D_OK) return rc;
-  if ((rc = check_self(src, targ, o)) != O3D_OK) return rc;
   if ((rc = ensure_init()) != O3D_OK) return rc;
   if (o->precision == PREC_FP64)
     return fail(O3D_EINVAL, "FP64 on device float arrays: use o3d_points_affect_points64_dev with double arrays");

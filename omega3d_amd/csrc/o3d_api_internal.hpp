@@ -14,6 +14,7 @@ const char* last_error();
 int ensure_init();
 int num_cus();
 hipStream_t lib_stream();
+int lib_device();   // the context's device ordinal
 std::mutex& pool_mutex();
 int pool_buffer(int slot, size_t bytes, void** out);
 
@@ -66,6 +67,16 @@ constexpr int64_t FF_AUTO_MIN_N = 65536;
 // direct EXP calls with at least this many targets visit them in Morton order
 constexpr int64_t TARGET_ORDER_MIN_N = 32768;
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);
+// the direct method of p2p_dev (one-pass kernel, source split, EXP target order).
+// ws == nullptr: scratch from the stream's workspace; otherwise ws_bytes of the
+// caller's scratch, never grown: a split takes fewer slices to fit, and a target
+// order that does not fit is an error
+int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun,
+               void* ws, size_t ws_bytes);
+int check_p2p_args(const o3d_points* src, const o3d_points* targ, const o3d_opts* o);
+// the no-allocation entry point's scratch on `device` (the current device): its
+// address is looked up once; called by the library's init
+int noalloc_setup(int device);
 int bem_matrix_dev(const o3d_panels* src, const o3d_panels* targ, int32_t nunk, int32_t same, int32_t maxlev,
                    float* coeffs, hipStream_t st, o3d_bem_matrix_profile* prof);
 

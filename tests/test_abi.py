@@ -137,3 +137,22 @@ def test_out_of_range_node_index_is_rejected_before_any_launch():
     m = np.zeros(2 * 2 * 2 * 2, np.float32)
     assert L.o3d_bem_matrix(C.byref(p), C.byref(p), 2, 1, 3, m.ctypes.data_as(C.POINTER(C.c_float))) == 1
     assert "names node 4" in _lib.last_error()
+
+
+def test_no_allocation_call_refuses_before_any_launch():
+    """The no-allocation entry point refuses what its fixed scratch cannot serve --
+    the factored method, FP64, more than O3D_NOALLOC_MAX_TARGETS targets -- on the
+    host, before it touches a device."""
+    from omega3d_amd import _lib
+    L = _lib.lib
+    s, k1 = _pts(10)
+    t, k2 = _pts(10)
+    na = L.o3d_points_affect_points_noalloc_dev
+    assert na(C.byref(s), C.byref(t), C.byref(_lib.make_opts("rm", method="factored")), None) == 1
+    assert "factored" in _lib.last_error()
+    assert na(C.byref(s), C.byref(t), C.byref(_lib.make_opts("rm", precision="fp64")), None) == 1
+    big, k3 = _pts(10)
+    big.n = 4194305
+    assert na(C.byref(s), C.byref(big), C.byref(_lib.make_opts("rm", method="direct")), None) == 1
+    assert "at most 4194304" in _lib.last_error()
+    assert "O3D_NOALLOC_MAX_TARGETS 4194304LL" in open(HEADER).read()
