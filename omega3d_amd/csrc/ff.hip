@@ -154,9 +154,9 @@ int run(const o3d_points* src, o3d_points* targ, bool self_excl, int T, hipStrea
     HIPCHK(hipEventRecord(ev[0].e, st));
   }
   hipLaunchKernelGGL(k_ff_bbox_init, dim3(1), dim3(64), 0, st, L.bb);
-  hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(ns, 256), 2048u)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, L.bb);
+  hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(ns, 256), FF_BBOX_GRID)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, L.bb);
   if (!same)
-    hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(nt, 256), 2048u)), dim3(256), 0, st, tv.x, tv.y, tv.z, nt,
+    hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(nt, 256), FF_BBOX_GRID)), dim3(256), 0, st, tv.x, tv.y, tv.z, nt,
                        L.bb);
   hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(ns, 256)), dim3(256), 0, st, sv.x, sv.y, sv.z, ns, L.bb, L.ks_in, L.is_in);
   HIPCHK(hipGetLastError());
@@ -264,7 +264,7 @@ int point_order(const float* x, const float* y, const float* z, int64_t n, void*
   int* iout = (int*)(p + 256 + 3 * a);
   void* tmp = p + 256 + 4 * a;
   hipLaunchKernelGGL(k_ff_bbox_init, dim3(1), dim3(64), 0, st, bb);
-  hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(n, 256), 2048u)), dim3(256), 0, st, x, y, z, n, bb);
+  hipLaunchKernelGGL(k_ff_bbox, dim3(std::min(grid_of(n, 256), FF_BBOX_GRID)), dim3(256), 0, st, x, y, z, n, bb);
   hipLaunchKernelGGL(k_ff_keys, dim3(grid_of(n, 256)), dim3(256), 0, st, x, y, z, n, bb, kin, iin);
   HIPCHK(hipGetLastError());
   unsigned* sk = nullptr;

@@ -45,7 +45,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int FF_TB = 256;          // targets per workgroup: 4 waves x 64
 constexpr float FF_KAPPA = 2.0f;    // minimum gap, in block radii
-constexpr int FF_FOLD = 16;         // near field, compensated: sources per FP32 partial
+constexpr int FF_FOLD = 32;         // near field, compensated: sources per FP32 gradient partial
 constexpr int FF_M0 = 16;           // near field: fewest tiles per work item
 constexpr int FF_FAR_THREADS = 256; // far field: 4 waves of 64 targets per block
 constexpr int FF_TP = 264;          // far field: LDS row of the FP16 moment tables (halves, tile <= 256)
@@ -93,9 +93,14 @@ __global__ void k_ff_bbox_init(unsigned* bb) {
   else if (threadIdx.x < 6) bb[threadIdx.x] = 0u;
 }
 
-// bounding box of one or two point sets (x, y, z SoA) into bb[0..5] (encoded)
+// bounding box of one or two point sets (x, y, z SoA) into bb[0..5] (encoded).
+// Each workgroup reduces its strided share through shuffles and LDS and then
+// issues 6 atomics, so a launch of FF_BBOX_GRID workgroups makes few atomics on
+// the 6 shared words (one per wave took ~0.5 ms at N = 1M).
+constexpr unsigned FF_BBOX_GRID = 512;
 __global__ __launch_bounds__(256) void k_ff_bbox(const float* x, const float* y, const float* z, int64_t n,
                                                  unsigned* bb) {
+  __shared__ float red[4][6];
   float lo[3] = {__builtin_huge_valf(), __builtin_huge_valf(), __builtin_huge_valf()};
   float hi[3] = {-__builtin_huge_valf(), -__builtin_huge_valf(), -__builtin_huge_valf()};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -111,12 +116,19 @@ __global__ __launch_bounds__(256) void k_ff_bbox(const float* x, const float* y,
       hi[d] = fmaxf(hi[d], __shfl_xor(hi[d], off));
     }
   }
+  const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      atomicMin(&bb[d], fenc(lo[d]));
-      atomicMax(&bb[3 + d], fenc(hi[d]));
-    }
+    for (int d = 0; d < 3; ++d) { red[w][d] = lo[d]; red[w][3 + d] = hi[d]; }
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int d = threadIdx.x;
+    const int nw = (int)(blockDim.x >> 6);
+    float v = red[0][d];
+    for (int k = 1; k < nw; ++k) v = d < 3 ? fminf(v, red[k][d]) : fmaxf(v, red[k][d]);
+    if (d < 3) atomicMin(&bb[d], fenc(v));
+    else atomicMax(&bb[d], fenc(v));
   }
 }
 
@@ -634,11 +646,9 @@ __device__ __forceinline__ void near_item(const FFArgs& a, int64_t b, int r0, in
   float tx[1] = {q.x}, ty[1] = {q.y}, tz[1] = {q.z}, tt[1] = {q.w};
   float acc[1][NACC];
   float hi[3] = {0.f, 0.f, 0.f}, lo[3] = {0.f, 0.f, 0.f};
-  double gd[9];
+  double gr[NACC];
 #pragma unroll
-  for (int c = 0; c < NACC; ++c) acc[0][c] = 0.0f;
-#pragma unroll
-  for (int c = 0; c < 9; ++c) gd[c] = 0.0;
+  for (int c = 0; c < NACC; ++c) { acc[0][c] = 0.0f; gr[c] = 0.0; }
 
   const unsigned* mrow = a.mask + b * a.mwords;
   int rank = 0;
@@ -663,13 +673,15 @@ __device__ __forceinline__ void near_item(const FFArgs& a, int64_t b, int r0, in
       float thr[1] = {CORE == CORE_EXP ? exp_threshold(a.tcs[ti] + tt[0]) : 0.0f};
       if (PREC == PREC_COMP) {
         // In Morton order a near tile holds the target's closest sources, whose large
-        // terms cancel; FP32 partials are therefore folded every FF_FOLD sources.
+        // terms cancel; the gradient's FP32 partials are therefore folded into FP64
+        // every FF_FOLD sources, the velocity's into its TwoSum pair once per tile.
         for (int j0 = 0; j0 < cnt; j0 += FF_FOLD) {
           const int j1 = j0 + FF_FOLD < cnt ? j0 + FF_FOLD : cnt;
           for (int j = j0; j < j1; ++j)
             source_acc<CORE, true, 1>(tx, ty, tz, tt, thr, sh[2 * j], sh[2 * j + 1], acc);
-          fold_tile_comp(acc[0], hi, lo, gd, true);
+          fold_grad_raw(acc[0], gr);
         }
+        fold_vel_comp(acc[0], hi, lo);
       } else {
         for (int j = 0; j < cnt; ++j)
           source_acc<CORE, true, 1>(tx, ty, tz, tt, thr, sh[2 * j], sh[2 * j + 1], acc);
@@ -680,8 +692,7 @@ __device__ __forceinline__ void near_item(const FFArgs& a, int64_t b, int r0, in
     r[0] = (double)hi[0] + (double)lo[0];
     r[1] = (double)hi[1] + (double)lo[1];
     r[2] = (double)hi[2] + (double)lo[2];
-#pragma unroll
-    for (int c = 0; c < 9; ++c) r[3 + c] = gd[c];
+    grad_from<double>(gr, r + 3);
   } else {
     float g[9];
     grad_from<float>(acc[0], g);

@@ -79,6 +79,13 @@ __device__ __forceinline__ void two_sum(float& hi, float& lo, float b) {
   lo += e;
 }
 
+// u + (hi + lo) in FP32: TwoSum of u and hi, the two low parts added, one rounding
+__device__ __forceinline__ float add_comp(float u, float hi, float lo) {
+  float s = u, e = lo;
+  two_sum(s, e, hi);
+  return s + e;
+}
+
 // per-particle core quantity kept in LDS / registers so that each pair needs
 // one add to form the pair's core parameter:
 //   RM, WL: sigma^2  (r2 = rho^2 + ss^2 + st^2;  R^2 = ss^2 + st^2)

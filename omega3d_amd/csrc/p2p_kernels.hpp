@@ -242,21 +242,26 @@ __device__ __forceinline__ void grad_from(const T* a, T* g) {
   g[8] = T(3) * (a[3] + a[7]);
 }
 
-__device__ __forceinline__ void fold_tile_comp(float* acc, float* hi, float* lo, double* gd, bool grad) {
+// Compensated accumulation in two parts.  The velocity's FP32 partial of a tile
+// goes into a TwoSum (hi, lo) pair.  The gradient's 11 raw FP32 sums (dyad D and
+// W, acc[3..13]) go into FP64 sums gr[3..13] every P2P_GFOLD sources: a close
+// source's large dyad term sets the rounding of every later FP32 addition until
+// the next fold, and the terms of a target's nearest sources cancel, so the FP32
+// partials are kept short (at a 512-source tile the gradient of a thin ring was
+// 4e-6 from FP64, at 64 sources 2e-7).  The gradient is formed from the FP64 sums
+// once per target (grad_from).
+constexpr int P2P_GFOLD = 64;
+__device__ __forceinline__ void fold_vel_comp(float* acc, float* hi, float* lo) {
   two_sum(hi[0], lo[0], acc[0]);
   two_sum(hi[1], lo[1], acc[1]);
   two_sum(hi[2], lo[2], acc[2]);
   acc[0] = acc[1] = acc[2] = 0.0f;
-  if (grad) {
-    double a[NACC];
+}
+__device__ __forceinline__ void fold_grad_raw(float* acc, double* gr) {
 #pragma unroll
-    for (int c = 3; c < NACC; ++c) a[c] = (double)acc[c];
-    double g[9];
-    grad_from<double>(a, g);
-#pragma unroll
-    for (int c = 0; c < 9; ++c) gd[c] += g[c];
-#pragma unroll
-    for (int c = 3; c < NACC; ++c) acc[c] = 0.0f;
+  for (int c = 3; c < NACC; ++c) {
+    gr[c] += (double)acc[c];
+    acc[c] = 0.0f;
   }
 }
 
@@ -294,8 +299,9 @@ __device__ __forceinline__ void stage_tile(const SrcView& s, int64_t s0, int cnt
 enum { OUT_SET = 0, OUT_ADD = 1, OUT_PART = 2 };
 
 // partial sums per target and slice: FP32 keeps the raw accumulators (NACC floats,
-// 3 without the gradient), compensated keeps hi + lo and the FP64 gradient (12 or 3
-// doubles).  Layout [slice][column][target].
+// 3 without the gradient), compensated keeps each velocity component's (hi, lo)
+// FP32 pair in one 8-byte column and the FP64 gradient (12 or 3 columns of 8
+// bytes).  Layout [slice][column][target].
 template <bool GRAD, int PREC>
 struct PartCols {
   static constexpr int n = PREC == PREC_COMP ? (GRAD ? 12 : 3) : (GRAD ? NACC : 3);
@@ -325,15 +331,13 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
   }
   float acc[TPT][NACC];
   float hi[TPT][3], lo[TPT][3];
-  double gd[TPT][9];
+  double gr[TPT][NACC];    // compensated: FP64 sums of the raw gradient accumulators [3..13]
 #pragma unroll
   for (int k = 0; k < TPT; ++k) {
 #pragma unroll
-    for (int c = 0; c < NACC; ++c) acc[k][c] = 0.0f;
+    for (int c = 0; c < NACC; ++c) { acc[k][c] = 0.0f; gr[k][c] = 0.0; }
 #pragma unroll
     for (int c = 0; c < 3; ++c) { hi[k][c] = 0.0f; lo[k][c] = 0.0f; }
-#pragma unroll
-    for (int c = 0; c < 9; ++c) gd[k][c] = 0.0;
   }
 
   for (int64_t s0 = s_lo; s0 < s_hi; s0 += tile) {
@@ -346,14 +350,37 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
     float thr[TPT];
 #pragma unroll
     for (int k = 0; k < TPT; ++k) thr[k] = CORE == CORE_EXP ? exp_threshold(__uint_as_float(s_tmax) + tt[k]) : 0.0f;
-    for (int j = 0; j < cnt; ++j) {
-      const float4 P = sh[2 * j];
-      const float4 A = sh[2 * j + 1];
-      source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
+    if (PREC == PREC_COMP && GRAD) {
+      for (int j0 = 0; j0 < cnt; j0 += P2P_GFOLD) {
+        const int j1 = j0 + P2P_GFOLD < cnt ? j0 + P2P_GFOLD : cnt;
+        for (int j = j0; j < j1; ++j) {
+          const float4 P = sh[2 * j];
+          const float4 A = sh[2 * j + 1];
+          source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
+        }
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) fold_grad_raw(acc[k], gr[k]);
+      }
+    } else {
+      for (int j = 0; j < cnt; ++j) {
+        const float4 P = sh[2 * j];
+        const float4 A = sh[2 * j + 1];
+        source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
+      }
     }
     if (PREC == PREC_COMP) {
 #pragma unroll
-      for (int k = 0; k < TPT; ++k) fold_tile_comp(acc[k], hi[k], lo[k], gd[k], GRAD);
+      for (int k = 0; k < TPT; ++k) fold_vel_comp(acc[k], hi[k], lo[k]);
+    }
+  }
+  // compensated: the gradient in ug order from the FP64 sums
+  double gd[TPT][9];
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) {
+    if (PREC == PREC_COMP && GRAD) grad_from<double>(gr[k], gd[k]);
+    else {
+#pragma unroll
+      for (int c = 0; c < 9; ++c) gd[k][c] = 0.0;
     }
   }
 
@@ -370,7 +397,7 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
       } else {
         double* pd = (double*)part + slice * NC * nt + i;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) pd[c * nt] = (double)hi[k][c] + (double)lo[k][c];
+        for (int c = 0; c < 3; ++c) reinterpret_cast<float2*>(pd)[c * nt] = make_float2(hi[k][c], lo[k][c]);
         if (GRAD) {
 #pragma unroll
           for (int c = 0; c < 9; ++c) pd[(3 + c) * nt] = gd[k][c];
@@ -424,9 +451,6 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
       }
     } else {
       double r[12];
-      r[0] = (double)hi[k][0] + (double)lo[k][0];
-      r[1] = (double)hi[k][1] + (double)lo[k][1];
-      r[2] = (double)hi[k][2] + (double)lo[k][2];
       if (GRAD) {
         const double kx = (double)ks[k] * sax[k], ky = (double)ks[k] * say[k], kz = (double)ks[k] * saz[k];
         r[3] = gd[k][0];
@@ -439,16 +463,17 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
         r[10] = gd[k][7] + kx;
         r[11] = gd[k][8];
       }
+      // velocity: the TwoSum pair added in FP32; gradient: its FP64 reduction rounded once
       if (OUT == OUT_ADD) {
-        t.u[i] = (float)((double)t.u[i] + r[0]);
-        t.v[i] = (float)((double)t.v[i] + r[1]);
-        t.w[i] = (float)((double)t.w[i] + r[2]);
+        t.u[i] = add_comp(t.u[i], hi[k][0], lo[k][0]);
+        t.v[i] = add_comp(t.v[i], hi[k][1], lo[k][1]);
+        t.w[i] = add_comp(t.w[i], hi[k][2], lo[k][2]);
         if (GRAD) {
 #pragma unroll
           for (int c = 0; c < 9; ++c) t.g[c][i] = (float)((double)t.g[c][i] + r[3 + c]);
         }
       } else {
-        t.u[i] = (float)r[0]; t.v[i] = (float)r[1]; t.w[i] = (float)r[2];
+        t.u[i] = hi[k][0] + lo[k][0]; t.v[i] = hi[k][1] + lo[k][1]; t.w[i] = hi[k][2] + lo[k][2];
         if (GRAD) {
 #pragma unroll
           for (int c = 0; c < 9; ++c) t.g[c][i] = (float)r[3 + c];
@@ -491,7 +516,8 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_split(P2PArgs a, void* part, int6
                                                  blockIdx.y);
 }
 
-// adds the nslice slices of each target in slice order (in double), takes out the
+// adds the nslice slices of each target in slice order -- velocity as FP32 TwoSum
+// pairs, the gradient's sums in FP64 (its cross-slice reduction) -- takes out the
 // self pair as p2p_span does, forms the gradient and adds into the outputs
 template <int CORE, bool THICK, bool GRAD, int PREC>
 __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* part, int nslice) {
@@ -499,6 +525,7 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* pa
   const int64_t nt = a.nt;
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nt) return;
+  float vh[3] = {0.f, 0.f, 0.f}, vl[3] = {0.f, 0.f, 0.f};
   double sum[NC];
 #pragma unroll
   for (int c = 0; c < NC; ++c) sum[c] = 0.0;
@@ -506,15 +533,22 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* pa
     if (PREC == PREC_FP32) {
       const float* pf = (const float*)part + (int64_t)y * NC * nt + i;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sum[c] += (double)pf[c * nt];
+      for (int c = 0; c < 3; ++c) two_sum(vh[c], vl[c], pf[c * nt]);
+#pragma unroll
+      for (int c = 3; c < NC; ++c) sum[c] += (double)pf[c * nt];
     } else {
       const double* pd = (const double*)part + (int64_t)y * NC * nt + i;
 #pragma unroll
-      for (int c = 0; c < NC; ++c) sum[c] += pd[c * nt];
+      for (int c = 0; c < 3; ++c) {
+        const float2 v = reinterpret_cast<const float2*>(pd)[c * nt];
+        two_sum(vh[c], vl[c], v.x);
+        vl[c] += v.y;
+      }
+#pragma unroll
+      for (int c = 3; c < NC; ++c) sum[c] += pd[c * nt];
     }
   }
   double r[12];
-  r[0] = sum[0]; r[1] = sum[1]; r[2] = sum[2];
   if constexpr (GRAD) {
     double kx = 0.0, ky = 0.0, kz = 0.0;
     if (a.self_excl) {
@@ -537,9 +571,9 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* pa
       r[11] = sum[11];
     }
   }
-  a.t.u[i] = (float)((double)a.t.u[i] + r[0]);
-  a.t.v[i] = (float)((double)a.t.v[i] + r[1]);
-  a.t.w[i] = (float)((double)a.t.w[i] + r[2]);
+  a.t.u[i] = add_comp(a.t.u[i], vh[0], vl[0]);
+  a.t.v[i] = add_comp(a.t.v[i], vh[1], vl[1]);
+  a.t.w[i] = add_comp(a.t.w[i], vh[2], vl[2]);
   if (GRAD) {
 #pragma unroll
     for (int c = 0; c < 9; ++c) a.t.g[c][i] = (float)((double)a.t.g[c][i] + r[3 + c]);

@@ -359,9 +359,9 @@ __global__ __launch_bounds__(BLOCK) void k_pan_g7(PanArgs a) {
     }
   } else {
     float* u = (float*)a.u; float* v = (float*)a.v; float* w = (float*)a.w;
-    u[i] = (float)((double)u[i] + (double)hi[0] + (double)lo[0]);
-    v[i] = (float)((double)v[i] + (double)hi[1] + (double)lo[1]);
-    w[i] = (float)((double)w[i] + (double)hi[2] + (double)lo[2]);
+    u[i] = add_comp(u[i], hi[0], lo[0]);
+    v[i] = add_comp(v[i], hi[1], lo[1]);
+    w[i] = add_comp(w[i], hi[2], lo[2]);
     if (GRAD) {
 #pragma unroll
       for (int k = 0; k < 9; ++k) ((float*)a.g[k])[i] = (float)((double)((float*)a.g[k])[i] + gd[k]);

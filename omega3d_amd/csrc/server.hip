@@ -19,8 +19,9 @@
 //
 // A step of a large session with the RM or EXP core (by default >= 65,536
 // particles) skips the ticks: both of its stages run the factored far-field
-// method, each followed by its stage update, on a second stream that overlaps the
-// ticks, on scratch the server sizes for its largest such session.  Which way a
+// method, each followed by its stage update, on one of a few further streams
+// ("lanes") that overlap the ticks and one another, each lane with scratch the
+// server sizes for its largest such session.  Which way a
 // session's steps go depends only on the session and the server's options, so a
 // step's result still does not depend on its companions.
 #include "o3d_api_internal.hpp"
@@ -319,6 +320,22 @@ struct Tick {
 struct FFJob {
   hipEvent_t done = nullptr;
   int64_t rid = 0;
+  double work = 0;     // the step's interactions
+};
+
+// Factored steps run on SRV_FF_LANES streams of their own, each with scratch for
+// the largest factored session; a step goes to the lane with the least work
+// queued.  A session has at most one step admitted at a time, so its steps stay
+// in submission order whichever lanes they take.  A 65,536-particle session gives
+// the far-field kernel half the GPU's workgroup slots, so steps of several
+// sessions must overlap to fill it.
+constexpr int SRV_FF_LANES = 4;
+struct FFLane {
+  hipStream_t st = nullptr;
+  void* ws = nullptr;
+  size_t bytes = 0;
+  std::deque<FFJob> inflight;
+  double pending = 0;
 };
 
 }  // namespace
@@ -341,12 +358,14 @@ struct o3d_server {
   std::deque<int64_t> queue;       // queued request ids in arrival order
   std::vector<int64_t> active;     // admitted, in admission order
   std::deque<Tick> inflight;
-  // factored steps: their stream, their scratch and the steps enqueued on it
-  hipStream_t ff_stream = nullptr;
-  void* ffws = nullptr;
-  size_t ffws_bytes = 0;
-  std::deque<FFJob> ff_inflight;
+  // factored steps: their lanes (stream, scratch, steps enqueued on it)
+  FFLane lanes[SRV_FF_LANES];
   int64_t n_factored = 0;
+  bool ff_idle() const {
+    for (const FFLane& l : lanes)
+      if (!l.inflight.empty()) return false;
+    return true;
+  }
   Clock::time_point t0 = Clock::now();
   // per-tick device tables, double buffered
   size_t inst_cap = 0, item_cap = 0, move_cap = 0;
@@ -581,11 +600,15 @@ MoveInst move_inst(const Session& ses, const Request& r) {
   return m;
 }
 
-// Both stages of a factored step on the second stream: zero the stage's 12 result
-// rows (the factored method accumulates), evaluate the session on itself (self
-// pair included, as in the ticks), then the stage update.
+// Both stages of a factored step on the least loaded lane: zero the stage's 12
+// result rows (the factored method accumulates), evaluate the session on itself
+// (self pair included, as in the ticks), then the stage update.
 int enqueue_factored_step(o3d_server* S, Request& r, const Session& ses) {
-  hipStream_t st = S->ff_stream;
+  int li = 0;
+  for (int k = 1; k < SRV_FF_LANES; ++k)
+    if (S->lanes[k].pending < S->lanes[li].pending) li = k;
+  FFLane& lane = S->lanes[li];
+  hipStream_t st = lane.st;
   const int64_t n = ses.n;
   const unsigned zgrid = (unsigned)std::min<int64_t>((12 * n + BLOCK - 1) / BLOCK, 4096);
   const unsigned mgrid = (unsigned)std::min<int64_t>((n + BLOCK - 1) / BLOCK, (int64_t)S->move_grid);
@@ -607,16 +630,18 @@ int enqueue_factored_step(o3d_server* S, Request& r, const Session& ses) {
     o.precision = r.prec;
     o.self_term = O3D_SELF_INCLUDE;
     o.method = O3D_METHOD_FACTORED;
-    const int rc = p2p_factored_ws(&p, &p, &o, st, S->ffws, S->ffws_bytes);
+    const int rc = p2p_factored_ws(&p, &p, &o, st, lane.ws, lane.bytes);
     if (rc != O3D_OK) return rc;
     hipLaunchKernelGGL(k_serve_move_one, dim3(mgrid), dim3(BLOCK), 0, st, move_inst(ses, r));
     HIPCHK(hipGetLastError());
   }
   FFJob j;
   j.rid = r.id;
+  j.work = r.interactions;
   HIPCHK(hipEventCreateWithFlags(&j.done, hipEventDisableTiming));
   HIPCHK(hipEventRecord(j.done, st));
-  S->ff_inflight.push_back(j);
+  lane.inflight.push_back(j);
+  lane.pending += j.work;
   r.ff_launched = true;
   S->n_factored++;
   return O3D_OK;
@@ -632,19 +657,21 @@ int ensure_ff_scratch(o3d_server* S, int64_t n, int32_t core) {
   if (!session_factored(S, probe)) return O3D_OK;
   const size_t need = ff_workspace_bytes(n, n, true, 0);
   if (need == 0) return fail(O3D_EDEVICE, "factored scratch query failed");
-  if (need <= S->ffws_bytes) return O3D_OK;
-  void* p = nullptr;
-  if (hipMalloc(&p, need) != hipSuccess) return fail(O3D_ENOMEM, "factored scratch of %zu bytes", need);
-  if (S->ffws) {
-    const hipError_t e = hipStreamSynchronize(S->ff_stream);
-    if (e != hipSuccess) {
-      hipFree(p);
-      return fail(O3D_EDEVICE, "factored stream: %s", hipGetErrorString(e));
+  for (FFLane& lane : S->lanes) {
+    if (need <= lane.bytes) continue;
+    void* p = nullptr;
+    if (hipMalloc(&p, need) != hipSuccess) return fail(O3D_ENOMEM, "factored scratch of %zu bytes", need);
+    if (lane.ws) {
+      const hipError_t e = hipStreamSynchronize(lane.st);
+      if (e != hipSuccess) {
+        hipFree(p);
+        return fail(O3D_EDEVICE, "factored stream: %s", hipGetErrorString(e));
+      }
+      hipFree(lane.ws);
     }
-    hipFree(S->ffws);
+    lane.ws = p;
+    lane.bytes = need;
   }
-  S->ffws = p;
-  S->ffws_bytes = need;
   return O3D_OK;
 }
 
@@ -658,19 +685,21 @@ int launch_factored(o3d_server* S) {
   return O3D_OK;
 }
 
-// finished factored steps (in stream order)
+// finished factored steps (in each lane's stream order)
 void retire_factored(o3d_server* S) {
-  while (!S->ff_inflight.empty()) {
-    FFJob& j = S->ff_inflight.front();
+  for (FFLane& lane : S->lanes)
+  while (!lane.inflight.empty()) {
+    FFJob& j = lane.inflight.front();
     const hipError_t q = hipEventQuery(j.done);
-    if (q == hipErrorNotReady) return;
+    if (q == hipErrorNotReady) break;
     const double when = S->now();
     if (q != hipSuccess) S->loop_status = fail(O3D_EDEVICE, "factored step failed: %s", hipGetErrorString(q));
     Request& r = S->requests.at(j.rid);
     S->active.erase(std::remove(S->active.begin(), S->active.end(), j.rid), S->active.end());
     finish_request(S, r, q == hipSuccess ? O3D_RQ_DONE : O3D_RQ_FAILED, q == hipSuccess ? O3D_OK : O3D_EDEVICE, when);
     hipEventDestroy(j.done);
-    S->ff_inflight.pop_front();
+    lane.pending -= j.work;
+    lane.inflight.pop_front();
     S->done_cv.notify_all();
   }
 }
@@ -812,7 +841,7 @@ void loop_main(o3d_server* S) {
   while (true) {
     retire(S, false);
     retire_factored(S);
-    if (S->stop && S->queue.empty() && S->active.empty() && S->inflight.empty() && S->ff_inflight.empty()) break;
+    if (S->stop && S->queue.empty() && S->active.empty() && S->inflight.empty() && S->ff_idle()) break;
     admit(S);
     bool ran = false;
     int rc = launch_factored(S);
@@ -826,7 +855,7 @@ void loop_main(o3d_server* S) {
       S->done_cv.notify_all();
     }
     if (ran) continue;
-    if (!S->inflight.empty() || !S->ff_inflight.empty()) {
+    if (!S->inflight.empty() || !S->ff_idle()) {
       // let submissions in while the GPU works
       lk.unlock();
       std::this_thread::sleep_for(std::chrono::microseconds(200));
@@ -869,7 +898,7 @@ int o3d_server_create(o3d_server** out, const o3d_server_opts* opts) {
     return fail(O3D_ENOMEM, "arena of %zu bytes", S->opts.hbm_budget);
   S->arena.init(S->opts.hbm_budget);
   HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
-  HIPCHK(hipStreamCreateWithFlags(&S->ff_stream, hipStreamNonBlocking));
+  for (FFLane& lane : S->lanes) HIPCHK(hipStreamCreateWithFlags(&lane.st, hipStreamNonBlocking));
   for (int b = 0; b < 2; ++b) {
     HIPCHK(hipMalloc(&S->d_tab[b], sizeof(ServeTables)));
     HIPCHK(hipHostMalloc(&S->h_tab[b], sizeof(ServeTables), hipHostMallocDefault));
@@ -899,7 +928,7 @@ int o3d_server_destroy(o3d_server* S) {
   S->cv.notify_all();
   if (S->loop.joinable()) S->loop.join();
   hipStreamSynchronize(S->stream);
-  hipStreamSynchronize(S->ff_stream);
+  for (FFLane& lane : S->lanes) hipStreamSynchronize(lane.st);
   for (int b = 0; b < 2; ++b) {
     if (S->graph[b]) hipGraphExecDestroy(S->graph[b]);
     if (S->graph_src[b]) hipGraphDestroy(S->graph_src[b]);
@@ -908,8 +937,10 @@ int o3d_server_destroy(o3d_server* S) {
     hipHostFree(S->h_tab[b]);
   }
   hipStreamDestroy(S->stream);
-  hipStreamDestroy(S->ff_stream);
-  if (S->ffws) hipFree(S->ffws);
+  for (FFLane& lane : S->lanes) {
+    hipStreamDestroy(lane.st);
+    if (lane.ws) hipFree(lane.ws);
+  }
   hipFree(S->arena_base);
   delete S;
   return O3D_OK;
@@ -1094,7 +1125,8 @@ int o3d_server_get_stats(o3d_server* S, o3d_server_stats* st) {
   st->hbm_peak = S->arena.peak;
   st->ticks = S->tick_no;
   st->factored_steps = S->n_factored;
-  st->factored_scratch = S->ffws_bytes;
+  st->factored_scratch = 0;
+  for (const FFLane& lane : S->lanes) st->factored_scratch += lane.bytes;
   return O3D_OK;
 }
 

@@ -82,8 +82,26 @@ def batched(k=256, lo=2048, hi=262144, reps=2, tune=True, log=print):
         b.capture(s)
         ms_graph = timed(lambda: b.replay(s))
         ms_sep = timed(lambda: b.run_separate(s))
-        # accuracy on the largest and the smallest instances, all their targets
+        # accuracy: every instance on 128 spread targets, and the largest and the
+        # smallest on up to 32,768 targets, against the FP64 kernel
         errs = {}
+        per = []
+        for i, p in enumerate(inst):
+            sel = np.linspace(0, p.n - 1, 128).astype(np.int64)
+            vr, gr = _fp64_self("rm", p, sel)
+            v, g = res[i]
+            e = field_errors(v[:, sel], g[:, sel], vr, gr)
+            per.append((e["vel_max_rel"], e["grad_max_rel"]))
+        per = np.array(per)
+        worst = int(np.argmax(per.max(axis=1)))
+        errs["every_instance"] = {"instances": len(inst), "targets_per_instance": 128,
+                                  "vel_max_rel_max": float(per[:, 0].max()),
+                                  "grad_max_rel_max": float(per[:, 1].max()),
+                                  "vel_max_rel_median": float(np.median(per[:, 0])),
+                                  "grad_max_rel_median": float(np.median(per[:, 1])),
+                                  "within_1e-6": int((per.max(axis=1) <= 1e-6).sum()),
+                                  "worst_instance": {"index": worst, "n": int(n[worst]),
+                                                     "factored": bool(n[worst] >= 65536)}}
         for i in (int(np.argmax(n)), int(np.argmin(n))):
             p = inst[i]
             sel = np.arange(p.n) if p.n <= 32768 else np.linspace(0, p.n - 1, 32768).astype(np.int64)
@@ -122,31 +140,61 @@ def served_sessions(sessions=16, lo=1024, hi=1048576, seed=scenes.SEED_BASE + 4)
     return out
 
 
+def _latency_stats(lat, ns):
+    """p50/p99 (ms) of request latencies, overall and per particle-count bucket"""
+    lat, ns = np.asarray(lat, np.float64), np.asarray(ns, np.float64)
+    pct = lambda a: {"p50_ms": round(float(np.percentile(a, 50)) * 1e3, 3),
+                     "p99_ms": round(float(np.percentile(a, 99)) * 1e3, 3), "requests": int(a.size)}
+    out = {"all": pct(lat)}
+    for name, lo, hi in (("n<16384", 0, 16384), ("16384<=n<131072", 16384, 131072), ("n>=131072", 131072, 1 << 62)):
+        m = (ns >= lo) & (ns < hi)
+        if m.any():
+            out[name] = pct(lat[m])
+    return out
+
+
 def served(duration=20.0, sessions=16, lo=1024, hi=1048576, load=0.5, rate=None, seed=scenes.SEED_BASE + 4,
            log=print):
+    """The served workload, four ways on one request set:
+      poisson      arrivals at a fixed rate (load x the device's rate), p50/p99 per N bucket;
+      saturated    the same requests submitted at once, so the queue never drains:
+                   the served capacity in aggregate interactions/s;
+      per_instance the same requests as plain library calls, two evaluations per step
+                   (o3d_points_affect_points_dev, method AUTO) per request, one after
+                   another on one stream with no host synchronisation in between --
+                   the most favourable form of per-instance launches;
+      fp64_spot    the largest session's stage-1 sums, by the method the server takes
+                   for it, against the FP64 kernel on 1,024 targets, both precisions."""
     from .serve import Server
     t0 = time.perf_counter()
+    sess = served_sessions(sessions, lo, hi, seed)
+    ns = np.array([p.n for p, _ in sess], np.float64)
+    mean_pairs = float(np.mean(2.0 * ns * ns))
+    if rate is None:
+        # offered load relative to ~1.1e12 pairs/s (compensated FP32, mixed cores)
+        rate = load * 1.1e12 / mean_pairs
+    arr = scenes.Rng(seed, 12)
+    n_req = int(duration * rate) + 1
+    gaps = -np.log1p(-arr.uniform(n_req)) / rate
+    pick = (arr.uniform(n_req) * sessions).astype(int)
+    prec = np.where(arr.uniform(n_req) < 0.5, "comp", "fp32")
+    inter = float((2.0 * ns[pick] ** 2).sum())
+
+    def open_all(srv):
+        return [srv.open_session(p, core) for p, core in sess]
+
+    out = {"workload": f"RK2 step requests on {sessions} resident sessions, N log-uniform [{lo}, {hi}], "
+                       f"rings (RM) and ring pairs (EXP), plain and compensated mixed",
+           "sessions": sessions, "particles": int(ns.sum()), "requests": n_req,
+           "interactions": inter}
+    # ---- Poisson arrivals ----------------------------------------------------------
     srv = Server(hbm_budget=16 << 30, max_active=64)
     try:
-        sids, ns = [], []
-        for p, core in served_sessions(sessions, lo, hi, seed):
-            sids.append(srv.open_session(p, core))
-            ns.append(p.n)
-        ns = np.array(ns, np.float64)
+        sids = open_all(srv)
         log(f"served: {sessions} sessions, {int(ns.sum())} particles resident, set up in "
             f"{time.perf_counter() - t0:.1f}s")
-        # warm-up: one step of the smallest session
-        srv.wait(srv.submit(sids[int(np.argmin(ns))], 1e-3, "comp"))
+        srv.wait(srv.submit(sids[int(np.argmin(ns))], 1e-3, "comp"))     # warm-up
         srv.reset_stats()
-        mean_pairs = float(np.mean(2.0 * ns * ns))
-        if rate is None:
-            # offered load relative to ~1.1e12 pairs/s (compensated FP32, mixed cores)
-            rate = load * 1.1e12 / mean_pairs
-        arr = scenes.Rng(seed, 12)
-        n_req = int(duration * rate) + 1
-        gaps = -np.log1p(-arr.uniform(n_req)) / rate
-        pick = (arr.uniform(n_req) * sessions).astype(int)
-        prec = np.where(arr.uniform(n_req) < 0.5, "comp", "fp32")
         rids = []
         t_start = time.perf_counter()
         t_next = t_start
@@ -156,19 +204,87 @@ def served(duration=20.0, sessions=16, lo=1024, hi=1048576, load=0.5, rate=None,
             if dt > 0:
                 time.sleep(dt)
             rids.append(srv.submit(sids[pick[j]], 1e-3, str(prec[j])))
-        for r in rids:
-            srv.wait(r)
+        infos = [srv.wait(r) for r in rids]
         st = srv.stats()
         wall = time.perf_counter() - t_start
     finally:
         srv.close()
-    out = {"workload": f"Poisson RK2 step requests on {sessions} resident sessions, N log-uniform [{lo}, {hi}], "
-                       f"rings (RM) and ring pairs (EXP), plain and compensated mixed",
-           "rate_per_s": round(rate, 3), "requests": len(rids), "completed": st["completed"],
-           "failed": st["failed"], "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
-           "mean_s": round(st["mean_s"], 4), "ginter_per_s": round(st["ginter_per_s"], 2),
-           "wall_s": round(wall, 2), "ticks": st["ticks"], "hbm_peak_bytes": st["hbm_peak"]}
-    log("served: " + json.dumps(out))
+    lat = [i["latency_s"] for i in infos]
+    out["poisson"] = {"rate_per_s": round(rate, 3), "completed": st["completed"], "failed": st["failed"],
+                      "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
+                      "mean_s": round(st["mean_s"], 4), "ginter_per_s": round(st["ginter_per_s"], 2),
+                      "wall_s": round(wall, 2), "ticks": st["ticks"], "hbm_peak_bytes": st["hbm_peak"],
+                      "latency_by_n": _latency_stats(lat, ns[pick])}
+    log("served poisson: " + json.dumps(out["poisson"]))
+    # ---- saturation: every request at once ----------------------------------------------
+    srv = Server(hbm_budget=16 << 30, max_active=64)
+    try:
+        sids = open_all(srv)
+        srv.wait(srv.submit(sids[int(np.argmin(ns))], 1e-3, "comp"))
+        srv.reset_stats()
+        t_start = time.perf_counter()
+        rids = [srv.submit(sids[pick[j]], 1e-3, str(prec[j])) for j in range(n_req)]
+        infos = [srv.wait(r) for r in rids]
+        wall = time.perf_counter() - t_start
+        st = srv.stats()
+    finally:
+        srv.close()
+    lat = [i["latency_s"] for i in infos]
+    out["saturated"] = {"completed": st["completed"], "failed": st["failed"], "wall_s": round(wall, 3),
+                        "ginter_per_s": round(inter / wall / 1e9, 2), "ticks": st["ticks"],
+                        "factored_steps": st["factored_steps"], "latency_by_n": _latency_stats(lat, ns[pick])}
+    log("served saturated: " + json.dumps(out["saturated"]))
+    # ---- the same requests as plain per-instance calls ------------------------------------
+    L = _lib.lib
+    D = _lib.DeviceArray.from_host
+    dev = []
+    for p, core in sess:
+        dpos, dstr, drad = D(p.pos), D(p.str), D(p.rad)
+        dv, dg = _lib.DeviceArray((3, p.n)), _lib.DeviceArray((9, p.n))
+        pt = _lib.Points()
+        pt.n = p.n
+        for d in range(3):
+            pt.x[d], pt.s[d], pt.u[d] = dpos.row(d), dstr.row(d), dv.row(d)
+        pt.r = drad.ptr
+        for k in range(9):
+            pt.ug[k] = dg.row(k)
+        dev.append((pt, core, (dpos, dstr, drad, dv, dg)))
+    opts = {(core, pr): _lib.make_opts(core, True, pr) for _, core in sess for pr in ("comp", "fp32")}
+
+    def plain(j):
+        pt, core, _ = dev[pick[j]]
+        o = opts[(core, str(prec[j]))]
+        for _ in range(2):
+            _lib.check(L.o3d_points_affect_points_dev(C.byref(pt), C.byref(pt), C.byref(o), None))
+
+    for j in sorted({int(np.argmax(pick == k)) for k in set(pick.tolist())}):
+        plain(j)                 # every session once: workspaces grown before the clock starts
+    _lib.synchronize()
+    t_start = time.perf_counter()
+    for j in range(n_req):
+        plain(j)
+    _lib.synchronize()
+    wall = time.perf_counter() - t_start
+    out["per_instance"] = {"wall_s": round(wall, 3), "ginter_per_s": round(inter / wall / 1e9, 2)}
+    out["served_vs_per_instance"] = round(out["saturated"]["ginter_per_s"] / out["per_instance"]["ginter_per_s"], 3)
+    log("served per-instance: " + json.dumps(out["per_instance"]))
+    del dev
+    # ---- FP64 spot errors of the largest session ------------------------------------------
+    big = int(np.argmax(ns))
+    p, core = sess[big]
+    from . import influence
+    sel = np.linspace(0, p.n - 1, 1024).astype(np.int64)
+    vr, gr = _fp64_self(core, p, sel)
+    spot = {"session": big, "n": p.n, "core": core, "targets": len(sel),
+            "method": "factored" if core in ("rm", "exp") and p.n >= 65536 else "direct"}
+    for pr in ("comp", "fp32"):
+        v, g = influence.p2p(core, p.pos, p.str, p.rad, p.pos, p.rad, True, pr)
+        spot[pr] = field_errors(v[:, sel], g[:, sel], vr, gr)
+    out["fp64_spot"] = spot
+    log("served fp64 spot: " + json.dumps(spot))
+    # headline figures of the Poisson run, as before
+    out.update({"rate_per_s": out["poisson"]["rate_per_s"], "p50_s": out["poisson"]["p50_s"],
+                "p99_s": out["poisson"]["p99_s"], "ginter_per_s": out["poisson"]["ginter_per_s"]})
     return out
 
 
