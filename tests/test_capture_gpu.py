@@ -145,9 +145,10 @@ def test_bem_matrix_refuses_capture(env):
 
 
 def test_captured_exp_call_skips_target_order_with_same_bits(env):
-    """A direct EXP call of 40,000 targets visits them in Morton order when run
-    eagerly and in index order under capture (the sort records memory fills): the
-    replays give the eager result's bits."""
+    """A direct EXP call of 40,000 targets visits them in Morton order (a target's
+    sums do not depend on the order).  Captured after the eager call has sized the
+    stream's workspace, the order's sort -- kernels only -- is recorded too, and
+    the replays give the eager result's bits."""
     _lib, hip = env
     from omega3d_amd.batch import stream_create, stream_destroy
     st = stream_create()
@@ -183,3 +184,54 @@ def test_captured_exp_call_skips_target_order_with_same_bits(env):
             hip.hipGraphDestroy(graph)
     finally:
         stream_destroy(st)
+
+
+def test_captured_factored_call_replays_bitwise(env):
+    """An AUTO call of 70,000 particles (RM, compensated) takes the factored method:
+    Morton keys, the library's radix sorts and scans, far and near kernels -- all
+    kernel launches.  Captured after an eager call of the same size on the stream,
+    it records the whole pipeline and replays the eager bits, with eager work in
+    between; captured on a fresh stream, whose workspace cannot grow, it falls back
+    to the direct kernel and replays that kernel's bits."""
+    _lib, hip = env
+    from omega3d_amd.batch import stream_create, stream_destroy
+    n = 70000
+    for fresh in (False, True):
+        st = stream_create()
+        try:
+            keep, dv, dg, s, o = _setup(_lib, n, st)
+            o.self_term = 0
+            call = lambda: _lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None)
+            if fresh:
+                od = _lib.make_opts("rm", True, "comp", tile=512, tpt=1, stream=st, method="direct")
+                ref_call = lambda: _lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(od), None)
+                st2 = stream_create()
+                try:
+                    od.stream = st2
+                    dv.zero(); dg.zero(); _lib.synchronize()
+                    _lib.check(ref_call())
+                    _lib.check(_lib.lib.o3d_stream_synchronize(st2))
+                finally:
+                    stream_destroy(st2)
+            else:
+                dv.zero(); dg.zero(); _lib.synchronize()
+                _lib.check(call())
+                _lib.check(_lib.lib.o3d_stream_synchronize(st))
+            v0, g0 = dv.to_host(), dg.to_host()
+            print(f"capture: factored AUTO n={n} fresh={fresh}", flush=True)
+            graph, exe = _capture(_lib, hip, st, call)
+            try:
+                for rep in range(3):
+                    dv.zero(); dg.zero(); _lib.synchronize()
+                    assert hip.hipGraphLaunch(exe, C.c_void_p(st)) == 0
+                    _lib.check(_lib.lib.o3d_stream_synchronize(st))
+                    np.testing.assert_array_equal(dv.to_host(), v0)
+                    np.testing.assert_array_equal(dg.to_host(), g0)
+                    p = scenes.ring(3000, sigma=0.05)
+                    from omega3d_amd import influence
+                    influence.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp")   # eager work
+            finally:
+                hip.hipGraphExecDestroy(exe)
+                hip.hipGraphDestroy(graph)
+        finally:
+            stream_destroy(st)
