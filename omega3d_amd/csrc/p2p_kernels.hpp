@@ -494,8 +494,12 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
 
 // a.tperm (optional): slot i of the launch evaluates target a.tperm[i]; a target's
 // sums do not depend on the slot, so any order gives the same bits
+// one target per lane: at least 7 waves per SIMD (at most 72 VGPRs), 1% faster on
+// the two-ring workload than the 6 the compiler's own allocation (80 VGPRs) leaves;
+// the compensated EXP kernel then keeps a few values of its rare exponential branch
+// in scratch.  More targets per lane need their registers.
 template <int CORE, bool THICK, bool GRAD, int PREC, int TPT>
-__global__ __launch_bounds__(BLOCK) void k_p2p(P2PArgs a) {
+__global__ __launch_bounds__(BLOCK, TPT == 1 ? 7 : 1) void k_p2p(P2PArgs a) {
   extern __shared__ float4 sh[];
   const int64_t t_base = (int64_t)blockIdx.x * (BLOCK * TPT);
   p2p_span<CORE, THICK, GRAD, PREC, TPT, OUT_ADD>(a.s, a.t, 0, a.ns, a.nt, t_base, a.tile, a.self_excl != 0, sh,
