@@ -278,6 +278,36 @@ int point_order(const float* x, const float* y, const float* z, int64_t n, void*
 
 bool ff_same_points(const o3d_points* src, const o3d_points* targ) { return same_points(src, targ); }
 
+// Sources of a direct EXP call as rows sorted by perm (the targets' Morton order of
+// the same collection), and each tile's bounding sphere and largest core term, in
+// ws (sorted_sources_bytes) after the point order's own point_order_bytes(n)
+size_t sorted_sources_bytes(int64_t n, int tile) {
+  const size_t ntiles = (size_t)((n + tile - 1) / tile);
+  const auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  return 2 * r(sizeof(float4) * (size_t)n) + r(sizeof(float4) * ntiles) + r(sizeof(float) * ntiles);
+}
+
+int sorted_sources(const o3d_points* src, const int32_t* perm, int tile, void* ws, hipStream_t st, P2PArgs* a) {
+  const int64_t n = src->n;
+  const int ntiles = (int)((n + tile - 1) / tile);
+  const auto r = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  char* p = (char*)ws;
+  float4* sp = (float4*)p;
+  float4* sa = (float4*)(p + r(sizeof(float4) * (size_t)n));
+  float4* sph = (float4*)(p + 2 * r(sizeof(float4) * (size_t)n));
+  float* cs = (float*)(p + 2 * r(sizeof(float4) * (size_t)n) + r(sizeof(float4) * (size_t)ntiles));
+  const SrcView sv{src->x[0], src->x[1], src->x[2], src->r, src->s[0], src->s[1], src->s[2]};
+  hipLaunchKernelGGL((k_ff_gather_src<CORE_EXP>), dim3(grid_of(n, 256)), dim3(256), 0, st, sv, perm, n, sp, sa);
+  hipLaunchKernelGGL(k_ff_bounds, dim3((unsigned)ntiles), dim3(256), 0, st, (const float4*)sp, (const float4*)nullptr,
+                     n, tile, sph, cs, (float*)nullptr);
+  HIPCHK(hipGetLastError());
+  a->sp = sp;
+  a->sa = sa;
+  a->tsph = sph;
+  a->tcs = cs;
+  return O3D_OK;
+}
+
 std::mutex& stream_ws_mutex() { return g_ws_mu; }
 
 // caller holds stream_ws_mutex() for the whole enqueue, so that no other thread

@@ -331,8 +331,17 @@ int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipSt
   // not depend on the order: the results are the same bits.  Under graph capture
   // the order needs the stream's workspace to be large enough already; otherwise
   // the targets are visited in index order.
+  //
+  // A call on one collection with the configuration left to the library also reads
+  // its sources as rows sorted in that order: a tile of sorted sources is compact,
+  // and a tile beyond the far-branch distance of every target of a wave is summed
+  // with no per-pair test.  The sums then run in another order (the pair values
+  // are the same bits).  Not for a caller's fixed scratch (ws).
   if (o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX) {
     const size_t bytes = point_order_bytes(a.nt);
+    const bool want_sorted = !ws && auto_cfg && tpt == 1 && ff_same_points(src, targ);
+    const size_t sbytes = want_sorted ? bytes + sorted_sources_bytes(a.ns, tile) : bytes;
+    bool sorted = false;
     void* ow = ws;
     std::unique_lock<std::mutex> lk(stream_ws_mutex(), std::defer_lock);
     if (ws) {
@@ -341,15 +350,22 @@ int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipSt
                     (long long)a.nt, bytes, ws_bytes);
     } else {
       lk.lock();
-      if (stream_capturing(st) && !stream_workspace_fits(st, bytes)) ow = nullptr;
-      else {
+      const bool cap = stream_capturing(st);
+      if (!cap || stream_workspace_fits(st, sbytes)) {
+        const int rc = stream_workspace(st, sbytes, &ow);
+        if (rc != O3D_OK) return rc;
+        sorted = want_sorted;
+      } else if (stream_workspace_fits(st, bytes)) {
         const int rc = stream_workspace(st, bytes, &ow);
         if (rc != O3D_OK) return rc;
+      } else {
+        ow = nullptr;
       }
     }
     if (ow) {
-      const int rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ow, st, &a.tperm);
+      int rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ow, st, &a.tperm);
       if (rc != O3D_OK) return rc;
+      if (sorted && (rc = sorted_sources(src, a.tperm, tile, (char*)ow + bytes, st, &a)) != O3D_OK) return rc;
       hipError_t e = dispatch_p2p(o->core, a, thick, grad, o->precision, tpt, st);
       if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
       return O3D_OK;

@@ -51,6 +51,11 @@ bool stream_workspace_fits(hipStream_t st, size_t bytes);
 // Morton order of n points into a workspace of point_order_bytes(n): *perm gets the
 // permutation (slot -> point index), *keys (optional) the sorted 30-bit keys
 size_t point_order_bytes(int64_t n);
+// a direct EXP call's sources as rows in the targets' Morton order (one collection)
+// with tile spheres, for k_p2p (P2PArgs::sp, sa, tsph, tcs): scratch bytes, and the
+// enqueue into ws
+size_t sorted_sources_bytes(int64_t n, int tile);
+int sorted_sources(const o3d_points* src, const int32_t* perm, int tile, void* ws, hipStream_t st, P2PArgs* a);
 int point_order(const float* x, const float* y, const float* z, int64_t n, void* ws, hipStream_t st,
                 const int32_t** perm, const unsigned** keys = nullptr);
 // stable radix sort of (key, value) pairs by the low `bits` bits of the keys

@@ -31,6 +31,13 @@ struct P2PArgs {
   int32_t tile;        // sources per LDS tile (multiple of 64, <= 1024)
   int32_t self_excl;   // subtract the j == i pair (src and targ are one collection)
   const int32_t* tperm;   // k_p2p only: target order (nullptr: index order)
+  // k_p2p only, EXP core: sources staged from rows sorted in the targets' Morton
+  // order ({x, y, z, core term}, {alpha, 0}), with each tile's bounding sphere and
+  // largest core term (nullptr: the SoA arrays in index order)
+  const float4* sp;
+  const float4* sa;
+  const float4* tsph;
+  const float* tcs;
 };
 
 // double-precision views for the FP64 oracle kernel
@@ -55,6 +62,7 @@ __device__ __forceinline__ float fexp2(float x) { return __builtin_amdgcn_exp2f(
 __device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+__device__ __forceinline__ bool wave_all(bool p) { return __builtin_amdgcn_ballot_w64(!p) == 0; }
 
 // sum over the 64 lanes of a wave by a fixed butterfly (the same order on every run)
 __device__ __forceinline__ double wave_sum(double x) {

@@ -196,6 +196,22 @@ __device__ __forceinline__ void source_acc(const float* tx, const float* ty, con
   for (int k = 0; k < TPT; ++k) acc_pair<GRAD>(dx[k], dy[k], dz[k], K[k], Bp[k], A, acc[k]);
 }
 
+// one source against the lane's targets when every pair of the tile is known to be
+// in the exponential core's far branch: exp_far for all, no test (the same values,
+// bit for bit, as source_acc gives those pairs)
+template <bool GRAD, int TPT>
+__device__ __forceinline__ void source_acc_far(const float* tx, const float* ty, const float* tz, const float4 P,
+                                               const float4 A, float (*acc)[NACC]) {
+#pragma unroll
+  for (int k = 0; k < TPT; ++k) {
+    const float dx = tx[k] - P.x, dy = ty[k] - P.y, dz = tz[k] - P.z;
+    const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+    float K, Bp;
+    exp_far(d2, K, Bp);
+    acc_pair<GRAD>(dx, dy, dz, K, Bp, A, acc[k]);
+  }
+}
+
 template <int CORE, bool GRAD>
 __device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt, float thr, const float4 P,
                                          const float4 A, float* acc) {
@@ -293,6 +309,20 @@ __device__ __forceinline__ void stage_tile(const SrcView& s, int64_t s0, int cnt
   if (CORE == CORE_EXP) atomicMax(tmax, mx);
 }
 
+// stage sorted rows [s0, s0 + cnt) of sp/sa into LDS (pads: zero strength, unit core)
+__device__ __forceinline__ void stage_sorted(const float4* sp, const float4* sa, int64_t s0, int cnt, int tile,
+                                             float4* sh) {
+  for (int j = threadIdx.x; j < tile; j += BLOCK) {
+    float4 P = make_float4(0.f, 0.f, 0.f, 1.f), A = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (j < cnt) {
+      P = sp[s0 + j];
+      A = sa[s0 + j];
+    }
+    sh[2 * j] = P;
+    sh[2 * j + 1] = A;
+  }
+}
+
 // what p2p_span does with a chunk's sums: overwrite the outputs, add into them (the
 // reference's semantics), or store them as one slice of a source-split evaluation
 // (no self term; k_p2p_combine adds the slices in order and finishes).
@@ -311,10 +341,16 @@ struct PartCols {
 // [s_lo, s_hi) of (s, t), in tiles that start at s_lo.  Shared by the
 // single-collection, batched and serving kernels so that a target's result does not
 // depend on which launch computed it.  part/slice: OUT_PART's destination.
+//
+// ssp, ssa, stsph, stcs (k_p2p, EXP core only; s_lo = 0): the sources come from rows sorted in the
+// targets' Morton order with tile spheres, and a tile whose sphere is beyond the
+// far-branch distance of every target of the wave is summed with no per-pair test.
 template <int CORE, bool THICK, bool GRAD, int PREC, int TPT, int OUT>
 __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int64_t s_lo, int64_t s_hi, int64_t nt,
                                          int64_t t_base, int tile, bool self_excl, float4* sh, void* part,
-                                         int64_t slice, const int32_t* tperm = nullptr) {
+                                         int64_t slice, const int32_t* tperm = nullptr,
+                                         const float4* ssp = nullptr, const float4* ssa = nullptr,
+                                         const float4* stsph = nullptr, const float* stcs = nullptr) {
   __shared__ unsigned int s_tmax;
   const int tid = threadIdx.x;
   float tx[TPT], ty[TPT], tz[TPT], tt[TPT];
@@ -340,27 +376,52 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
     for (int c = 0; c < 3; ++c) { hi[k][c] = 0.0f; lo[k][c] = 0.0f; }
   }
 
+  const bool srt = CORE == CORE_EXP && ssp != nullptr;
   for (int64_t s0 = s_lo; s0 < s_hi; s0 += tile) {
     const int cnt = (int)((s_hi - s0) < tile ? (s_hi - s0) : tile);
     __syncthreads();
     if (CORE == CORE_EXP && tid == 0) s_tmax = 0u;
     __syncthreads();
-    stage_tile<CORE>(s, s0, cnt, tile, sh, &s_tmax);
+    if (srt) stage_sorted(ssp, ssa, s0, cnt, tile, sh);
+    else stage_tile<CORE>(s, s0, cnt, tile, sh, &s_tmax);
     __syncthreads();
+    const int64_t ti = srt ? (s0 - s_lo) / tile : 0;
+    const float tmax = srt ? stcs[ti] : __uint_as_float(s_tmax);
     float thr[TPT];
 #pragma unroll
-    for (int k = 0; k < TPT; ++k) thr[k] = CORE == CORE_EXP ? exp_threshold(__uint_as_float(s_tmax) + tt[k]) : 0.0f;
+    for (int k = 0; k < TPT; ++k) thr[k] = CORE == CORE_EXP ? exp_threshold(tmax + tt[k]) : 0.0f;
+    bool far_tile = false;
+    if (srt) {
+      // every pair of the tile is farther than the tile sphere's gap; with a margin
+      // of 1e-5 on the squared gap the exact far test (rho^6 > 256 cs^2) holds for
+      // each of them, so they take the far formulas in any case
+      const float4 c = stsph[ti];
+      bool all = true;
+#pragma unroll
+      for (int k = 0; k < TPT; ++k) {
+        const float ex = tx[k] - c.x, ey = ty[k] - c.y, ez = tz[k] - c.z;
+        const float gap = __fsqrt_rn(fmaf(ez, ez, fmaf(ey, ey, ex * ex))) - c.w;
+        all = all && gap > 0.0f && gap * gap > thr[k] * 1.00001f;
+      }
+      far_tile = wave_all(all);
+    }
     if (PREC == PREC_COMP && GRAD) {
       for (int j0 = 0; j0 < cnt; j0 += P2P_GFOLD) {
         const int j1 = j0 + P2P_GFOLD < cnt ? j0 + P2P_GFOLD : cnt;
-        for (int j = j0; j < j1; ++j) {
-          const float4 P = sh[2 * j];
-          const float4 A = sh[2 * j + 1];
-          source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
+        if (far_tile) {
+          for (int j = j0; j < j1; ++j) source_acc_far<GRAD, TPT>(tx, ty, tz, sh[2 * j], sh[2 * j + 1], acc);
+        } else {
+          for (int j = j0; j < j1; ++j) {
+            const float4 P = sh[2 * j];
+            const float4 A = sh[2 * j + 1];
+            source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
+          }
         }
 #pragma unroll
         for (int k = 0; k < TPT; ++k) fold_grad_raw(acc[k], gr[k]);
       }
+    } else if (far_tile) {
+      for (int j = 0; j < cnt; ++j) source_acc_far<GRAD, TPT>(tx, ty, tz, sh[2 * j], sh[2 * j + 1], acc);
     } else {
       for (int j = 0; j < cnt; ++j) {
         const float4 P = sh[2 * j];
@@ -494,16 +555,16 @@ __device__ __forceinline__ void p2p_chunk(const SrcView& s, const TgtView& t, in
 
 // a.tperm (optional): slot i of the launch evaluates target a.tperm[i]; a target's
 // sums do not depend on the slot, so any order gives the same bits
-// one target per lane: at least 7 waves per SIMD (at most 72 VGPRs), 1% faster on
-// the two-ring workload than the 6 the compiler's own allocation (80 VGPRs) leaves;
-// the compensated EXP kernel then keeps a few values of its rare exponential branch
-// in scratch.  More targets per lane need their registers.
+// one target per lane: at least 6 waves per SIMD (at most 80 VGPRs).  Measured on
+// the two-ring workload (compensated, sorted sources): 6 waves 704.5 ms, 7 waves
+// (72 VGPRs, a few values of the rare exponential branch in scratch) 710, 8 waves
+// 709.  More targets per lane need their registers.
 template <int CORE, bool THICK, bool GRAD, int PREC, int TPT>
-__global__ __launch_bounds__(BLOCK, TPT == 1 ? 7 : 1) void k_p2p(P2PArgs a) {
+__global__ __launch_bounds__(BLOCK, TPT == 1 ? 6 : 1) void k_p2p(P2PArgs a) {
   extern __shared__ float4 sh[];
   const int64_t t_base = (int64_t)blockIdx.x * (BLOCK * TPT);
   p2p_span<CORE, THICK, GRAD, PREC, TPT, OUT_ADD>(a.s, a.t, 0, a.ns, a.nt, t_base, a.tile, a.self_excl != 0, sh,
-                                                  nullptr, 0, a.tperm);
+                                                  nullptr, 0, a.tperm, a.sp, a.sa, a.tsph, a.tcs);
 }
 
 // ---------------------------------------------------------------------------

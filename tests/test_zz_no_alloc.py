@@ -4,7 +4,9 @@ o3d_points_affect_points_dev (method DIRECT) through the C ABI, on device arrays
 Three calls in a row, each smaller than the one before, so that the static scratch
 keeps a larger call's contents when the next one starts:
   * two rings of 524,288 particles (EXP, compensated): the Morton target order runs
-    in the static scratch; the results must be the _dev call's bits;
+    in the static scratch; the _dev call also reads the sources in that order (from
+    sorted rows in its own workspace), this call in index order, so the sums run in
+    another order: within 1e-6 (normwise relative) of the _dev call;
   * a ring of 20,000 particles (RM, compensated): the _dev call splits the sources
     into more slices than the static scratch holds partial sums for, so this call
     takes fewer slices: within 1e-6 (normwise relative) of the _dev call, and
@@ -57,7 +59,7 @@ def _fp64(L, p, core, sel):
 
 def test_noalloc_matches_dev_on_three_calls(L):
     na, dev = L.lib.o3d_points_affect_points_noalloc_dev, L.lib.o3d_points_affect_points_dev
-    cases = [(scenes.colliding_rings(n_each=524288), "exp", "comp", 0, True),
+    cases = [(scenes.colliding_rings(n_each=524288), "exp", "comp", 0, False),
              (scenes.ring(20000, sigma=0.03, seed=scenes.SEED_BASE + 7), "rm", "comp", 0, False),
              (scenes.ring(4096, sigma=0.05, seed=scenes.SEED_BASE + 8), "wl", "fp32", 1, True)]
     for p, core, prec, self_term, bitwise in cases:
