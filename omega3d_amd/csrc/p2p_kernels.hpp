@@ -212,6 +212,25 @@ __device__ __forceinline__ void source_acc_far(const float* tx, const float* ty,
   }
 }
 
+// source_acc_far over the staged sources [j0, j1), U at a time.  The LDS address is
+// kept in a vector register (made so by an opaque zero from the assembler), so the
+// reads of the U sources take it with immediate offsets and one add advances it;
+// an address known to be uniform lives in a scalar register and is moved to a
+// vector register before every read.
+template <bool GRAD, int TPT, int U>
+__device__ __forceinline__ void sources_acc_far(const float* tx, const float* ty, const float* tz, const float4* sh,
+                                                int j0, int j1, float (*acc)[NACC]) {
+  int vz;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+  const float4* q = sh + 2 * j0 + vz;
+  int j = j0;
+  for (; j + U <= j1; j += U, q += 2 * U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) source_acc_far<GRAD, TPT>(tx, ty, tz, q[2 * u], q[2 * u + 1], acc);
+  }
+  for (; j < j1; ++j, q += 2) source_acc_far<GRAD, TPT>(tx, ty, tz, q[0], q[1], acc);
+}
+
 template <int CORE, bool GRAD>
 __device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt, float thr, const float4 P,
                                          const float4 A, float* acc) {
@@ -267,6 +286,7 @@ __device__ __forceinline__ void grad_from(const T* a, T* g) {
 // 4e-6 from FP64, at 64 sources 2e-7).  The gradient is formed from the FP64 sums
 // once per target (grad_from).
 constexpr int P2P_GFOLD = 64;
+constexpr int P2P_FAR_UNROLL = 4;   // sources per step of the far-tile loop (two rings, compensated: 704.5 ms at 1, 681.9 at 2, 680.5 at 4)
 __device__ __forceinline__ void fold_vel_comp(float* acc, float* hi, float* lo) {
   two_sum(hi[0], lo[0], acc[0]);
   two_sum(hi[1], lo[1], acc[1]);
@@ -405,23 +425,29 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
       }
       far_tile = wave_all(all);
     }
-    if (PREC == PREC_COMP && GRAD) {
+    if (PREC == PREC_COMP) {
       for (int j0 = 0; j0 < cnt; j0 += P2P_GFOLD) {
         const int j1 = j0 + P2P_GFOLD < cnt ? j0 + P2P_GFOLD : cnt;
         if (far_tile) {
-          for (int j = j0; j < j1; ++j) source_acc_far<GRAD, TPT>(tx, ty, tz, sh[2 * j], sh[2 * j + 1], acc);
+          sources_acc_far<GRAD, TPT, P2P_FAR_UNROLL>(tx, ty, tz, sh, j0, j1, acc);
         } else {
           for (int j = j0; j < j1; ++j) {
             const float4 P = sh[2 * j];
             const float4 A = sh[2 * j + 1];
             source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
           }
-        }
+          // a tile that may hold a target's nearest sources (in sorted order they
+          // gather in few tiles): its velocity partials are folded as often
 #pragma unroll
-        for (int k = 0; k < TPT; ++k) fold_grad_raw(acc[k], gr[k]);
+          for (int k = 0; k < TPT; ++k) fold_vel_comp(acc[k], hi[k], lo[k]);
+        }
+        if (GRAD) {
+#pragma unroll
+          for (int k = 0; k < TPT; ++k) fold_grad_raw(acc[k], gr[k]);
+        }
       }
     } else if (far_tile) {
-      for (int j = 0; j < cnt; ++j) source_acc_far<GRAD, TPT>(tx, ty, tz, sh[2 * j], sh[2 * j + 1], acc);
+      sources_acc_far<GRAD, TPT, P2P_FAR_UNROLL>(tx, ty, tz, sh, 0, cnt, acc);
     } else {
       for (int j = 0; j < cnt; ++j) {
         const float4 P = sh[2 * j];
