@@ -231,6 +231,24 @@ __device__ __forceinline__ void sources_acc_far(const float* tx, const float* ty
   for (; j < j1; ++j, q += 2) source_acc_far<GRAD, TPT>(tx, ty, tz, q[0], q[1], acc);
 }
 
+// source_acc over the staged sources [j0, j1), U at a time, with the LDS address in
+// a vector register as in sources_acc_far: for the cores whose pair code has no
+// branch (RM, WL, V2)
+template <int CORE, bool GRAD, int TPT, int U>
+__device__ __forceinline__ void sources_acc_u(const float* tx, const float* ty, const float* tz, const float* tt,
+                                              const float* thr, const float4* sh, int j0, int j1,
+                                              float (*acc)[NACC]) {
+  int vz;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+  const float4* q = sh + 2 * j0 + vz;
+  int j = j0;
+  for (; j + U <= j1; j += U, q += 2 * U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, q[2 * u], q[2 * u + 1], acc);
+  }
+  for (; j < j1; ++j, q += 2) source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, q[0], q[1], acc);
+}
+
 template <int CORE, bool GRAD>
 __device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt, float thr, const float4 P,
                                          const float4 A, float* acc) {
@@ -431,10 +449,14 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
         if (far_tile) {
           sources_acc_far<GRAD, TPT, P2P_FAR_UNROLL>(tx, ty, tz, sh, j0, j1, acc);
         } else {
-          for (int j = j0; j < j1; ++j) {
-            const float4 P = sh[2 * j];
-            const float4 A = sh[2 * j + 1];
-            source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
+          if (CORE != CORE_EXP && TPT == 1) {
+            sources_acc_u<CORE, GRAD, TPT, P2P_FAR_UNROLL>(tx, ty, tz, tt, thr, sh, j0, j1, acc);
+          } else {
+            for (int j = j0; j < j1; ++j) {
+              const float4 P = sh[2 * j];
+              const float4 A = sh[2 * j + 1];
+              source_acc<CORE, GRAD, TPT>(tx, ty, tz, tt, thr, P, A, acc);
+            }
           }
           // a tile that may hold a target's nearest sources (in sorted order they
           // gather in few tiles): its velocity partials are folded as often
@@ -448,6 +470,8 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
       }
     } else if (far_tile) {
       sources_acc_far<GRAD, TPT, P2P_FAR_UNROLL>(tx, ty, tz, sh, 0, cnt, acc);
+    } else if (CORE != CORE_EXP && TPT == 1) {
+      sources_acc_u<CORE, GRAD, TPT, P2P_FAR_UNROLL>(tx, ty, tz, tt, thr, sh, 0, cnt, acc);
     } else {
       for (int j = 0; j < cnt; ++j) {
         const float4 P = sh[2 * j];
