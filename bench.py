@@ -311,7 +311,8 @@ def main():
     log(f"bench: device {dev['name']} CUs {dev['cus']}")
     if args.valu_probe:
         out = {}
-        for op, name in enumerate(("v_fmac_f32", "v_add_f32", "v_mul_f32", "v_rsq_f32")):
+        for op, name in enumerate(("v_fmac_f32 (scalar-register operand)", "v_add_f32", "v_mul_f32", "v_rsq_f32",
+                                   "v_fmac_f32 (vector-register operands)")):
             ms, wi = C.c_float(), C.c_double()
             _lib.check(L.o3d_valu_probe(op, C.byref(ms), C.byref(wi)))
             simd_cycles = dev["cus"] * 4 * dev["clock_khz"] * 1e3 * ms.value * 1e-3

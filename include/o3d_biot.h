@@ -55,7 +55,9 @@ enum o3d_self { O3D_SELF_INCLUDE = 0, O3D_SELF_EXCLUDE = 1 };
 
 /* How the point->point sums are organised.  Both give the same direct sums over
  * every pair; they differ in rounding only.
- *   DIRECT:   one pair at a time on the vector ALUs, sources in index order.
+ *   DIRECT:   one pair at a time on the vector ALUs, sources in index order (an
+ *             EXP call of >= 32,768 targets on one collection, tile and tpt left
+ *             at 0, reads them in the targets' Morton order instead).
  *   FACTORED: particles in Morton order; a source tile well separated from a
  *             target block is summed through an exact factorisation
  *             K = Bp (|t'|^2 - 2 t'.s' + |s'|^2 + cores) as a product of the pair
@@ -507,7 +509,8 @@ int o3d_event_elapsed_ms(float* ms, void* start, void* stop);
 int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream);
 /* VALU issue-rate probe (no counterpart in the reference; calibrates the FP32
  * rooflines): a grid of 8 workgroups per CU runs 16 independent chains of one
- * instruction per lane (op 0 v_fmac_f32, 1 v_add_f32, 2 v_mul_f32, 3 v_rsq_f32);
+ * instruction per lane (op 0 v_fmac_f32 with a scalar-register operand, 1 v_add_f32,
+ * 2 v_mul_f32, 3 v_rsq_f32, 4 v_fmac_f32 with vector-register operands only);
  * reports the second launch's time and its wave64 instruction count.  Synchronous. */
 int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions);
 /* device name, CU count, clock (kHz), HBM bytes */

@@ -496,14 +496,16 @@ __global__ void k_copy_probe(float4* __restrict__ d, const float4* __restrict__ 
 }
 
 // VALU issue-rate probe: 16 independent chains of one FP32 instruction per lane
-// (op 0 v_fmac_f32, 1 v_add_f32, 2 v_mul_f32, 3 v_rsq_f32), iters times
+// (op 0 v_fmac_f32 with a scalar-register operand, 1 v_add_f32, 2 v_mul_f32,
+// 3 v_rsq_f32, 4 v_fmac_f32 with vector-register operands only), iters times
 template <int OP>
 __global__ __launch_bounds__(BLOCK) void k_valu_probe(float* out, int iters, float b) {
-  float x[16], y[16];
+  float x[16], y[16], z[16];
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     x[j] = threadIdx.x * 1e-3f + j;
     y[j] = 1.0f + (j + 1) * 1e-4f;   // never 1: every chain keeps its instruction
+    z[j] = 1e-7f * (float)(threadIdx.x + j + 1);   // per lane: not folded into a literal
   }
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
@@ -511,7 +513,8 @@ __global__ __launch_bounds__(BLOCK) void k_valu_probe(float* out, int iters, flo
       if (OP == 0) x[j] = fmaf(y[j], b, x[j]);
       else if (OP == 1) x[j] = x[j] + y[j];
       else if (OP == 2) x[j] = x[j] * y[j];
-      else x[j] = __builtin_amdgcn_rsqf(x[j]);
+      else if (OP == 3) x[j] = __builtin_amdgcn_rsqf(x[j]);
+      else x[j] = fmaf(z[j], z[(j + 1) & 15], x[j]);
     }
   }
   float s = 0.f;
@@ -738,7 +741,7 @@ int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream) {
 
 int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
   clear_error();
-  if (op < 0 || op > 3 || !ms || !wave_instructions) return fail(O3D_EINVAL, "valu probe: op 0..3 and outputs");
+  if (op < 0 || op > 4 || !ms || !wave_instructions) return fail(O3D_EINVAL, "valu probe: op 0..4 and outputs");
   int rc = ensure_init();
   if (rc) return rc;
   const int blocks = 8 * num_cus(), iters = op == 3 ? 1024 : 4096;
@@ -755,6 +758,7 @@ int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
     if (op == 1) hipLaunchKernelGGL(k_valu_probe<1>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
     if (op == 2) hipLaunchKernelGGL(k_valu_probe<2>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
     if (op == 3) hipLaunchKernelGGL(k_valu_probe<3>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
+    if (op == 4) hipLaunchKernelGGL(k_valu_probe<4>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, st));
     HIPCHK(hipEventSynchronize(e1));
