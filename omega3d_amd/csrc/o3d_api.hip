@@ -404,6 +404,11 @@ int p2p_host_impl(const P* src, P* targ, const o3d_opts* o, o3d_stats* stats) {
   std::lock_guard<std::mutex> lk(g_pool_mu);
   hipStream_t st = g_ctx.stream;
   const size_t sb = sizeof(T) * (size_t)ns, tb = sizeof(T) * (size_t)nt;
+  // a collection acting on itself (the same host position arrays) has its
+  // positions copied once, so the device call sees one collection; the radii
+  // are shared too when they are the same host array
+  const bool alias = ns == nt && src->x[0] == targ->x[0] && src->x[1] == targ->x[1] && src->x[2] == targ->x[2];
+  const bool alias_r = alias && targ->r == src->r;
   void* dsrc;
   void* dtgt;
   int rc;
@@ -419,11 +424,12 @@ int p2p_host_impl(const P* src, P* targ, const o3d_opts* o, o3d_stats* stats) {
   for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync((void*)ds.s[d], src->s[d], sb, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync((void*)ds.r, src->r, sb, hipMemcpyHostToDevice, st));
   dt.n = nt;
-  for (int d = 0; d < 3; ++d) { dt.x[d] = t + d * nt; dt.u[d] = t + (3 + d) * nt; }
-  dt.r = thick ? t + 6 * nt : nullptr;
+  for (int d = 0; d < 3; ++d) { dt.x[d] = alias ? ds.x[d] : t + d * nt; dt.u[d] = t + (3 + d) * nt; }
+  dt.r = thick ? (alias_r ? ds.r : t + 6 * nt) : nullptr;
   if (grad) for (int k = 0; k < 9; ++k) dt.ug[k] = t + (7 + k) * nt;
-  for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync((void*)dt.x[d], targ->x[d], tb, hipMemcpyHostToDevice, st));
-  if (thick) HIPCHK(hipMemcpyAsync((void*)dt.r, targ->r, tb, hipMemcpyHostToDevice, st));
+  if (!alias)
+    for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync((void*)dt.x[d], targ->x[d], tb, hipMemcpyHostToDevice, st));
+  if (thick && !alias_r) HIPCHK(hipMemcpyAsync((void*)dt.r, targ->r, tb, hipMemcpyHostToDevice, st));
   for (int d = 0; d < 3; ++d) HIPCHK(hipMemcpyAsync(dt.u[d], targ->u[d], tb, hipMemcpyHostToDevice, st));
   if (grad) for (int k = 0; k < 9; ++k) HIPCHK(hipMemcpyAsync(dt.ug[k], targ->ug[k], tb, hipMemcpyHostToDevice, st));
   if constexpr (sizeof(T) == 4) rc = p2p_dev((const o3d_points*)&ds, (o3d_points*)&dt, o, st, nullptr);
