@@ -304,6 +304,9 @@ __device__ __forceinline__ void grad_from(const T* a, T* g) {
 // 4e-6 from FP64, at 64 sources 2e-7).  The gradient is formed from the FP64 sums
 // once per target (grad_from).
 constexpr int P2P_GFOLD = 64;
+// sources per gradient fold in a tile known to be far from every target of the
+// wave (sorted sources only): no term of it is a nearest source's
+constexpr int P2P_GFOLD_FAR = 512;
 constexpr int P2P_FAR_UNROLL = 4;   // sources per step of the far-tile loop (two rings, compensated: 704.5 ms at 1, 681.9 at 2, 680.5 at 4)
 __device__ __forceinline__ void fold_vel_comp(float* acc, float* hi, float* lo) {
   two_sum(hi[0], lo[0], acc[0]);
@@ -444,8 +447,9 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
       far_tile = wave_all(all);
     }
     if (PREC == PREC_COMP) {
-      for (int j0 = 0; j0 < cnt; j0 += P2P_GFOLD) {
-        const int j1 = j0 + P2P_GFOLD < cnt ? j0 + P2P_GFOLD : cnt;
+      const int fold = far_tile ? P2P_GFOLD_FAR : P2P_GFOLD;
+      for (int j0 = 0; j0 < cnt; j0 += fold) {
+        const int j1 = j0 + fold < cnt ? j0 + fold : cnt;
         if (far_tile) {
           sources_acc_far<GRAD, TPT, P2P_FAR_UNROLL>(tx, ty, tz, sh, j0, j1, acc);
         } else {
