@@ -161,11 +161,13 @@ __device__ __forceinline__ void grad_of(const R* a, R* g) {
   g[8] = a[11] + a[15];
 }
 
-// stage panels [p0, p0 + cnt) into LDS:
+// a panel's record (written once per call by k_pan_geom, staged into LDS by k_pan):
 //   f0 = {x0,y0,z0,x1}  f1 = {y1,z1,x2,y2}
-//   f2 = {z2, level-0 absolute vortex strength (ts/area)*area, float}
-//   f3 = {level-0 absolute source strength ss*area, area, sqrt(area), 0}
-//   f4 = {cx, cy, cz, 0} (root centroid)   f5 = {tsx, tsy, tsz, ss} (raw inputs)
+//   f2 = {level-0 absolute vortex strength (ts/area)*area (3 floats),
+//         level-0 absolute source strength ss*area}
+//   f3 = {cx, cy, cz (root centroid), far2}   f4 = {z2, area, sqrt(area), near2}
+//   f5 = {tsx, tsy, tsz, ss} (raw inputs)
+// so that a surely separated pair reads f2 and f3 only.
 // The reference passes the sheet strength ts/area and multiplies it back by the
 // area at level 0 (Influence.h:737, Kernels.h:1136-1146); that round trip is kept.
 // squared-distance bands of the root decision (the reference's float test
@@ -196,27 +198,30 @@ __device__ __forceinline__ void panel_record(const PanArgs& a, int64_t p, float4
     }
     s[0] = make_float4(x0, y0, z0, x1);
     s[1] = make_float4(y1, z1, x2, y2);
-    s[2] = make_float4(z2, gx, gy, gz);
+    s[2] = make_float4(gx, gy, gz, gs);
     float far2, near2;
     root_bands(sqrtf(ar), far2, near2);
-    s[3] = make_float4(gs, ar, sqrtf(ar), far2);
-    s[4] = make_float4(cx, cy, cz, near2);
+    s[3] = make_float4(cx, cy, cz, far2);
+    s[4] = make_float4(z2, ar, sqrtf(ar), near2);
     s[5] = make_float4(tx, ty, tz, sg);
   }
 }
 
-__device__ __forceinline__ void stage_panels(const PanArgs& a, int64_t p0, int cnt, float4* sh) {
-  for (int j = threadIdx.x; j < cnt; j += BLOCK) panel_record(a, p0 + j, sh + PAN_F4 * j);
+// copy the records of panels [p0, p0 + cnt) into LDS
+__device__ __forceinline__ void stage_panels(const float4* rec, int64_t p0, int cnt, float4* sh) {
+  const float4* r = rec + PAN_F4 * p0;
+  for (int f = threadIdx.x; f < PAN_F4 * cnt; f += BLOCK) sh[f] = r[f];
 }
 
-// the records of all panels in global memory, for the near kernel, and (cen, when
-// given) each panel's root centroid with its far band, the near kernel's prefilter
+// the records of all panels in global memory, for the root and near kernels, and
+// (cen, when given) each panel's root centroid with its far band, the near
+// kernel's prefilter
 static __global__ __launch_bounds__(BLOCK) void k_pan_geom(PanArgs a, float4* rec, float4* cen) {
   const int64_t p = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (p >= a.np) return;
   float4* s = rec + PAN_F4 * p;
   panel_record(a, p, s);
-  if (cen) cen[p] = make_float4(s[4].x, s[4].y, s[4].z, s[3].w);
+  if (cen) cen[p] = s[3];
 }
 
 // a panel's root quantities: level-0 absolute strengths (Kernels.h:1136-1146),
@@ -228,11 +233,11 @@ template <class R>
 __device__ __forceinline__ void panel_root(const float4* s, R& gx, R& gy, R& gz, R& gs, R& cx0, R& cy0, R& cz0,
                                            R& lim, RootLim& rl) {
   const float4 f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3], f4 = s[4];
-  const R ar = f3.y;
+  const R ar = f4.y;
   if constexpr (sizeof(R) == 4) {
-    gx = f2.y; gy = f2.z; gz = f2.w; gs = f3.x;
-    cx0 = f4.x; cy0 = f4.y; cz0 = f4.z;
-    lim = f3.z * 4.0f;
+    gx = f2.x; gy = f2.y; gz = f2.z; gs = f2.w;
+    cx0 = f3.x; cy0 = f3.y; cz0 = f3.z;
+    lim = f4.z * 4.0f;
     rl = RootLim{lim, f3.w, f4.w};
   } else {
     rl = RootLim{0.0f, 0.0f, 0.0f};
@@ -241,7 +246,7 @@ __device__ __forceinline__ void panel_root(const float4* s, R& gx, R& gy, R& gz,
 #pragma clang fp contract(off)
       gx = (R(f5.x) / ar) * ar; gy = (R(f5.y) / ar) * ar; gz = (R(f5.z) / ar) * ar; gs = R(f5.w) * ar;
     }
-    centroid_exact<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f2.x, cx0, cy0, cz0);
+    centroid_exact<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w, f4.x, cx0, cy0, cz0);
     lim = sqrt(ar) * R(4);
   }
 }
@@ -277,8 +282,8 @@ __device__ __forceinline__ bool root_leaf(R tx, R ty, R tz, R cx0, R cy0, R cz0,
 // one (target, panel) tree walk, accumulating into acc
 template <class R, bool GRAD>
 __device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, int maxlev, bool active, R* acc) {
-  const float4 f0 = s[0], f1 = s[1], f2 = s[2], f3 = s[3];
-  const R ar = f3.y;
+  const float4 f0 = s[0], f1 = s[1], f4 = s[4];
+  const R ar = f4.y;
   R gx, gy, gz, gs, cx0, cy0, cz0, lim;
   RootLim rl;
   panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
@@ -300,7 +305,7 @@ __device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, in
         leaf = leaf0;
         cx = cx0; cy = cy0; cz = cz0;
       } else {
-        R t[9] = {R(f0.x), R(f0.y), R(f0.z), R(f0.w), R(f1.x), R(f1.y), R(f1.z), R(f1.w), R(f2.x)};
+        R t[9] = {R(f0.x), R(f0.y), R(f0.z), R(f0.w), R(f1.x), R(f1.y), R(f1.z), R(f1.w), R(f4.x)};
         R sa = ar;
         for (int l = 1; l <= lev; ++l) {
           child_of<R>(t, (path >> (2 * (l - 1))) & 3);
@@ -330,8 +335,25 @@ __device__ __forceinline__ void walk_panel(R tx, R ty, R tz, const float4* s, in
   }
 }
 
+// the root decision of one pair: a leaf is evaluated, a recursed pair noted
+template <class R, bool GRAD>
+__device__ __forceinline__ void pan_root_pair(const float4* s, R tx, R ty, R tz, int maxlev, bool active, R* acc,
+                                              bool& has_near) {
+  R gx, gy, gz, gs, cx0, cy0, cz0, lim;
+  RootLim rl;
+  panel_root<R>(s, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
+  if (root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, rl, maxlev, active)) {
+    if (active) leaf_acc<R, GRAD>(tx, ty, tz, cx0, cy0, cz0, gx, gy, gz, gs, acc);
+  } else {
+    has_near = true;
+  }
+}
+
+constexpr int PAN_UNROLL = 4;   // panels per step of k_pan's separated-pair loop
+
+// rec: the records of all panels (k_pan_geom)
 template <bool GRAD, int PREC>
-__global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
+__global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a, const float4* rec) {
   using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
   extern __shared__ float4 shp[];
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -347,18 +369,40 @@ __global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a) {
   for (int64_t p0 = 0; p0 < a.np; p0 += a.tile) {
     const int cnt = (int)((a.np - p0) < a.tile ? (a.np - p0) : a.tile);
     __syncthreads();
-    stage_panels(a, p0, cnt, shp);
+    stage_panels(rec, p0, cnt, shp);
     __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
-      R gx, gy, gz, gs, cx0, cy0, cz0, lim;
-      RootLim rl;
-      panel_root<R>(shp + PAN_F4 * j, gx, gy, gz, gs, cx0, cy0, cz0, lim, rl);
-      if (root_leaf<R>(tx, ty, tz, cx0, cy0, cz0, lim, rl, a.maxlev, active)) {
-        if (active) leaf_acc<R, GRAD>(tx, ty, tz, cx0, cy0, cz0, gx, gy, gz, gs, acc);
-      } else {
-        has_near = true;
+    int j = 0;
+    if constexpr (sizeof(R) == 4) {
+      // PAN_UNROLL panels at a time while every lane's pairs are surely leaves at the
+      // root (d2 beyond the far band): one wave-uniform branch, the leaves with no
+      // per-pair mask, the LDS address in a vector register (made so by an opaque
+      // zero) with immediate offsets.  The same leaves, in the same order and with
+      // the same bits, as the per-pair path below; an idle lane's sums are dropped.
+      int vz;
+      asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+      const float4* q = shp + vz;
+      for (; j + PAN_UNROLL <= cnt; j += PAN_UNROLL, q += PAN_F4 * PAN_UNROLL) {
+        float4 g[PAN_UNROLL], c[PAN_UNROLL];
+        uint64_t near = 0;   // lanes with a pair not surely a leaf, as a scalar mask
+#pragma unroll
+        for (int u = 0; u < PAN_UNROLL; ++u) {
+          g[u] = q[PAN_F4 * u + 2];
+          c[u] = q[PAN_F4 * u + 3];
+          const float ex = tx - c[u].x, ey = ty - c[u].y, ez = tz - c[u].z;
+          near |= __builtin_amdgcn_ballot_w64(!(fmaf(ez, ez, fmaf(ey, ey, ex * ex)) > c[u].w));
+        }
+        if (near == 0) {
+#pragma unroll
+          for (int u = 0; u < PAN_UNROLL; ++u)
+            leaf_acc<R, GRAD>(tx, ty, tz, c[u].x, c[u].y, c[u].z, g[u].x, g[u].y, g[u].z, g[u].w, acc);
+        } else {
+#pragma unroll
+          for (int u = 0; u < PAN_UNROLL; ++u) pan_root_pair<R, GRAD>(q + PAN_F4 * u, tx, ty, tz, a.maxlev, active,
+                                                                      acc, has_near);
+        }
       }
     }
+    for (; j < cnt; ++j) pan_root_pair<R, GRAD>(shp + PAN_F4 * j, tx, ty, tz, a.maxlev, active, acc, has_near);
     if (PREC == PREC_COMP) {
       two_sum(hi[0], lo[0], (float)acc[0]);
       two_sum(hi[1], lo[1], (float)acc[1]);

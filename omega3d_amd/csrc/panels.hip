@@ -34,7 +34,7 @@ static hipError_t launch_pan(const PanArgs& a, bool grad, int prec, hipStream_t 
   hipError_t e = hipErrorInvalidValue;
 #define O3D_PL(G, P)                                                                      \
   if (grad == G && prec == P) {                                                           \
-    hipLaunchKernelGGL((k_pan<G, P>), grid, dim3(BLOCK), lds, st, a);                     \
+    hipLaunchKernelGGL((k_pan<G, P>), grid, dim3(BLOCK), lds, st, a, (const float4*)rec);  \
     e = hipGetLastError();                                                                \
     if (e == hipSuccess) {                                                                \
       hipLaunchKernelGGL((k_pan_near<G, P>), grid_near, dim3(BLOCK), 0, st, a, (const float4*)rec, \

@@ -63,11 +63,11 @@ __device__ __forceinline__ void ptpan_record(const PtPanArgs& a, int64_t i, floa
   centroid_exact(x0, y0, z0, x1, y1, z1, x2, y2, z2, cx, cy, cz);
   s[0] = make_float4(x0, y0, z0, x1);
   s[1] = make_float4(y1, z1, x2, y2);
-  s[2] = make_float4(z2, 0.0f, 0.0f, 0.0f);
+  s[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float far2, near2;
   root_bands(sqrtf(ar), far2, near2);
-  s[3] = make_float4(0.0f, ar, sqrtf(ar), far2);
-  s[4] = make_float4(cx, cy, cz, near2);
+  s[3] = make_float4(cx, cy, cz, far2);
+  s[4] = make_float4(z2, ar, sqrtf(ar), near2);
   s[5] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
 }
 
@@ -79,7 +79,7 @@ __device__ __forceinline__ void ptpan_record(const PtPanArgs& a, int64_t i, floa
 template <int PREC>
 __device__ __forceinline__ void ptpan_strength(float4* s, float4 A) {
   if (PREC == PREC_FP64) s[5] = make_float4(A.x, A.y, A.z, 0.0f);
-  else { s[2].y = A.x; s[2].z = A.y; s[2].w = A.z; }
+  else { s[2].x = A.x; s[2].y = A.y; s[2].z = A.z; }
 }
 
 // One lane per target panel, particles [y*slen, (y+1)*slen) of slice y =
@@ -97,7 +97,7 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
   const int64_t ic = active ? i : a.np - 1;
   float4 s[PAN_F4];
   ptpan_record(a, ic, s);
-  const float ar = s[3].y;
+  const float ar = s[4].y;
   R acc[16];
 #pragma unroll
   for (int c = 0; c < 16; ++c) acc[c] = R(0);
