@@ -351,9 +351,12 @@ __device__ __forceinline__ void pan_root_pair(const float4* s, R tx, R ty, R tz,
 
 constexpr int PAN_UNROLL = 4;   // panels per step of k_pan's separated-pair loop
 
-// rec: the records of all panels (k_pan_geom)
+// rec: the records of all panels (k_pan_geom).  At least 5 waves per SIMD (96
+// VGPRs; the FP64 gradient sums spill once per tile, outside the panel loop): on
+// the icosphere 1.7% faster than the 103 VGPRs and 4 waves the compiler chooses,
+// and 6 waves (80 VGPRs) 2% slower.
 template <bool GRAD, int PREC>
-__global__ __launch_bounds__(BLOCK) void k_pan(PanArgs a, const float4* rec) {
+__global__ __launch_bounds__(BLOCK, 5) void k_pan(PanArgs a, const float4* rec) {
   using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
   extern __shared__ float4 shp[];
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
