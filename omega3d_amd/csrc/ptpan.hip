@@ -88,6 +88,8 @@ __device__ __forceinline__ void ptpan_strength(float4* s, float4 A) {
 // the root.  Each slice leaves its sum (FP64) in part[y][0..2][panel], the count
 // of pairs that need the recursion in cnt and the first PTPAN_CAP of their
 // particles in ent; k_ptpan_combine adds the slices in order.
+constexpr int PTPAN_UNROLL = 4;   // particles per step of k_ptpan's separated-pair loop
+
 template <int PREC>
 __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
   using R = typename std::conditional<PREC == PREC_FP64, double, float>::type;
@@ -119,10 +121,11 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
       shs[2 * j + 1] = make_float4(a.ax[g], a.ay[g], a.az[g], 0.0f);
     }
     __syncthreads();
-    for (int j = 0; j < cnt; ++j) {
-      const float4 P = shs[2 * j], A = shs[2 * j + 1];
+    // the root decision of particle j of the tile: a leaf is evaluated, a recursed
+    // pair listed
+    auto pair = [&](const float4 P, const float4 A, int j) {
       if (root_leaf<R>(P.x, P.y, P.z, cx0, cy0, cz0, lim, rl, a.maxlev, active)) {
-        if (!active) continue;
+        if (!active) return;
         R ax, ay, az;
         if (PREC == PREC_FP64) {
 #pragma clang fp contract(off)
@@ -135,7 +138,38 @@ __global__ __launch_bounds__(BLOCK) void k_ptpan(PtPanArgs a) {
         if (nnear < PTPAN_CAP) ent[nnear] = (int32_t)(p0 + j);
         ++nnear;
       }
+    };
+    int j = 0;
+    if constexpr (sizeof(R) == 4) {
+      // PTPAN_UNROLL particles at a time while they are surely beyond every lane's
+      // far band: one scalar branch on the OR of the ballots, the leaves with no
+      // per-pair mask, the LDS address in a vector register (made so by an opaque
+      // zero); the same leaves in the same order as the per-pair path, and an idle
+      // lane's sums are dropped
+      int vz;
+      asm volatile("v_mov_b32 %0, 0" : "=v"(vz));
+      const float4* q = shs + vz;
+      for (; j + PTPAN_UNROLL <= cnt; j += PTPAN_UNROLL, q += 2 * PTPAN_UNROLL) {
+        float4 P[PTPAN_UNROLL], A[PTPAN_UNROLL];
+        uint64_t near = 0;
+#pragma unroll
+        for (int u = 0; u < PTPAN_UNROLL; ++u) {
+          P[u] = q[2 * u];
+          A[u] = q[2 * u + 1];
+          const float ex = P[u].x - cx0, ey = P[u].y - cy0, ez = P[u].z - cz0;
+          near |= __builtin_amdgcn_ballot_w64(!(fmaf(ez, ez, fmaf(ey, ey, ex * ex)) > rl.far2));
+        }
+        if (near == 0) {
+#pragma unroll
+          for (int u = 0; u < PTPAN_UNROLL; ++u)
+            leaf_acc<R, false>(P[u].x, P[u].y, P[u].z, cx0, cy0, cz0, A[u].x, A[u].y, A[u].z, R(0), acc);
+        } else {
+#pragma unroll
+          for (int u = 0; u < PTPAN_UNROLL; ++u) pair(P[u], A[u], j + u);
+        }
+      }
     }
+    for (; j < cnt; ++j) pair(shs[2 * j], shs[2 * j + 1], j);
     if (PREC == PREC_COMP) {
       two_sum(hi[0], lo[0], (float)acc[0]);
       two_sum(hi[1], lo[1], (float)acc[1]);
