@@ -318,7 +318,20 @@ def main():
             simd_cycles = dev["cus"] * 4 * dev["clock_khz"] * 1e3 * ms.value * 1e-3
             out[name] = {"ms": round(ms.value, 4), "wave_instructions": wi.value,
                          "cycles_per_wave_instruction_at_max_clock": round(simd_cycles / wi.value, 3)}
-        print(json.dumps({"valu_probe": out, "clock_mhz": dev["clock_khz"] // 1000, "cus": dev["cus"]}))
+        # one wave per SIMD: FMAs alone, MFMAs alone, both in the same waves, and one
+        # MFMA wave beside one FMA wave (two per SIMD); each probe iteration of a wave
+        # is 64 FMAs and/or 4 v_mfma_f32_32x32x16_f16
+        mix = {}
+        for op, name in ((5, "fma waves, 1 per SIMD"), (6, "mfma waves, 1 per SIMD"),
+                         (7, "mfma and fma in the same wave, 1 per SIMD"),
+                         (8, "1 mfma wave + 1 fma wave per SIMD")):
+            ms, wi = C.c_float(), C.c_double()
+            _lib.check(L.o3d_valu_probe(op, C.byref(ms), C.byref(wi)))
+            waves = dev["cus"] * 4
+            iters = wi.value / waves / (64.0 if op == 5 else 4.0)
+            mix[name] = {"ms": round(ms.value, 4), "ns_per_iteration_per_wave": round(ms.value * 1e6 / iters, 2)}
+        print(json.dumps({"valu_probe": out, "mfma_valu_overlap": mix, "clock_mhz": dev["clock_khz"] // 1000,
+                          "cus": dev["cus"]}))
         return
 
     if args.bem_only:

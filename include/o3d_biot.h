@@ -510,8 +510,11 @@ int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream);
 /* VALU issue-rate probe (no counterpart in the reference; calibrates the FP32
  * rooflines): a grid of 8 workgroups per CU runs 16 independent chains of one
  * instruction per lane (op 0 v_fmac_f32 with a scalar-register operand, 1 v_add_f32,
- * 2 v_mul_f32, 3 v_rsq_f32, 4 v_fmac_f32 with vector-register operands only);
- * reports the second launch's time and its wave64 instruction count.  Synchronous. */
+ * 2 v_mul_f32, 3 v_rsq_f32, 4 v_fmac_f32 with vector-register operands only, all at
+ * 32 waves per CU); ops 5-8 at one wave per SIMD: 5 v_fmac_f32 (vector-register
+ * operands), 6 v_mfma_f32_32x32x16_f16, 7 both in the same waves (16 FMAs per MFMA),
+ * 8 one MFMA wave and one FMA wave per SIMD.  Reports the second launch's time and
+ * its wave64 count of the probed instruction (ops 6-8: the MFMAs).  Synchronous. */
 int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions);
 /* device name, CU count, clock (kHz), HBM bytes */
 int o3d_device_info(char* name, int name_len, int* cus, int* clock_khz, size_t* hbm_bytes);

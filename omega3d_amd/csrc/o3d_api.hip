@@ -529,6 +529,56 @@ __global__ __launch_bounds__(BLOCK) void k_valu_probe(float* out, int iters, flo
   out[(int64_t)blockIdx.x * BLOCK + threadIdx.x] = s;
 }
 
+// matrix-core and VALU issue together: per iteration four independent
+// v_mfma_f32_32x32x16_f16 (accumulators in registers) and/or 64 v_fmac_f32 with
+// vector-register operands (16 independent chains).  MODE 0: FMAs only; 1: MFMAs
+// only; 2: both in every wave; 3: waves 0-3 of the block MFMAs, waves 4-7 FMAs.
+typedef float o3d_f32x16 __attribute__((ext_vector_type(16)));
+typedef _Float16 o3d_f16x8 __attribute__((ext_vector_type(8)));
+template <int MODE>
+__global__ __launch_bounds__(512, 1) void k_mix_probe(float* out, int iters) {
+  const int wave = threadIdx.x >> 6;
+  const bool mf = MODE == 1 || MODE == 2 || (MODE == 3 && wave < 4);
+  const bool va = MODE == 0 || MODE == 2 || (MODE == 3 && wave >= 4);
+  o3d_f32x16 acc[4];
+  o3d_f16x8 A, B;
+  float x[16], z[16];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[m][r] = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    A[e] = (_Float16)(1e-3f * (float)(threadIdx.x + e));
+    B[e] = (_Float16)(1e-3f * (float)e);
+  }
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    x[j] = threadIdx.x * 1e-3f + j;
+    z[j] = 1e-7f * (float)(threadIdx.x + j + 1);
+  }
+  for (int it = 0; it < iters; ++it) {
+    if (mf) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) acc[m] = __builtin_amdgcn_mfma_f32_32x32x16_f16(A, B, acc[m], 0, 0, 0);
+    }
+    if (va) {
+#pragma unroll
+      for (int rep = 0; rep < 4; ++rep)
+#pragma unroll
+        for (int j = 0; j < 16; ++j) x[j] = fmaf(z[j], z[(j + 1) & 15], x[j]);
+    }
+  }
+  float s = 0.0f;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) s += acc[m][r];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) s += x[j];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 }  // namespace o3d
 
 using namespace o3d;
@@ -747,10 +797,13 @@ int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream) {
 
 int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
   clear_error();
-  if (op < 0 || op > 4 || !ms || !wave_instructions) return fail(O3D_EINVAL, "valu probe: op 0..4 and outputs");
+  if (op < 0 || op > 8 || !ms || !wave_instructions) return fail(O3D_EINVAL, "valu probe: op 0..8 and outputs");
   int rc = ensure_init();
   if (rc) return rc;
-  const int blocks = 8 * num_cus(), iters = op == 3 ? 1024 : 4096;
+  // ops 0-4: 8 workgroups of 256 lanes per CU; ops 5-7: one workgroup of 256 lanes
+  // per CU (one wave per SIMD); op 8: one of 512 lanes per CU (two waves per SIMD)
+  const int blocks = op < 5 ? 8 * num_cus() : num_cus(), iters = op == 3 ? 1024 : op < 5 ? 4096 : 8192;
+  const int threads = op == 8 ? 512 : BLOCK;
   std::lock_guard<std::mutex> lk(pool_mutex());
   void* buf = nullptr;
   if ((rc = pool_buffer(4, sizeof(float) * (size_t)blocks * BLOCK, &buf)) != O3D_OK) return rc;
@@ -765,6 +818,10 @@ int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
     if (op == 2) hipLaunchKernelGGL(k_valu_probe<2>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
     if (op == 3) hipLaunchKernelGGL(k_valu_probe<3>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
     if (op == 4) hipLaunchKernelGGL(k_valu_probe<4>, dim3(blocks), dim3(BLOCK), 0, st, (float*)buf, iters, 0.f);
+    if (op == 5) hipLaunchKernelGGL(k_mix_probe<0>, dim3(blocks), dim3(threads), 0, st, (float*)buf, iters);
+    if (op == 6) hipLaunchKernelGGL(k_mix_probe<1>, dim3(blocks), dim3(threads), 0, st, (float*)buf, iters);
+    if (op == 7) hipLaunchKernelGGL(k_mix_probe<2>, dim3(blocks), dim3(threads), 0, st, (float*)buf, iters);
+    if (op == 8) hipLaunchKernelGGL(k_mix_probe<3>, dim3(blocks), dim3(threads), 0, st, (float*)buf, iters);
     HIPCHK(hipGetLastError());
     HIPCHK(hipEventRecord(e1, st));
     HIPCHK(hipEventSynchronize(e1));
@@ -772,7 +829,10 @@ int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
   HIPCHK(hipEventElapsedTime(ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  *wave_instructions = (double)blocks * (BLOCK / 64) * (double)iters * 16.0;
+  // ops 0-4: the probed instruction; 5: the FMAs; 6-8: the MFMAs (op 7 also issues
+  // 16 FMAs per MFMA in the same waves, op 8 as many in the other waves)
+  const double waves = (double)blocks * (op < 5 ? BLOCK / 64 : 4);
+  *wave_instructions = waves * (double)iters * (op < 5 ? 16.0 : op == 5 ? 64.0 : 4.0);
   return O3D_OK;
 }
 
