@@ -382,13 +382,25 @@ def bem_matrix_run(levels=5, nunk=3, reps=3, sample_sources=32, log=print):
         for k in range(nunk):
             _lib.check(L.o3d_memcpy_dtoh(col.ctypes.data, out.ptr + 4 * (int(j) * nunk + k) * R, 4 * R))
             cols[:, a * nunk + k] = col
+    # the same block written by the runtime's own fill (hipMemset) with no arithmetic:
+    # the write rate this GPU reaches for these bytes, the practical ceiling of k_bm_far
+    fill = []
+    for _ in range(reps + 1):
+        _lib.synchronize()
+        t = time.perf_counter()
+        out.zero()
+        _lib.synchronize()
+        fill.append((time.perf_counter() - t) * 1e3)
+    fill_ms = min(fill[1:])
     out.free()
     res = {"workload": f"BEM influence matrix of the {P}-panel icosphere on itself, {nunk} unknowns per panel, "
                        f"maxlev 3: {R} x {R} floats ({gb:.2f} GB)",
            "call_ms": round(call_ms, 3), "count_ms": round(best.count_ms, 3), "far_ms": round(best.far_ms, 3),
            "near_ms": round(best.near_ms, 3), "recursed_pairs": int(best.near_pairs), "panel_pairs": P * P,
            "matrix_gb": round(gb, 3), "call_write_gbs": round(gb / call_ms * 1e3, 1),
-           "far_write_gbs": round(gb / best.far_ms * 1e3, 1)}
+           "far_write_gbs": round(gb / best.far_ms * 1e3, 1),
+           "fill_ms": round(fill_ms, 3), "fill_write_gbs": round(gb / fill_ms * 1e3, 1),
+           "far_write_vs_fill": round(fill_ms / best.far_ms, 3)}
     log("bem_matrix: " + json.dumps(res))
     sample = {"nodes": nodes, "idx": idx, "area": area, "sources": js, "columns": cols, "nunk": nunk}
     return res, sample
