@@ -458,33 +458,33 @@ __global__ __launch_bounds__(BLOCK) void k_bm_near(BmNearArgs a) {
     } else {
       // level 1 on lanes 0..15; the open pairs' triangles go to LDS for the items
       bool open1 = false;
+      float sv1[9], tv1[9];
       if (lane < 16) {
-        float sv1[9], tv1[9];
         bm_child(s0, lane >> 2, sv1);
         bm_child(t0, lane & 3, tv1);
         float ex, ey, ez, d2;
-        if (bm_sep(sv1, tv1, t41, ex, ey, ez, d2)) {
-          bm_leaf_d(ex, ey, ez, d2, 0.0625, G);
-        } else {
-          open1 = true;
-          float4* row = (float4*)(l1 + lane * BM_L1);
-          row[0] = make_float4(sv1[0], sv1[1], sv1[2], sv1[3]);
-          row[1] = make_float4(sv1[4], sv1[5], sv1[6], sv1[7]);
-          row[2] = make_float4(sv1[8], tv1[0], tv1[1], tv1[2]);
-          row[3] = make_float4(tv1[3], tv1[4], tv1[5], tv1[6]);
-          row[4] = make_float4(tv1[7], tv1[8], 0.f, 0.f);
-        }
+        if (bm_sep(sv1, tv1, t41, ex, ey, ez, d2)) bm_leaf_d(ex, ey, ez, d2, 0.0625, G);
+        else open1 = true;
       }
       const unsigned long long m1 = __builtin_amdgcn_ballot_w64(open1);
+      if (open1) {
+        // the open pairs' rows in lane order, so that item k reads row k / 16
+        const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(m1 >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)m1, 0u));
+        float4* row = (float4*)(l1 + rank * BM_L1);
+        row[0] = make_float4(sv1[0], sv1[1], sv1[2], sv1[3]);
+        row[1] = make_float4(sv1[4], sv1[5], sv1[6], sv1[7]);
+        row[2] = make_float4(sv1[8], tv1[0], tv1[1], tv1[2]);
+        row[3] = make_float4(tv1[3], tv1[4], tv1[5], tv1[6]);
+        row[4] = make_float4(tv1[7], tv1[8], 0.f, 0.f);
+      }
       __builtin_amdgcn_wave_barrier();
       const int items = 16 * __builtin_popcountll(m1);
       for (int k = lane; k < items; k += 64) {
-        unsigned long long m = m1;
-        for (int t = 0; t < (k >> 4); ++t) m &= m - 1;
-        const int c1 = __builtin_ctzll(m), c2 = k & 15;
+        const int c2 = k & 15;
         float s1v[9], t1v[9];
         {
-          const float4* row = (const float4*)(l1 + c1 * BM_L1);
+          const float4* row = (const float4*)(l1 + (k >> 4) * BM_L1);
           const float4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3], r4 = row[4];
           s1v[0] = r0.x; s1v[1] = r0.y; s1v[2] = r0.z; s1v[3] = r0.w;
           s1v[4] = r1.x; s1v[5] = r1.y; s1v[6] = r1.z; s1v[7] = r1.w; s1v[8] = r2.x;
