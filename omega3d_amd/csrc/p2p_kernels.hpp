@@ -162,6 +162,30 @@ __device__ __forceinline__ void acc_pair(float dx, float dy, float dz, float K, 
   }
 }
 
+// K and Bp of one pair from its separation d and core parameter st.  RM and WL need
+// rho^2 only as rho^2 + st: that sum is the FMA chain of |d|^2 started at st (one
+// add fewer per pair than |d|^2 then + st; at d = 0 it is st exactly, the self
+// pair's value of kb).  WL's numerator rho^2 + 2.5 st is then (rho^2 + st) + 1.5 st.
+template <int CORE>
+__device__ __forceinline__ void kb_d(float dx, float dy, float dz, float st, float thr, float& K, float& Bp) {
+  if (CORE == CORE_RM || CORE == CORE_WL) {
+    const float dd = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, st)));
+    const float ir = frsq(dd);
+    const float ir2 = ir * ir;
+    if (CORE == CORE_RM) {                     // CoreFunc.h:65-71
+      K = ir2 * ir;
+      Bp = K * ir2;
+    } else {                                   // CoreFunc.h:265-274
+      const float top = fmaf(1.5f, st, dd);
+      const float dn5 = ir2 * ir2 * ir;
+      K = top * dn5;
+      Bp = dn5 * fmaf((float)(5.0 / 3.0), top * ir2, (float)(-2.0 / 3.0));
+    }
+  } else {
+    kb<CORE>(fmaf(dz, dz, fmaf(dy, dy, dx * dx)), st, thr, K, Bp);
+  }
+}
+
 // one source against the TPT targets of this lane.  For the exponential core the
 // far-branch test of all TPT pairs is combined into a single wave-uniform branch,
 // so that the common case is one basic block holding all TPT pair evaluations.
@@ -190,7 +214,7 @@ __device__ __forceinline__ void source_acc(const float* tx, const float* ty, con
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < TPT; ++k) kb<CORE>(d2[k], P.w + tt[k], thr[k], K[k], Bp[k]);
+    for (int k = 0; k < TPT; ++k) kb_d<CORE>(dx[k], dy[k], dz[k], P.w + tt[k], thr[k], K[k], Bp[k]);
   }
 #pragma unroll
   for (int k = 0; k < TPT; ++k) acc_pair<GRAD>(dx[k], dy[k], dz[k], K[k], Bp[k], A, acc[k]);
@@ -255,9 +279,8 @@ __device__ __forceinline__ void pair_acc(float tx, float ty, float tz, float tt,
   const float dx = tx - P.x;
   const float dy = ty - P.y;
   const float dz = tz - P.z;
-  const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
   float K, Bp;
-  kb<CORE>(d2, P.w + tt, thr, K, Bp);
+  kb_d<CORE>(dx, dy, dz, P.w + tt, thr, K, Bp);
   const float cx = fmaf(dz, A.y, -dy * A.z);
   const float cy = fmaf(dx, A.z, -dz * A.x);
   const float cz = fmaf(dy, A.x, -dx * A.y);
