@@ -5,7 +5,8 @@ from the stream's device workspace, which cannot grow while the stream is being
 captured.  Captured after a call of the same size on that stream, the call replays
 bitwise equal to its eager result, on every replay, with eager work in between.
 Captured on a stream whose workspace is too small, it takes the one-pass kernel
-instead (the same sum in another order).
+instead (the same sum in another order).  The same holds for the target order and
+sorted source rows of large EXP calls, and for the factored method.
 """
 import ctypes as C
 
@@ -182,6 +183,59 @@ def test_captured_exp_call_skips_target_order_with_same_bits(env):
         finally:
             hip.hipGraphExecDestroy(exe)
             hip.hipGraphDestroy(graph)
+    finally:
+        stream_destroy(st)
+
+
+def test_captured_sorted_exp_call_replays_bitwise(env):
+    """A direct EXP call on one collection of 1,048,576 particles, configuration left
+    to the library, orders its targets and reads its sources as sorted rows (sort,
+    gather and tile bounds: kernels only).  Captured after the eager call has sized
+    the stream's workspace, every replay gives the eager result's bits, with eager
+    work in between; and those bits differ from the index-order sum of an explicit
+    tile, which shows that the sorted path was the one captured."""
+    _lib, hip = env
+    from omega3d_amd.batch import stream_create, stream_destroy
+    st = stream_create()
+    try:
+        p = scenes.colliding_rings(n_each=524288)
+        n = p.n
+        D = _lib.DeviceArray.from_host
+        keep = [D(p.pos), D(p.str), D(p.rad)]
+        dv, dg = _lib.DeviceArray((3, n)), _lib.DeviceArray((9, n))
+        s = _lib.Points()
+        s.n = n
+        for d in range(3):
+            s.x[d], s.s[d], s.u[d] = keep[0].row(d), keep[1].row(d), dv.row(d)
+        s.r = keep[2].ptr
+        for k in range(9):
+            s.ug[k] = dg.row(k)
+        o = _lib.make_opts("exp", True, "comp", stream=st, method="direct")
+        call = lambda: _lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None)
+        dv.zero(); dg.zero(); _lib.synchronize()
+        _lib.check(call())
+        _lib.check(_lib.lib.o3d_stream_synchronize(st))
+        v0, g0 = dv.to_host(), dg.to_host()
+        print(f"sorted capture: N={n} eager done", flush=True)
+        graph, exe = _capture(_lib, hip, st, call)
+        try:
+            for rep in range(3):
+                dv.zero(); dg.zero(); _lib.synchronize()
+                assert hip.hipGraphLaunch(exe, C.c_void_p(st)) == 0
+                _lib.check(_lib.lib.o3d_stream_synchronize(st))
+                np.testing.assert_array_equal(dv.to_host(), v0)
+                np.testing.assert_array_equal(dg.to_host(), g0)
+                print(f"sorted capture: replay {rep} equal", flush=True)
+        finally:
+            hip.hipGraphExecDestroy(exe)
+            hip.hipGraphDestroy(graph)
+        ou = _lib.make_opts("exp", True, "comp", tile=512, tpt=1, stream=st, method="direct")
+        dv.zero(); dg.zero(); _lib.synchronize()
+        _lib.check(_lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(ou), None))
+        _lib.check(_lib.lib.o3d_stream_synchronize(st))
+        vu, gu = dv.to_host(), dg.to_host()
+        assert not (np.array_equal(vu, v0) and np.array_equal(gu, g0))
+        assert rel_errors(v0, vu)[0] <= 1e-6 and rel_errors(g0, gu)[0] <= 1e-6
     finally:
         stream_destroy(st)
 
