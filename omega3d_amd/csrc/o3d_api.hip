@@ -309,17 +309,38 @@ int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipSt
                                        : 1;
   if (nsl > 1) {
     void* part = ws;
+    // an EXP call on one collection also takes its targets in Morton order and its
+    // sources as sorted rows, as the one-pass kernel below does (in the stream's
+    // workspace after the partial sums); the partials are then in that order
+    const bool ord = !ws && o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX &&
+                     ff_same_points(src, targ);
+    const size_t pal = (pbytes + 255) & ~(size_t)255;
+    const size_t obytes = ord ? point_order_bytes(a.nt) + sorted_sources_bytes(a.ns, tile) : 0;
+    bool sorted = false;
     std::unique_lock<std::mutex> lk(stream_ws_mutex(), std::defer_lock);
     if (!ws) {
       lk.lock();
-      // under graph capture the workspace cannot grow: the one-pass kernel then
-      if (stream_capturing(st) && !stream_workspace_fits(st, pbytes)) part = nullptr;
-      else {
+      // under graph capture the workspace cannot grow: without room for the order,
+      // index order; without room for the partials, the one-pass kernel
+      const bool cap = stream_capturing(st);
+      if (ord && (!cap || stream_workspace_fits(st, pal + obytes))) {
+        const int rc = stream_workspace(st, pal + obytes, &part);
+        if (rc != O3D_OK) return rc;
+        sorted = true;
+      } else if (cap && !stream_workspace_fits(st, pbytes)) {
+        part = nullptr;
+      } else {
         const int rc = stream_workspace(st, pbytes, &part);
         if (rc != O3D_OK) return rc;
       }
     }
     if (part) {
+      if (sorted) {
+        char* ow = (char*)part + pal;
+        int rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ow, st, &a.tperm);
+        if (rc != O3D_OK) return rc;
+        if ((rc = sorted_sources(src, a.tperm, tile, ow + point_order_bytes(a.nt), st, &a)) != O3D_OK) return rc;
+      }
       hipError_t e = dispatch_p2p_split(o->core, a, thick, grad, o->precision, nsl, slen, part, st);
       if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));
       return O3D_OK;

@@ -406,7 +406,7 @@ struct PartCols {
 // single-collection, batched and serving kernels so that a target's result does not
 // depend on which launch computed it.  part/slice: OUT_PART's destination.
 //
-// ssp, ssa, stsph, stcs (k_p2p, EXP core only; s_lo = 0): the sources come from rows sorted in the
+// ssp, ssa, stsph, stcs (EXP core only; s_lo a multiple of tile): the sources come from rows sorted in the
 // targets' Morton order with tile spheres, and a tile whose sphere is beyond the
 // far-branch distance of every target of the wave is summed with no per-pair test.
 template <int CORE, bool THICK, bool GRAD, int PREC, int TPT, int OUT>
@@ -449,7 +449,7 @@ __device__ __forceinline__ void p2p_span(const SrcView& s, const TgtView& t, int
     if (srt) stage_sorted(ssp, ssa, s0, cnt, tile, sh);
     else stage_tile<CORE>(s, s0, cnt, tile, sh, &s_tmax);
     __syncthreads();
-    const int64_t ti = srt ? (s0 - s_lo) / tile : 0;
+    const int64_t ti = srt ? s0 / tile : 0;   // s_lo is a whole number of tiles
     const float tmax = srt ? stcs[ti] : __uint_as_float(s_tmax);
     float thr[TPT];
 #pragma unroll
@@ -655,18 +655,20 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_split(P2PArgs a, void* part, int6
   const int64_t lo = (int64_t)blockIdx.y * slen;
   const int64_t hi = lo + slen < a.ns ? lo + slen : a.ns;
   p2p_span<CORE, THICK, GRAD, PREC, 1, OUT_PART>(a.s, a.t, lo, hi, a.nt, t_base, a.tile, false, sh, part,
-                                                 blockIdx.y);
+                                                 blockIdx.y, a.tperm, a.sp, a.sa, a.tsph, a.tcs);
 }
 
 // adds the nslice slices of each target in slice order -- velocity as FP32 TwoSum
 // pairs, the gradient's sums in FP64 (its cross-slice reduction) -- takes out the
-// self pair as p2p_span does, forms the gradient and adds into the outputs
+// self pair as p2p_span does, forms the gradient and adds into the outputs; with
+// a.tperm the slices are in that order and the outputs go to the original index
 template <int CORE, bool THICK, bool GRAD, int PREC>
 __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* part, int nslice) {
   constexpr int NC = PartCols<GRAD, PREC>::n;
   const int64_t nt = a.nt;
   const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
   if (i >= nt) return;
+  const int64_t io = a.tperm ? (int64_t)a.tperm[i] : i;
   float vh[3] = {0.f, 0.f, 0.f}, vl[3] = {0.f, 0.f, 0.f};
   double sum[NC];
 #pragma unroll
@@ -695,8 +697,8 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* pa
     double kx = 0.0, ky = 0.0, kz = 0.0;
     if (a.self_excl) {
       float ks, Bp;
-      kb<CORE>(0.0f, core_term<CORE>(a.s.r[i]) + (THICK ? core_term<CORE>(a.t.r[i]) : 0.0f), 0.0f, ks, Bp);
-      kx = (double)ks * a.s.ax[i]; ky = (double)ks * a.s.ay[i]; kz = (double)ks * a.s.az[i];
+      kb<CORE>(0.0f, core_term<CORE>(a.s.r[io]) + (THICK ? core_term<CORE>(a.t.r[io]) : 0.0f), 0.0f, ks, Bp);
+      kx = (double)ks * a.s.ax[io]; ky = (double)ks * a.s.ay[io]; kz = (double)ks * a.s.az[io];
     }
     if constexpr (PREC == PREC_FP32) {
       sum[11] -= kx; sum[12] -= ky; sum[13] -= kz;
@@ -713,12 +715,12 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* pa
       r[11] = sum[11];
     }
   }
-  a.t.u[i] = add_comp(a.t.u[i], vh[0], vl[0]);
-  a.t.v[i] = add_comp(a.t.v[i], vh[1], vl[1]);
-  a.t.w[i] = add_comp(a.t.w[i], vh[2], vl[2]);
+  a.t.u[io] = add_comp(a.t.u[io], vh[0], vl[0]);
+  a.t.v[io] = add_comp(a.t.v[io], vh[1], vl[1]);
+  a.t.w[io] = add_comp(a.t.w[io], vh[2], vl[2]);
   if (GRAD) {
 #pragma unroll
-    for (int c = 0; c < 9; ++c) a.t.g[c][i] = (float)((double)a.t.g[c][i] + r[3 + c]);
+    for (int c = 0; c < 9; ++c) a.t.g[c][io] = (float)((double)a.t.g[c][io] + r[3 + c]);
   }
 }
 

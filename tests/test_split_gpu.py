@@ -5,7 +5,8 @@ and adds the slices in a fixed order.  Checked against the oracle (FP64) and the
 one-pass kernel on every core, thick and thin targets, with and without the
 gradient and the self pair, at ragged sizes (slices of unequal length, sources
 shorter than one tile, targets beyond one workgroup), for accumulation into
-non-zero outputs and for repeatability."""
+non-zero outputs and for repeatability; and the EXP split of one collection, which
+visits its targets in Morton order and reads its sources as sorted rows."""
 import numpy as np
 import pytest
 
@@ -92,3 +93,36 @@ def test_split_single_ring_differs_only_in_order(infl):
     v1, g1 = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "fp32", self_term=1, tile=256, tpt=1)
     assert not (np.array_equal(v, v1) and np.array_equal(g, g1))
     assert rel_errors(v, v1)[0] <= 2e-5 and rel_errors(g, g1)[0] <= 2e-5
+
+
+@pytest.mark.parametrize("n", [40000, 262144])
+def test_split_exp_one_collection_in_morton_order(infl, n):
+    """An EXP call on one collection of 32,768 or more particles that splits its
+    sources also visits its targets in Morton order and reads its sources as sorted
+    rows with tile spheres: the partial sums are in that order and the combine adds
+    each target's slices into its original index.  Against FP64 on spread targets,
+    self pair included and excluded, both precisions; into non-zero outputs; and
+    the same bits on a repeated call."""
+    from omega3d_amd.metrics import self_term_gradient
+    p = scenes.colliding_rings(n_each=n // 2, seed=scenes.SEED_BASE + 73)
+    sel = np.linspace(0, n - 1, 512).astype(np.int64)
+    rng = np.random.default_rng(73)
+    v0 = rng.normal(size=(3, n)).astype(np.float32)
+    g0 = rng.normal(size=(9, n)).astype(np.float32)
+    ov, og = oracle.p2p("exp", p.pos, p.str, p.rad, p.pos[:, sel], p.rad[sel], True, "f64")
+    for selfx in (0, 1):
+        ogx = og - self_term_gradient("exp", p.str[:, sel], p.rad[sel]) if selfx else og
+        for prec in ("comp", "fp32"):
+            v, g = infl.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, prec, self_term=selfx, method="direct")
+            ev, eg = rel_errors(v[:, sel], ov), rel_errors(g[:, sel], ogx)
+            print(f"split EXP N={n} self_term={selfx} {prec}: vel {ev} grad {eg}", flush=True)
+            tol = 1e-6 if prec == "comp" else 1e-2   # plain FP32: the float sum's own error
+            assert ev[0] <= tol and eg[0] <= tol
+            v2, g2 = infl.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, prec, self_term=selfx,
+                              method="direct")
+            np.testing.assert_array_equal(v, v2)
+            np.testing.assert_array_equal(g, g2)
+            va, ga = infl.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, prec, self_term=selfx,
+                              method="direct", vel=v0.copy(), grad=g0.copy())
+            np.testing.assert_allclose(va, v0 + v, rtol=1e-6, atol=1e-6 * np.abs(v).max())
+            np.testing.assert_allclose(ga, g0 + g, rtol=1e-6, atol=1e-6 * np.abs(g).max())
