@@ -309,14 +309,15 @@ int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipSt
                                        : 1;
   if (nsl > 1) {
     void* part = ws;
-    // an EXP call on one collection also takes its targets in Morton order and its
-    // sources as sorted rows, as the one-pass kernel below does (in the stream's
-    // workspace after the partial sums); the partials are then in that order
-    const bool ord = !ws && o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX &&
-                     ff_same_points(src, targ);
+    // an EXP call also takes its targets in Morton order (the same bits), and on one
+    // collection its sources as sorted rows, as the one-pass kernel below does (in
+    // the stream's workspace after the partial sums); the partials are then in that
+    // order
+    const bool ord = !ws && o->core == CORE_EXP && a.nt >= TARGET_ORDER_MIN_N && a.nt <= INT32_MAX;
+    const bool srt = ord && ff_same_points(src, targ);
     const size_t pal = (pbytes + 255) & ~(size_t)255;
-    const size_t obytes = ord ? point_order_bytes(a.nt) + sorted_sources_bytes(a.ns, tile) : 0;
-    bool sorted = false;
+    const size_t obytes = ord ? point_order_bytes(a.nt) + (srt ? sorted_sources_bytes(a.ns, tile) : 0) : 0;
+    bool ordered = false;
     std::unique_lock<std::mutex> lk(stream_ws_mutex(), std::defer_lock);
     if (!ws) {
       lk.lock();
@@ -326,7 +327,7 @@ int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipSt
       if (ord && (!cap || stream_workspace_fits(st, pal + obytes))) {
         const int rc = stream_workspace(st, pal + obytes, &part);
         if (rc != O3D_OK) return rc;
-        sorted = true;
+        ordered = true;
       } else if (cap && !stream_workspace_fits(st, pbytes)) {
         part = nullptr;
       } else {
@@ -335,11 +336,12 @@ int p2p_direct(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipSt
       }
     }
     if (part) {
-      if (sorted) {
+      if (ordered) {
         char* ow = (char*)part + pal;
         int rc = point_order(a.t.x, a.t.y, a.t.z, a.nt, ow, st, &a.tperm);
         if (rc != O3D_OK) return rc;
-        if ((rc = sorted_sources(src, a.tperm, tile, ow + point_order_bytes(a.nt), st, &a)) != O3D_OK) return rc;
+        if (srt && (rc = sorted_sources(src, a.tperm, tile, ow + point_order_bytes(a.nt), st, &a)) != O3D_OK)
+          return rc;
       }
       hipError_t e = dispatch_p2p_split(o->core, a, thick, grad, o->precision, nsl, slen, part, st);
       if (e != hipSuccess) return fail(O3D_EDEVICE, "p2p launch failed: %s", hipGetErrorString(e));

@@ -126,3 +126,43 @@ def test_split_exp_one_collection_in_morton_order(infl, n):
                               method="direct", vel=v0.copy(), grad=g0.copy())
             np.testing.assert_allclose(va, v0 + v, rtol=1e-6, atol=1e-6 * np.abs(v).max())
             np.testing.assert_allclose(ga, g0 + g, rtol=1e-6, atol=1e-6 * np.abs(g).max())
+
+
+def test_split_exp_targets_in_morton_order_same_bits():
+    """EXP sources acting on another collection of 32,768 or more targets: the split
+    visits the targets in Morton order, which does not change any target's sums, so
+    the result has the bits of the index-order split that the no-allocation entry
+    point takes (a caller's scratch keeps index order; 4,096 sources are at most 16
+    slices, whose partial sums fit its scratch, so both calls split alike)."""
+    import ctypes as C
+    from omega3d_amd import _lib
+    assert _lib.lib.o3d_init(-1) == 0, _lib.last_error()
+    src = scenes.ring(4096, R=1.0, sigma=0.02, seed=scenes.SEED_BASE + 74)
+    tgt = scenes.shell_points(40000, 0.9, 1.1, seed=scenes.SEED_BASE + 75)
+    D = _lib.DeviceArray.from_host
+    keep = [D(src.pos), D(src.str), D(src.rad), D(tgt)]
+    s = _lib.Points()
+    s.n = src.n
+    for d in range(3):
+        s.x[d], s.s[d] = keep[0].row(d), keep[1].row(d)
+    s.r = keep[2].ptr
+    res = {}
+    for name in ("o3d_points_affect_points_dev", "o3d_points_affect_points_noalloc_dev"):
+        for prec in ("comp", "fp32"):
+            dv, dg = _lib.DeviceArray((3, tgt.shape[1])), _lib.DeviceArray((9, tgt.shape[1]))
+            dv.zero()
+            dg.zero()
+            t = _lib.Points()
+            t.n = tgt.shape[1]
+            for d in range(3):
+                t.x[d], t.u[d] = keep[3].row(d), dv.row(d)
+            for k in range(9):
+                t.ug[k] = dg.row(k)
+            o = _lib.make_opts("exp", True, prec, method="direct")
+            _lib.check(getattr(_lib.lib, name)(C.byref(s), C.byref(t), C.byref(o), None))
+            _lib.synchronize()
+            res[name, prec] = dv.to_host(), dg.to_host()
+    for prec in ("comp", "fp32"):
+        a, b = res["o3d_points_affect_points_dev", prec], res["o3d_points_affect_points_noalloc_dev", prec]
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
