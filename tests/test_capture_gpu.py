@@ -90,6 +90,37 @@ def test_captured_split_call_replays_bitwise(env):
         stream_destroy(st)
 
 
+def test_captured_ordered_exp_split_replays_bitwise(env):
+    """An EXP call on one collection of 40,000 particles splits its sources, visits
+    its targets in Morton order and reads its sources as sorted rows, all in the
+    stream's workspace after the partial sums.  Captured after the eager call sized
+    that workspace, every replay gives the eager bits."""
+    _lib, hip = env
+    from omega3d_amd.batch import stream_create, stream_destroy
+    st = stream_create()
+    try:
+        keep, dv, dg, s, _ = _setup(_lib, 40000, st)
+        o = _lib.make_opts("exp", True, "comp", self_term=1, stream=st, method="direct")
+        call = lambda: _lib.lib.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None)
+        dv.zero(); dg.zero(); _lib.synchronize()
+        _lib.check(call())
+        _lib.check(_lib.lib.o3d_stream_synchronize(st))
+        v0, g0 = dv.to_host(), dg.to_host()
+        graph, exe = _capture(_lib, hip, st, call)
+        try:
+            for rep in range(3):
+                dv.zero(); dg.zero(); _lib.synchronize()
+                assert hip.hipGraphLaunch(exe, C.c_void_p(st)) == 0
+                _lib.check(_lib.lib.o3d_stream_synchronize(st))
+                np.testing.assert_array_equal(dv.to_host(), v0)
+                np.testing.assert_array_equal(dg.to_host(), g0)
+        finally:
+            hip.hipGraphExecDestroy(exe)
+            hip.hipGraphDestroy(graph)
+    finally:
+        stream_destroy(st)
+
+
 def test_capture_on_a_fresh_stream_falls_back_to_one_pass(env):
     _lib, hip = env
     from omega3d_amd.batch import stream_create, stream_destroy
