@@ -509,7 +509,7 @@ int o3d_event_elapsed_ms(float* ms, void* start, void* stop);
 int o3d_copy_probe(void* dst, const void* src, size_t bytes, void* stream);
 /* VALU issue-rate probe (no counterpart in the reference; calibrates the FP32
  * rooflines): a grid of 8 workgroups per CU runs 16 independent chains of one
- * instruction per lane (op 0 v_fmac_f32 with a scalar-register operand, 1 v_add_f32,
+ * instruction per lane, 64 instructions per loop iteration (op 0 v_fmac_f32 with a scalar-register operand, 1 v_add_f32,
  * 2 v_mul_f32, 3 v_rsq_f32, 4 v_fmac_f32 with vector-register operands only, all at
  * 32 waves per CU); ops 5-8 at one wave per SIMD: 5 v_fmac_f32 (vector-register
  * operands), 6 v_mfma_f32_32x32x16_f16, 7 both in the same waves (16 FMAs per MFMA),

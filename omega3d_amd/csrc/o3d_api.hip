@@ -526,7 +526,9 @@ __global__ void k_copy_probe(float4* __restrict__ d, const float4* __restrict__ 
 
 // VALU issue-rate probe: 16 independent chains of one FP32 instruction per lane
 // (op 0 v_fmac_f32 with a scalar-register operand, 1 v_add_f32, 2 v_mul_f32,
-// 3 v_rsq_f32, 4 v_fmac_f32 with vector-register operands only), iters times
+// 3 v_rsq_f32, 4 v_fmac_f32 with vector-register operands only), four steps of
+// each chain per loop iteration (64 instructions between two loop branches, so the
+// branch and counter cost ~1/20 of the loop), iters times
 template <int OP>
 __global__ __launch_bounds__(BLOCK) void k_valu_probe(float* out, int iters, float b) {
   float x[16], y[16], z[16];
@@ -538,7 +540,8 @@ __global__ __launch_bounds__(BLOCK) void k_valu_probe(float* out, int iters, flo
   }
   for (int it = 0; it < iters; ++it) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
+    for (int jj = 0; jj < 64; ++jj) {
+      const int j = jj & 15;
       if (OP == 0) x[j] = fmaf(y[j], b, x[j]);
       else if (OP == 1) x[j] = x[j] + y[j];
       else if (OP == 2) x[j] = x[j] * y[j];
@@ -825,7 +828,7 @@ int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
   if (rc) return rc;
   // ops 0-4: 8 workgroups of 256 lanes per CU; ops 5-7: one workgroup of 256 lanes
   // per CU (one wave per SIMD); op 8: one of 512 lanes per CU (two waves per SIMD)
-  const int blocks = op < 5 ? 8 * num_cus() : num_cus(), iters = op == 3 ? 1024 : op < 5 ? 4096 : 8192;
+  const int blocks = op < 5 ? 8 * num_cus() : num_cus(), iters = op == 3 ? 256 : op < 5 ? 1024 : 8192;
   const int threads = op == 8 ? 512 : BLOCK;
   std::lock_guard<std::mutex> lk(pool_mutex());
   void* buf = nullptr;
@@ -855,7 +858,7 @@ int o3d_valu_probe(int32_t op, float* ms, double* wave_instructions) {
   // ops 0-4: the probed instruction; 5: the FMAs; 6-8: the MFMAs (op 7 also issues
   // 16 FMAs per MFMA in the same waves, op 8 as many in the other waves)
   const double waves = (double)blocks * (op < 5 ? BLOCK / 64 : 4);
-  *wave_instructions = waves * (double)iters * (op < 5 ? 16.0 : op == 5 ? 64.0 : 4.0);
+  *wave_instructions = waves * (double)iters * (op < 5 ? 64.0 : op == 5 ? 64.0 : 4.0);
   return O3D_OK;
 }
 
