@@ -661,37 +661,94 @@ __global__ __launch_bounds__(BLOCK) void k_p2p_split(P2PArgs a, void* part, int6
 // adds the nslice slices of each target in slice order -- velocity as FP32 TwoSum
 // pairs, the gradient's sums in FP64 (its cross-slice reduction) -- takes out the
 // self pair as p2p_span does, forms the gradient and adds into the outputs; with
-// a.tperm the slices are in that order and the outputs go to the original index
+// a.tperm the slices are in that order and the outputs go to the original index.
+// A block takes COMBINE_TG targets, one per lane of each wave: wave w sums columns
+// w, w + COMBINE_NW, ..., each over every slice in order (so a column's sum is the
+// same chain of operations whatever wave runs it), loading COMBINE_CH slices of its
+// columns before adding them; the first wave finishes the targets from the column
+// sums in LDS.
+constexpr int COMBINE_TG = 64, COMBINE_NW = BLOCK / COMBINE_TG, COMBINE_CH = 8;
+// the columns W, W + COMBINE_NW, ... below NC of one target: slices [y0, y0 + ny)
+// loaded, then added in order
+template <int NC, int PREC, int W, int CPT, bool FULL, typename Elem>
+__device__ __forceinline__ void combine_chunk(const Elem* pe, int64_t nt, int y0, int ny, float* h, float* l,
+                                              double* sm) {
+  Elem v[CPT][COMBINE_CH];
+#pragma unroll
+  for (int m = 0; m < CPT; ++m)
+#pragma unroll
+    for (int k = 0; k < COMBINE_CH; ++k)
+      if (FULL || k < ny) v[m][k] = pe[((int64_t)(y0 + k) * NC + W + m * COMBINE_NW) * nt];
+#pragma unroll
+  for (int m = 0; m < CPT; ++m) {
+    constexpr int c0 = W;
+    const int c = c0 + m * COMBINE_NW;
+#pragma unroll
+    for (int k = 0; k < COMBINE_CH; ++k) {
+      if (!FULL && k >= ny) break;
+      if (c < 3) {
+        if (PREC == PREC_FP32) {
+          two_sum(h[m], l[m], (float)v[m][k]);
+        } else {
+          const float2 hl = __builtin_bit_cast(float2, (double)v[m][k]);
+          two_sum(h[m], l[m], hl.x);
+          l[m] += hl.y;
+        }
+      } else {
+        sm[m] += (double)v[m][k];
+      }
+    }
+  }
+}
+// one wave's share of k_p2p_combine: the sums of its columns into LDS
+template <int NC, int PREC, int W>
+__device__ __forceinline__ void combine_wave(const void* part, int64_t nt, int64_t i, int nslice, int lane,
+                                             double (*cs)[COMBINE_TG], float2 (*vs)[COMBINE_TG]) {
+  constexpr int CPT = W < NC ? (NC - W + COMBINE_NW - 1) / COMBINE_NW : 0;
+  if constexpr (CPT > 0) {
+    using Elem = typename std::conditional<PREC == PREC_FP32, float, double>::type;
+    const Elem* pe = (const Elem*)part + i;
+    float h[CPT], l[CPT];
+    double sm[CPT];
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) { h[m] = 0.f; l[m] = 0.f; sm[m] = 0.0; }
+    int y0 = 0;
+    for (; y0 + COMBINE_CH <= nslice; y0 += COMBINE_CH)
+      combine_chunk<NC, PREC, W, CPT, true>(pe, nt, y0, COMBINE_CH, h, l, sm);
+    if (y0 < nslice) combine_chunk<NC, PREC, W, CPT, false>(pe, nt, y0, nslice - y0, h, l, sm);
+#pragma unroll
+    for (int m = 0; m < CPT; ++m) {
+      const int c = W + m * COMBINE_NW;
+      if (c < 3) vs[c][lane] = make_float2(h[m], l[m]);
+      else cs[c][lane] = sm[m];
+    }
+  }
+}
 template <int CORE, bool THICK, bool GRAD, int PREC>
 __global__ __launch_bounds__(BLOCK) void k_p2p_combine(P2PArgs a, const void* part, int nslice) {
   constexpr int NC = PartCols<GRAD, PREC>::n;
+  static_assert(COMBINE_NW == 4, "one column group per wave of the block");
+  __shared__ double cs[NC][COMBINE_TG];
+  __shared__ float2 vs[3][COMBINE_TG];
   const int64_t nt = a.nt;
-  const int64_t i = (int64_t)blockIdx.x * BLOCK + threadIdx.x;
-  if (i >= nt) return;
+  const int lane = threadIdx.x % COMBINE_TG;
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / COMBINE_TG));
+  const int64_t i = (int64_t)blockIdx.x * COMBINE_TG + lane;
+  if (i < nt) {
+    if (w == 0) combine_wave<NC, PREC, 0>(part, nt, i, nslice, lane, cs, vs);
+    else if (w == 1) combine_wave<NC, PREC, 1>(part, nt, i, nslice, lane, cs, vs);
+    else if (w == 2) combine_wave<NC, PREC, 2>(part, nt, i, nslice, lane, cs, vs);
+    else combine_wave<NC, PREC, 3>(part, nt, i, nslice, lane, cs, vs);
+  }
+  __syncthreads();
+  if (w != 0 || i >= nt) return;
   const int64_t io = a.tperm ? (int64_t)a.tperm[i] : i;
-  float vh[3] = {0.f, 0.f, 0.f}, vl[3] = {0.f, 0.f, 0.f};
+  float vh[3], vl[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) { vh[c] = vs[c][lane].x; vl[c] = vs[c][lane].y; }
   double sum[NC];
 #pragma unroll
-  for (int c = 0; c < NC; ++c) sum[c] = 0.0;
-  for (int y = 0; y < nslice; ++y) {
-    if (PREC == PREC_FP32) {
-      const float* pf = (const float*)part + (int64_t)y * NC * nt + i;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) two_sum(vh[c], vl[c], pf[c * nt]);
-#pragma unroll
-      for (int c = 3; c < NC; ++c) sum[c] += (double)pf[c * nt];
-    } else {
-      const double* pd = (const double*)part + (int64_t)y * NC * nt + i;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float2 v = reinterpret_cast<const float2*>(pd)[c * nt];
-        two_sum(vh[c], vl[c], v.x);
-        vl[c] += v.y;
-      }
-#pragma unroll
-      for (int c = 3; c < NC; ++c) sum[c] += pd[c * nt];
-    }
-  }
+  for (int c = 3; c < NC; ++c) sum[c] = cs[c][lane];
   double r[12];
   if constexpr (GRAD) {
     double kx = 0.0, ky = 0.0, kz = 0.0;
@@ -864,7 +921,8 @@ hipError_t launch_p2p_split_prec(const P2PArgs& a, int nslice, int64_t slen, voi
                      part, slen);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_p2p_combine<CORE, THICK, GRAD, PREC>), dim3(nb), dim3(BLOCK), 0, st, a, (const void*)part,
+  const unsigned nbc = (unsigned)((a.nt + COMBINE_TG - 1) / COMBINE_TG);
+  hipLaunchKernelGGL((k_p2p_combine<CORE, THICK, GRAD, PREC>), dim3(nbc), dim3(BLOCK), 0, st, a, (const void*)part,
                      nslice);
   return hipGetLastError();
 }
