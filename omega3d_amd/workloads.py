@@ -296,27 +296,29 @@ def single_ring_run(n=8192, reps=20, log=print):
     p = scenes.ring(n, sigma=0.05)
     D = _lib.DeviceArray.from_host
     dpos, dstr, drad = D(p.pos), D(p.str), D(p.rad)
-    dv, dg = _lib.DeviceArray((3, n)), _lib.DeviceArray((9, n))
+    # velocity rows 0-2 and gradient rows 3-11 of one array, zeroed by one fill per step
+    dout = _lib.DeviceArray((12, n))
     s = _lib.Points()
     s.n = n
     for d in range(3):
-        s.x[d], s.s[d], s.u[d] = dpos.row(d), dstr.row(d), dv.row(d)
+        s.x[d], s.s[d], s.u[d] = dpos.row(d), dstr.row(d), dout.row(d)
     s.r = drad.ptr
     for k in range(9):
-        s.ug[k] = dg.row(k)
+        s.ug[k] = dout.row(3 + k)
     o = _lib.make_opts("rm", True, "fp32", self_term=1)
     e0, e1 = _lib.Event(), _lib.Event()
-    dv.zero(); dg.zero()
+    dout.zero()
     _lib.check(L.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None))
     _lib.synchronize()
     e0.record()
     for _ in range(reps):
-        dv.zero(); dg.zero()
+        dout.zero()
         _lib.check(L.o3d_points_affect_points_dev(C.byref(s), C.byref(s), C.byref(o), None))
     e1.record()
     _lib.synchronize()
     ms = e0.elapsed_ms(e1) / reps
-    vel, grad = dv.to_host(), dg.to_host()
+    out_h = dout.to_host()
+    vel, grad = out_h[:3], out_h[3:]
     d64 = lambda a: _lib.DeviceArray.from_host(np.ascontiguousarray(a, np.float64))
     p64, s64_, r64 = d64(p.pos), d64(p.str), d64(p.rad)
     v64, g64 = d64(np.zeros((3, n))), d64(np.zeros((9, n)))
