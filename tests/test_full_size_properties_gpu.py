@@ -64,3 +64,27 @@ def test_two_rings_scale_exactly_with_the_strengths(rings, method, prec):
     if method == "direct":
         vn, gn = _dev_call(c.pos, -c.str, c.rad, prec, method)
         assert np.array_equal(vn, -v) and np.array_equal(gn, -g), "negation"
+
+
+@pytest.mark.parametrize("rule,core", [("recursive", "wl"), ("gauss7", "rm")])
+def test_sphere_panels_scale_exactly_with_the_strengths(rings, rule, core):
+    """The same identities for the 20,480-panel sphere acting on 524,288 points in
+    1.02 < r < 1.5, by the reference's recursive centroid rule and by the 7-point
+    rule with its closed-form near field: panel sums are FP32 (the near field FP64)
+    products and sums of the sheet strengths, so both factors are exact."""
+    from omega3d_amd import influence
+    pan = scenes.icosphere_panels()
+    tp = scenes.shell_points()
+    assert pan.np == 20480 and tp.shape[1] == 524288
+    ss = pan.ss if rule == "recursive" else None
+    print(f"full-size properties: {pan.np} panels -> {tp.shape[1]} points, {rule} rule", flush=True)
+
+    def call(f):
+        return influence.pan(core, pan.nodes, pan.idx, f * pan.ts, pan.area, None if ss is None else f * ss, tp,
+                             True, "comp", rule=rule)
+    v, g = call(1.0)
+    assert np.isfinite(v).all() and np.isfinite(g).all() and np.abs(v).max() > 0
+    v2, g2 = call(2.0)
+    assert np.array_equal(v2, 2.0 * v) and np.array_equal(g2, 2.0 * g), "scaling by 2"
+    vn, gn = call(-1.0)
+    assert np.array_equal(vn, -v) and np.array_equal(gn, -g), "negation"
