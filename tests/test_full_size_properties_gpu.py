@@ -88,3 +88,21 @@ def test_sphere_panels_scale_exactly_with_the_strengths(rings, rule, core):
     assert np.array_equal(v2, 2.0 * v) and np.array_equal(g2, 2.0 * g), "scaling by 2"
     vn, gn = call(-1.0)
     assert np.array_equal(vn, -v) and np.array_equal(gn, -g), "negation"
+
+
+def test_particles_on_sphere_panels_scale_exactly_with_the_strengths(rings):
+    """points_affect_panels (the BEM right-hand side's sum) from 524,288 vortex
+    particles in 1.02 < r < 1.5 onto the 20,480-panel sphere: both factors exact."""
+    from omega3d_amd import influence
+    pan = scenes.icosphere_panels()
+    pos = scenes.shell_points(524288)
+    rng = np.random.default_rng(5)
+    st = (rng.standard_normal((3, pos.shape[1])) / pos.shape[1]).astype(np.float32)
+    print(f"full-size properties: {pos.shape[1]} particles -> {pan.np} panels", flush=True)
+
+    def call(f):
+        return influence.ptpan(pos, f * st, pan.nodes, pan.idx, pan.area, "comp")
+    v = call(1.0)
+    assert np.isfinite(v).all() and np.abs(v).max() > 0
+    assert np.array_equal(call(2.0), 2.0 * v), "scaling by 2"
+    assert np.array_equal(call(-1.0), -v), "negation"
