@@ -12,8 +12,14 @@ from_json / init_elements):
     float/double mixing (FlowFeature.cpp:205-1019).  "block of random" is unseeded
     in the reference (std::random_device, :561-562); here it takes a seed.  A
     particle emitter adds one particle per step (step_elements, :702-712) and none
-    at start.  Disabled features are skipped.  ``bodies`` (boundary meshes) are
-    reported, not meshed.
+    at start.  Disabled features are skipped;
+  * ``bodies``: each mesh of a body is made into flat triangles at the particle
+    spacing by the reference's meshers (BoundaryFeature.cpp:39-60, GeomHelper.cpp:
+    41-127): "sphere" (the 12-node icosahedron of radius 0.5, split 1 -> 4 and
+    projected back to the sphere while the estimated triangle size exceeds 1.2 ips,
+    then scaled), "rect" (the 12-triangle unit cube, split likewise, then scaled) and
+    "quad" (a bilinear grid, BoundaryFeature.cpp:695-760); "disk" and mesh files need
+    libigl and are listed in ``unmeshed``.
 
 VTU output of a particle collection (Points.h:851-1039 through VtkXmlWriter.h):
 byte for byte what the reference writes -- the same element and attribute order
@@ -258,6 +264,166 @@ def block_of_random(center, size, maxstr, num, seed=0):
     return pos, st
 
 
+
+# ---------------------------------------------------------------------------
+# Boundary bodies: the meshes of a scene's "bodies" (BoundaryFeature.cpp:39-60
+# dispatch; each geometry's from_json and init_elements), as flat triangles with
+# a boundary-condition velocity per panel.
+# ---------------------------------------------------------------------------
+
+# the icosahedron of radius 0.5 that the sphere mesher starts from (Icosahedron.h
+# ico0): its 12 vertices are the cyclic permutations of (0, +-A, +-B), in the
+# order of _ico0_nodes, and its 20 faces wound outward
+_ICO_A, _ICO_B = 0.2628655, 0.4253255
+_ICO0_FACES = [(0, 1, 2), (3, 4, 5), (4, 3, 6), (7, 4, 8), (9, 3, 10), (9, 6, 3), (0, 9, 10), (7, 2, 1),
+               (2, 8, 11), (3, 5, 10), (10, 5, 1), (6, 9, 11), (8, 4, 6), (0, 10, 1), (11, 0, 2), (7, 5, 4),
+               (7, 1, 5), (6, 11, 8), (9, 0, 11), (2, 7, 8)]
+# the unit cube of the box mesher (Cube.h cube0): node i at (i >> 2, (i >> 1) & 1, i & 1),
+# two triangles per side, wound outward
+_CUBE0_FACES = [(0, 4, 5), (0, 5, 1), (4, 6, 7), (4, 7, 5), (1, 5, 7), (1, 7, 3), (6, 2, 3), (6, 3, 7),
+                (2, 0, 1), (2, 1, 3), (2, 6, 4), (2, 4, 0)]
+
+
+def _ico0_nodes():
+    """ico0's 12 vertices, (nn, 3) float32"""
+    rows = [(_ICO_B, 0.0, _ICO_A), (0.0, _ICO_A, _ICO_B), (0.0, -_ICO_A, _ICO_B), (0.0, _ICO_A, -_ICO_B),
+            (-_ICO_B, 0.0, -_ICO_A), (-_ICO_A, _ICO_B, 0.0), (0.0, -_ICO_A, -_ICO_B), (-_ICO_B, 0.0, _ICO_A),
+            (-_ICO_A, -_ICO_B, 0.0), (_ICO_B, 0.0, -_ICO_A), (_ICO_A, _ICO_B, 0.0), (_ICO_A, -_ICO_B, 0.0)]
+    return np.array(rows, np.float32)
+
+
+def upsample(nodes, faces):
+    """One in-plane subdivision (IglRefine.cpp:32-75, igl::upsample): a node at the
+    midpoint of every edge, each triangle split into four of the same winding; the
+    original nodes keep their indices and the midpoints follow in order of first
+    appearance.  nodes (nn, 3) float32, faces (nf, 3) int; midpoints in double."""
+    nn = nodes.shape[0]
+    f = np.asarray(faces, np.int64)
+    e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])      # edges ab, bc, ca of every face
+    key = np.minimum(e[:, 0], e[:, 1]) * nn + np.maximum(e[:, 0], e[:, 1])
+    nf = f.shape[0]
+    order = np.stack([key[:nf], key[nf:2 * nf], key[2 * nf:]], axis=1).reshape(-1)   # face-major order
+    uniq, first = np.unique(order, return_index=True)
+    rank = np.empty(len(uniq), np.int64)
+    rank[np.argsort(first, kind="stable")] = np.arange(len(uniq))
+    mid_of = dict(zip(uniq.tolist(), (nn + rank).tolist()))
+    a, b = uniq // nn, uniq % nn
+    mids = np.empty((len(uniq), 3), np.float64)
+    mids[rank] = 0.5 * (nodes[a].astype(np.float64) + nodes[b].astype(np.float64))
+    ab = np.array([mid_of[k] for k in key[:nf].tolist()], np.int64)
+    bc = np.array([mid_of[k] for k in key[nf:2 * nf].tolist()], np.int64)
+    ca = np.array([mid_of[k] for k in key[2 * nf:].tolist()], np.int64)
+    out = np.stack([np.stack([f[:, 0], ab, ca], 1), np.stack([ab, f[:, 1], bc], 1),
+                    np.stack([ca, bc, f[:, 2]], 1), np.stack([ab, bc, ca], 1)], axis=1).reshape(-1, 3)
+    return np.concatenate([nodes, mids.astype(np.float32)]), out
+
+
+def _ovoid(sx, sy, sz, ips):
+    """generate_ovoid (GeomHelper.cpp:41-91): ico0, refined while the estimated
+    triangle size 0.35 max(scale) (halved per pass) exceeds 1.2 ips, each pass
+    projected back to radius 0.5, then scaled per axis"""
+    x, f = _ico0_nodes(), np.array(_ICO0_FACES, np.int64)
+    meansize = _f(0.35 * float(max(sx, sy, sz)))
+    while float(meansize) > 1.2 * float(ips):
+        x, f = upsample(x, f)
+        rad = np.sqrt((x.astype(np.float64) ** 2).sum(axis=1)).astype(np.float32)
+        x = (x * (0.5 / rad.astype(np.float64)).astype(np.float32)[:, None]).astype(np.float32)
+        meansize = _f(meansize * _f(0.5))
+    return (x * np.array([sx, sy, sz], np.float32)).astype(np.float32), f
+
+
+def _cuboid(dx, dy, dz, ips):
+    """generate_cuboid (GeomHelper.cpp:94-127): cube0 refined while 0.7 max(scale)
+    (halved per pass) exceeds 1.2 ips, then scaled per axis"""
+    x = np.array([((i >> 2) & 1, (i >> 1) & 1, i & 1) for i in range(8)], np.float32)
+    f = np.array(_CUBE0_FACES, np.int64)
+    meansize = _f(0.7 * float(max(dx, dy, dz)))
+    while float(meansize) > 1.2 * float(ips):
+        x, f = upsample(x, f)
+        meansize = _f(meansize * _f(0.5))
+    return (x * np.array([dx, dy, dz], np.float32)).astype(np.float32), f
+
+
+def _quad(p0, p1, p2, p3, ips):
+    """BoundaryQuad::init_elements (BoundaryFeature.cpp:695-760): num1 x num2 cells
+    of a bilinear quad (num = int(mean opposite side / ips), at least 1), two
+    triangles per cell, in the reference's float/double mixing"""
+    p0, p1, p2, p3 = (np.array(p, np.float32) for p in (p0, p1, p2, p3))
+    d = lambda a, b: math.sqrt(sum(float(a[k] - b[k]) ** 2 for k in range(3)))
+    side1 = _f(0.5 * (d(p1, p0) + d(p3, p2)))
+    side2 = _f(0.5 * (d(p2, p1) + d(p0, p3)))
+    num1 = max(1, int(side1 / ips))
+    num2 = max(1, int(side2 / ips))
+    x = np.empty(((num1 + 1) * (num2 + 1), 3), np.float32)
+    k = 0
+    for i in range(num1 + 1):
+        s = float(_f(_f(i) / _f(num1)))
+        for j in range(num2 + 1):
+            t = float(_f(_f(j) / _f(num2)))
+            w = [_f((1.0 - s) * (1.0 - t)), _f(s * (1.0 - t)), _f(s * t), _f((1.0 - s) * t)]
+            for c in range(3):
+                acc = _f(w[0] * p0[c])
+                acc = _f(acc + _f(w[1] * p1[c]))
+                acc = _f(acc + _f(w[2] * p2[c]))
+                x[k, c] = _f(acc + _f(w[3] * p3[c]))
+            k += 1
+    f = []
+    for i in range(num1):
+        i1, i2 = (num2 + 1) * i, (num2 + 1) * (i + 1)
+        for j in range(num2):
+            f.append((i1 + j, i2 + j, i2 + j + 1))
+            f.append((i1 + j, i2 + j + 1, i1 + j + 1))
+    return x, np.array(f, np.int64)
+
+
+def _scale3(m):
+    sc = m.get("scale", 1.0)
+    if isinstance(sc, (list, tuple)):
+        return [_f(v) for v in sc[:3]]
+    return [_f(sc)] * 3
+
+
+def mesh_boundary(m, ips):
+    """One mesh entry of a body (parse_boundary_json, BoundaryFeature.cpp:39-60).
+    Returns (nodes (nn, 3) float32, faces (np, 3) int64, bc (np, 3) float32), or None
+    with the reason for geometries not meshed here: "disk" (igl::decimate) and mesh
+    files (ExteriorFromFile, read through libigl).  "sphere" and "rect" are reactive
+    zero-velocity panels, reversed when "external" is false (ElementPacket.h:140-154
+    swaps each triangle's first two nodes); a "quad" carries its "vel" and is always
+    external."""
+    g = m.get("geometry")
+    if g in ("sphere", "rect"):
+        sx, sy, sz = _scale3(m)
+        x, f = (_ovoid if g == "sphere" else _cuboid)(sx, sy, sz, ips)
+        x = (x + np.array(m["translation"], np.float32)).astype(np.float32)
+        if not m.get("external", True):
+            f = f[:, [1, 0, 2]]
+        return x, f, np.zeros((f.shape[0], 3), np.float32), None
+    if g == "quad":
+        x, f = _quad(m["startpt"], m["p1"], m["p2"], m["p3"], ips)
+        bc = np.tile(np.array(m["vel"], np.float32), (f.shape[0], 1))
+        return x, f, bc, None
+    if g == "disk":
+        return None, None, None, "the disk mesher simplifies its triangles with igl::decimate (GeomHelper.cpp:207-330)"
+    return None, None, None, "mesh files are read through libigl (ExteriorFromFile)"
+
+
+def panel_records(nodes, faces):
+    """area and in-plane basis of flat triangles (Surfaces.h:766-816: x1 along
+    node0 -> node1, x2 its in-plane normal toward node 2, n = x1 x x2, area =
+    base * height / 2), in double; returns area (np,) and n (np, 3)"""
+    p = nodes.astype(np.float64)
+    p0, p1, p2 = p[faces[:, 0]], p[faces[:, 1]], p[faces[:, 2]]
+    e1 = p1 - p0
+    base = np.linalg.norm(e1, axis=1)
+    x1 = e1 / base[:, None]
+    e2 = p2 - p0
+    e2 = e2 - (e2 * x1).sum(1)[:, None] * x1
+    height = np.linalg.norm(e2, axis=1)
+    n = np.cross(x1, e2 / height[:, None])
+    return 0.5 * base * height, n
+
+
 class Scene:
     """A parsed JSON scene: time step, Reynolds number, freestream, particle spacing
     and core radius, and the particle collections of its enabled flow structures."""
@@ -312,6 +478,46 @@ class Scene:
         self.pos = np.concatenate(pos, axis=1) if pos else np.zeros((3, 0), f32)
         self.str = np.concatenate(st, axis=1) if st else np.zeros((3, 0), f32)
         self.rad = np.full(self.pos.shape[1], self.vdelta, f32)
+        # bodies: each mesh at the particle spacing (main_batch.cpp:85), placed by the
+        # body's translation at t = 0; bodies that move by formula or start rotated
+        # are listed, not meshed
+        self.boundaries = []     # dict(body, geometry, nodes (nn, 3), faces (np, 3), bc (np, 3))
+        self.unmeshed = []       # (body, geometry, reason)
+        for b in self.bodies:
+            name = b.get("name", "")
+            tr, rot = b.get("translation", [0.0, 0.0, 0.0]), b.get("rotation", [0.0, 0.0, 0.0])
+            fixed = all(isinstance(v, (int, float)) for v in list(tr) + list(rot))
+            for m in b.get("meshes", []):
+                g = m.get("geometry")
+                if not fixed or any(float(v) != 0.0 for v in rot):
+                    self.unmeshed.append((name, g, "the body moves by formula or starts rotated"))
+                    continue
+                x, f, bc, why = mesh_boundary(m, self.ips)
+                if why:
+                    self.unmeshed.append((name, g, why))
+                    continue
+                x = (x + np.array(tr, f32)).astype(f32)
+                self.boundaries.append(dict(body=name, geometry=g, nodes=x, faces=f, bc=bc))
+
+    def surfaces(self):
+        """The meshed bodies as one panel set (Surfaces<float> layout, zero sheet
+        strengths, as the reference creates reactive panels) and their boundary
+        velocities (3, np)."""
+        from .scenes import Panels
+        xs, fs, bcs, off = [], [], [], 0
+        for bd in self.boundaries:
+            xs.append(bd["nodes"])
+            fs.append(bd["faces"] + off)
+            bcs.append(bd["bc"])
+            off += bd["nodes"].shape[0]
+        x = np.concatenate(xs) if xs else np.zeros((0, 3), f32)
+        f = np.concatenate(fs) if fs else np.zeros((0, 3), np.int64)
+        area, _ = panel_records(x, f)
+        npn = f.shape[0]
+        pan = Panels(np.ascontiguousarray(x.T, dtype=f32), f.reshape(-1).astype(np.uint32),
+                     np.zeros((3, npn), f32), area.astype(f32), np.zeros(npn, f32))
+        bc = np.ascontiguousarray((np.concatenate(bcs) if bcs else np.zeros((0, 3), f32)).T, dtype=f32)
+        return pan, bc
 
     @property
     def n(self):

@@ -8,6 +8,9 @@ pinned by geometry: ring particles on the circle, in the plane normal to the axi
 tangent strengths summing to circulation x circumference; the reference's counts
 (1 + 2 pi R / ips stations, layered disks); blob and block strengths summing to the
 given strength; the example scenes of the reference parse with the expected counts.
+Their bodies' meshes (sphere, box, quads) are pinned the same way: closed and
+consistently wound, on the sphere or the box faces, the reference's panel counts,
+areas, normals toward the fluid, boundary velocities.
 """
 import ctypes as C
 import json
@@ -140,3 +143,108 @@ def test_thick_ring_blob_block_and_rules():
 def test_omega2d_files_are_rejected():
     with pytest.raises(ValueError):
         scene.Scene({"version": {"Omega2D": 1}})
+
+
+# --- boundary bodies ---------------------------------------------------------
+# The reference's meshers need libigl (igl::upsample) and Eigen, so they cannot be
+# built here; the meshes are pinned by geometry and by the reference's own counts.
+
+def _closed_and_oriented(faces):
+    """every directed edge once and its reverse once: a closed, consistently wound surface"""
+    e = np.concatenate([faces[:, [0, 1]], faces[:, [1, 2]], faces[:, [2, 0]]])
+    fwd = set(map(tuple, e.tolist()))
+    assert len(fwd) == len(e)
+    assert all((b, a) in fwd for a, b in fwd)
+    return len(fwd) // 2
+
+
+def _passes(start, ips):
+    """refinement passes of the sphere and box meshers (GeomHelper.cpp:57-74, 110-113)"""
+    k, m = 0, float(np.float32(start))
+    while m > 1.2 * float(ips):
+        k, m = k + 1, float(np.float32(m * 0.5))
+    return k
+
+
+def test_upsample_splits_each_triangle_in_four():
+    x = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    x2, f2 = scene.upsample(x, np.array([[0, 1, 2]]))
+    assert x2.shape == (6, 3) and f2.shape == (4, 3)
+    assert np.array_equal(x2[:3], x)
+    a, n = scene.panel_records(x2, f2)
+    assert np.allclose(a, 0.125) and np.allclose(n, [0, 0, 1])
+
+
+def test_sphere_body_of_an_example():
+    s = scene.Scene(os.path.join(GOLD, "scenes", "flow_over_sphere.json"))
+    assert s.n == 0 and s.unmeshed == [] and len(s.boundaries) == 1
+    b = s.boundaries[0]
+    k = _passes(0.35, s.ips)
+    x, f = b["nodes"].astype(np.float64), b["faces"]
+    assert k == 2 and len(f) == 20 * 4 ** k and len(x) == 10 * 4 ** k + 2
+    assert np.allclose(np.linalg.norm(x, axis=1), 0.5, rtol=0, atol=1e-6)
+    ne = _closed_and_oriented(f)
+    assert len(x) - ne + len(f) == 2
+    a, n = scene.panel_records(b["nodes"], f)
+    cen = x[f].mean(axis=1)
+    assert ((cen * n).sum(1) > 0).all()                      # external: normals into the fluid
+    assert 0.95 * math.pi < a.sum() < math.pi                # inscribed in the sphere of area pi
+    assert np.all(b["bc"] == 0)
+
+
+def test_box_and_cavity_bodies_of_the_examples():
+    s = scene.Scene(os.path.join(GOLD, "scenes", "flow_over_cube.json"))
+    b = s.boundaries[0]
+    k = _passes(0.7, s.ips)
+    x, f = b["nodes"].astype(np.float64), b["faces"]
+    assert k == 4 and len(f) == 12 * 4 ** k
+    assert np.all(np.abs(x) <= 0.5 + 1e-7)
+    tri = x[f]
+    on_face = [np.all(np.abs(np.abs(tri[:, :, c]) - 0.5) < 1e-7, axis=1) for c in range(3)]
+    assert np.all(np.sum(on_face, axis=0) == 1)               # each panel in exactly one face plane
+    ne = _closed_and_oriented(f)
+    assert len(np.unique(x, axis=0)) - ne + len(f) == 2
+    a, n = scene.panel_records(b["nodes"], f)
+    assert abs(a.sum() - 6.0) < 1e-9
+    assert ((tri.mean(axis=1) * n).sum(1) > 0).all()
+
+    doc = json.load(open(os.path.join(GOLD, "scenes", "driven_cavity.json")))
+    s = scene.Scene(doc)
+    ips = float(s.ips)
+    meshes = doc["bodies"][0]["meshes"]
+    assert len(s.boundaries) == len(meshes) == 6
+    for m, b in zip(meshes, s.boundaries):
+        p0, p1, p2, p3 = (np.array(m[k], np.float64) for k in ("startpt", "p1", "p2", "p3"))
+        n1 = max(1, int(0.5 * (np.linalg.norm(p1 - p0) + np.linalg.norm(p3 - p2)) / ips))
+        n2 = max(1, int(0.5 * (np.linalg.norm(p2 - p1) + np.linalg.norm(p0 - p3)) / ips))
+        assert len(b["faces"]) == 2 * n1 * n2 == 450
+        a, n = scene.panel_records(b["nodes"], b["faces"])
+        assert abs(a.sum() - 1.0) < 1e-5
+        cen = b["nodes"].astype(np.float64)[b["faces"]].mean(axis=1)
+        assert ((cen - 0.5) * n).sum(1).max() < 0              # into the cavity, where the fluid is
+        assert np.all(b["bc"] == np.array(m["vel"], np.float32))
+    pan, bc = s.surfaces()
+    assert pan.np == 2700 and bc.shape == (3, 2700) and abs(float(pan.area.sum()) - 6.0) < 1e-4
+
+
+def test_body_options_and_unmeshed_geometries():
+    base = {"simparams": {"nominalDt": 0.05}, "flowparams": {"Re": 50.0}}
+    doc = dict(base, bodies=[{"name": "b", "translation": [0.0, 0.0, 1.0], "rotation": [0.0, 0.0, 0.0], "meshes": [
+        {"geometry": "sphere", "scale": 2.0, "translation": [1.0, 2.0, 3.0], "external": False},
+        {"geometry": "disk", "translation": [0, 0, 0]},
+        {"geometry": "wing.obj", "translation": [0, 0, 0]}]},
+        {"name": "spun", "translation": [0, 0, 0], "rotation": [0.0, 10.0, 0.0],
+         "meshes": [{"geometry": "rect", "translation": [0, 0, 0]}]},
+        {"name": "moving", "translation": ["sin(t)", 0, 0], "rotation": [0, 0, 0],
+         "meshes": [{"geometry": "rect", "translation": [0, 0, 0]}]}])
+    s = scene.Scene(doc)
+    assert len(s.boundaries) == 1
+    b = s.boundaries[0]
+    x = b["nodes"].astype(np.float64) - [1.0, 2.0, 4.0]
+    assert np.allclose(np.linalg.norm(x, axis=1), 1.0, atol=2e-6)
+    assert len(b["faces"]) == 20 * 4 ** _passes(0.7, s.ips)  # 0.35 x the largest scale
+    a, n = scene.panel_records(b["nodes"], b["faces"])
+    assert ((x[b["faces"]].mean(axis=1) * n).sum(1) < 0).all()   # "external": false turns them inward
+    reasons = {(u[0], u[1]): u[2] for u in s.unmeshed}
+    assert set(reasons) == {("b", "disk"), ("b", "wing.obj"), ("spun", "rect"), ("moving", "rect")}
+    assert "decimate" in reasons[("b", "disk")] and "libigl" in reasons[("b", "wing.obj")]

@@ -45,3 +45,35 @@ def test_example_scene_velocities_and_steps(mods, tmp_path):
     # the rings moved: a finite displacement, of the order of dt x the ring speed
     disp = np.linalg.norm(x - s.pos, axis=0)
     assert 0 < disp.max() < 1.0
+
+
+@pytest.mark.parametrize("name", ["flow_over_sphere.json", "flow_over_cube.json", "driven_cavity.json"])
+def test_example_scene_bodies_through_the_panel_kernel(mods, name):
+    """The meshed bodies of the reference's example scenes (icosphere, refined cube,
+    cavity quads), given a random tangential sheet strength, acting on 400 points
+    near their surfaces on the fluid side (every recursion depth of the panel rule
+    occurs): the GPU against the oracle in FP64 and in the reference's float
+    semantics, with the bars of the full-sphere panel test."""
+    infl, _ = mods
+    s = scene.Scene(os.path.join(GOLD, "scenes", name))
+    pan, bc = s.surfaces()
+    nodes = pan.nodes.T.astype(np.float64)
+    faces = pan.idx.reshape(-1, 3).astype(np.int64)
+    area, n = scene.panel_records(pan.nodes.T, faces)
+    rng = np.random.default_rng(11)
+    g = rng.standard_normal((pan.np, 3))
+    gam = g - (g * n).sum(1)[:, None] * n
+    pan.ts = np.ascontiguousarray((gam * area[:, None]).T, dtype=np.float32)
+    # points off the centroids of random panels along their normals (into the fluid)
+    pick = rng.integers(0, pan.np, 400)
+    cen = nodes[faces[pick]].mean(axis=1)
+    h = np.sqrt(area[pick]) * rng.uniform(0.05, 3.0, 400)
+    tp = np.ascontiguousarray((cen + h[:, None] * n[pick]).T, dtype=np.float32)
+    v, gr = infl.pan("wl", pan.nodes, pan.idx, pan.ts, pan.area, pan.ss, tp, True, "comp")
+    ov, og = oracle.pan("wl", pan.nodes, pan.idx, pan.ts, pan.area, pan.ss, tp, True, "f64")
+    rv, rg = oracle.pan("wl", pan.nodes, pan.idx, pan.ts, pan.area, pan.ss, tp, True, "f32d")
+    ev, eg = rel_errors(v, ov), rel_errors(gr, og)
+    print(f"{name}: {pan.np} panels, comp vs fp64 {ev} {eg}; reference float vs fp64 {rel_errors(rv, ov)}")
+    band = lambda r32, r64: max(2e-6, 3.0 * rel_errors(r32, r64)[0])
+    assert ev[0] <= band(rv, ov) and eg[0] <= band(rg, og)
+    assert rel_errors(v, rv)[0] <= 1e-5 and rel_errors(gr, rg)[0] <= 1e-5
