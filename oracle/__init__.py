@@ -290,8 +290,9 @@ def pap(core, nodes, idx, ts, area, ss, tnodes, tidx, mode="f32d", maxlev=RECURS
 def surface(nodes, idx):
     """A boundary surface for bem_matrix: (nodes, idx, area, bases) with the area
     and bases of compute_bases."""
+    idx = np.ascontiguousarray(np.asarray(idx).reshape(-1, 3), dtype=np.uint32)   # (panels, 3), flat accepted
     b, area = bem_bases(nodes, idx)
-    return (_as(nodes, np.float32), np.ascontiguousarray(idx, dtype=np.uint32), area, b)
+    return (_as(nodes, np.float32), idx, area, b)
 
 
 def _bem_args(surf, dt):

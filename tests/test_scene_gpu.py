@@ -77,3 +77,41 @@ def test_example_scene_bodies_through_the_panel_kernel(mods, name):
     band = lambda r32, r64: max(2e-6, 3.0 * rel_errors(r32, r64)[0])
     assert ev[0] <= band(rv, ov) and eg[0] <= band(rg, og)
     assert rel_errors(v, rv)[0] <= 1e-5 and rel_errors(gr, rg)[0] <= 1e-5
+
+
+def test_example_sphere_scene_bem(mods):
+    """The BEM pieces on the meshed body of flow_over_sphere.json (320 panels): the
+    influence matrix (3 unknowns per panel) against the oracle in FP64 tree
+    decisions and in the reference's float semantics, with the bars of the matrix
+    tests, and the right-hand side of the scene's freestream (no particles yet)
+    bitwise against the oracle."""
+    import ctypes as C
+    from omega3d_amd import _lib
+    infl, _ = mods
+    s = scene.Scene(os.path.join(GOLD, "scenes", "flow_over_sphere.json"))
+    pan, bc = s.surfaces()
+    assert pan.np == 320
+    S = oracle.surface(pan.nodes, pan.idx)
+    g = infl.bem_matrix(S[:3], None, 3, -1)
+    m64 = oracle.bem_matrix("wl", S, S, 3, True, 3, mode="f64t")
+    m32 = oracle.bem_matrix("wl", S, S, 3, True, 3, mode="f32")
+    nerr = lambda a, b: float(np.abs(a.astype(np.float64) - b).max() / np.abs(b).max())
+    print(f"sphere scene matrix: gpu vs f64t {nerr(g, m64):.3g}, reference float vs f64t {nerr(m32, m64):.3g}")
+    assert nerr(g, m64) <= 1e-6 and nerr(g, m32) <= 3 * nerr(m32, m64) + 1e-7
+    D = _lib.DeviceArray.from_host
+    dn, di = D(pan.nodes), D(pan.idx.view(np.float32))
+    p = _lib.Panels()
+    p.np, p.nn = pan.np, pan.nodes.shape[1]
+    for d in range(3):
+        p.x[d] = dn.row(d)
+    p.idx = di.ptr
+    raw = bc.copy()                       # zero panel velocities: no particles in the scene
+    dpu, drhs = D(raw), _lib.DeviceArray((3 * pan.np,))
+    pu = (C.c_void_p * 3)(*[dpu.row(d) for d in range(3)])
+    b = infl.bem_opts(tuple(s.fs), None, 3)
+    _lib.check(_lib.lib.o3d_bem_rhs_from_vels_dev(C.byref(p), C.byref(b), pu, C.c_void_p(drhs.ptr), None))
+    _lib.synchronize()
+    opu, orhs = oracle.bem_rhs(pan.nodes, pan.idx, raw, tuple(s.fs), nunk=3)
+    np.testing.assert_array_equal(dpu.to_host(), opu)
+    np.testing.assert_array_equal(drhs.to_host(), orhs)
+    assert np.abs(orhs).max() > 0.1                  # the freestream (1, 0, 0) projected on the panels
