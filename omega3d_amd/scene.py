@@ -21,7 +21,8 @@ from_json / init_elements):
     "quad" (a bilinear grid, BoundaryFeature.cpp:695-760); "disk" and mesh files need
     libigl and are listed in ``unmeshed``.
 
-VTU output of a particle collection (Points.h:851-1039 through VtkXmlWriter.h):
+VTU output of a particle collection (Points.h:851-1039 through VtkXmlWriter.h) and
+of a panel set (Surfaces.h:1301-1450):
 byte for byte what the reference writes -- the same element and attribute order
 (tinyxml2's printer, attributes in std::map order), base64 data whose UInt32 length
 header holds the length of the base64 text (the reference's own convention).
@@ -90,6 +91,56 @@ def points_vtu_bytes(pos, time, strength=None, radius=None, vel=None):
     out.append(_data_array(I4, {"Name": "velocity", "NumberOfComponents": "3", "type": "Float32"}, inter(vel)))
     out += [f"{I3}</PointData>\n", f"{I2}</Piece>\n", f"{I1}</UnstructuredGrid>\n", "</VTKFile>\n"]
     return "".join(out).encode("ascii")
+
+
+def panels_vtu_bytes(nodes, idx, time, sheet=None, source=None):
+    """The VTU file Surfaces::write_vtk (Surfaces.h:1301-1450) writes for a panel
+    set: nodes (3, nn), idx the 3 node indices of each panel (flat or (np, 3)),
+    sheet (3, np) the vortex sheet strength vector of each panel (ps0 x1 + ps1 x2,
+    vortex_sheet_to_vector, Surfaces.h:340-361) for an active or reactive surface
+    (None: inert, no cell data), source (np,) its source sheet strengths when it
+    has them.  The reference never writes point data for panels."""
+    nodes = np.asarray(nodes, f32)
+    idx = np.asarray(idx).reshape(-1)
+    nn, npn = nodes.shape[1], idx.size // 3
+    inter = lambda a: np.ascontiguousarray(np.asarray(a, f32).T).tobytes()
+    I1, I2, I3, I4 = " " * 4, " " * 8, " " * 12, " " * 16
+    out = ['<?xml version="1.0"?>\n',
+           '<VTKFile type="UnstructuredGrid" version="0.1" byte_order="LittleEndian" header_type="UInt32">\n',
+           f"{I1}<UnstructuredGrid>\n", f"{I2}<FieldData>\n",
+           _data_array(I3, {"Name": "TimeValue", "NumberOfTuples": "1", "type": "Float64"}, struct.pack("<d", time)),
+           f"{I2}</FieldData>\n",
+           f'{I2}<Piece NumberOfCells="{npn}" NumberOfPoints="{nn}">\n',
+           f"{I3}<Points>\n",
+           _data_array(I4, {"Name": "position", "NumberOfComponents": "3", "type": "Float32"}, inter(nodes)),
+           f"{I3}</Points>\n", f"{I3}<Cells>\n",
+           _data_array(I4, {"Name": "connectivity", "type": "Int32"}, idx.astype("<i4").tobytes()),
+           _data_array(I4, {"Name": "offsets", "type": "Int32"}, (3 * np.arange(1, npn + 1)).astype("<i4").tobytes()),
+           _data_array(I4, {"Name": "types", "type": "UInt8"}, np.full(npn, 5, np.uint8).tobytes()),
+           f"{I3}</Cells>\n"]
+    if sheet is not None:
+        src = source is not None
+        out.append(f'{I3}<CellData Scalars="source sheet strength">\n' if src else f"{I3}<CellData>\n")
+        out.append(_data_array(I4, {"Name": "vortex sheet strength", "NumberOfComponents": "3", "type": "Float32"},
+                               inter(sheet)))
+        if src:
+            out.append(_data_array(I4, {"Name": "source sheet strength", "type": "Float32"},
+                                   np.asarray(source, "<f4").tobytes()))
+        out.append(f"{I3}</CellData>\n")
+    out += [f"{I2}</Piece>\n", f"{I1}</UnstructuredGrid>\n", "</VTKFile>\n"]
+    return "".join(out).encode("ascii")
+
+
+def panel_vtu_name(index, frame):
+    """panel_II_FFFFF.vtu (Surfaces.h:1315-1316)"""
+    return f"panel_{index:02d}_{frame:05d}.vtu"
+
+
+def write_panels_vtu(path, nodes, idx, time, sheet=None, source=None):
+    data = panels_vtu_bytes(nodes, idx, time, sheet, source)
+    with open(path, "wb") as fh:
+        fh.write(data)
+    return path
 
 
 def vtu_name(index, frame, inert=False):

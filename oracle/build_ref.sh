@@ -36,7 +36,7 @@ for pair in RM:rm WL:wl EXPONENTIAL:exp V2:v2; do
        -I"$TMP/$name" -I"$REF" "$HERE/ref_harness.cpp" -o "$OUT/libo3dref_$name.so"
 done
 # the reference's VTK XML writer (VtkXmlWriter.h, vendored tinyxml2 and cppcodec),
-# driven through Points::write_vtk's call sequence by ref_vtk_harness.cpp
+# driven through Points::write_vtk's and Surfaces::write_vtk's call sequences by ref_vtk_harness.cpp
 LIB=${O3D_REFERENCE:-/root/reference}/lib
 if [ -f "$REF/VtkXmlWriter.h" ] && [ -f "$LIB/tinyxml2/tinyxml2.cpp" ]; then
   $CXX -std=c++17 -O2 -fPIC -shared -I"$REF" -I"$LIB" "$HERE/ref_vtk_harness.cpp" "$LIB/tinyxml2/tinyxml2.cpp" \

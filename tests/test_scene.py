@@ -1,8 +1,8 @@
 """Scene input and particle output (omega3d_amd/scene.py).
 
-VTU output is checked byte for byte against files the reference's own writer
-produced (tests/golden/vtk_*.vtu from VtkXmlWriter.h, see make_vtk_golden.py), and
-against that writer live when oracle/_ref holds it.  The JSON scene generators
+VTU output of particles and of panels is checked byte for byte against files the
+reference's own writer produced (tests/golden/vtk_*.vtu from VtkXmlWriter.h, see
+make_vtk_golden.py), and against that writer live when oracle/_ref holds it.  The JSON scene generators
 cannot be built from the reference (FlowFeature.h needs Eigen3), so they are
 pinned by geometry: ring particles on the circle, in the plane normal to the axis,
 tangent strengths summing to circulation x circumference; the reference's counts
@@ -50,6 +50,38 @@ def test_vtu_bytes_match_the_live_reference_writer(tmp_path):
                                      None if inert else a[6], np.stack(a[7:10]))
         assert got == open(fn, "rb").read(), (n, inert)
 
+
+
+@pytest.mark.parametrize("name", ["vtk_pan_src", "vtk_pan", "vtk_pan_inert"])
+def test_panel_vtu_bytes_match_the_reference_writer(name):
+    g = np.load(os.path.join(GOLD, "vtk_inputs.npz"))
+    got = scene.panels_vtu_bytes(g[f"{name}_nodes"], g[f"{name}_idx"], float(g[f"{name}_time"]),
+                                 g[f"{name}_vs"] if g[f"{name}_vort"] else None,
+                                 g[f"{name}_ss"] if g[f"{name}_src"] else None)
+    assert got == open(os.path.join(GOLD, name + ".vtu"), "rb").read()
+
+
+@pytest.mark.skipif(not os.path.exists(REFVTK), reason="oracle/_ref/libo3dref_vtk.so not built")
+def test_panel_vtu_bytes_match_the_live_reference_writer(tmp_path):
+    """the meshed bodies of the example scenes, with random sheet strengths, through
+    the reference's writer and this one"""
+    L = C.CDLL(REFVTK)
+    L.ref_write_panels_vtk.argtypes = [C.c_char_p, C.c_int64] + [C.c_void_p] * 3 + [C.c_int64] + [C.c_void_p] * 5 + [
+        C.c_double]
+    rng = np.random.default_rng(4)
+    for f in ("flow_over_sphere.json", "driven_cavity.json"):
+        pan, _ = scene.Scene(os.path.join(GOLD, "scenes", f)).surfaces()
+        vs = rng.normal(size=(3, pan.np)).astype(np.float32)
+        ss = rng.normal(size=pan.np).astype(np.float32)
+        for src in (True, False):
+            keep = [np.ascontiguousarray(a) for a in (pan.nodes[0], pan.nodes[1], pan.nodes[2], pan.idx,
+                                                      vs[0], vs[1], vs[2], ss)]
+            ptr = [a.ctypes.data_as(C.c_void_p) for a in keep]
+            fn = str(tmp_path / scene.panel_vtu_name(0, 7))
+            L.ref_write_panels_vtk(fn.encode(), pan.nodes.shape[1], ptr[0], ptr[1], ptr[2], pan.np, ptr[3],
+                                   ptr[4], ptr[5], ptr[6], ptr[7] if src else None, 0.75)
+            got = scene.panels_vtu_bytes(pan.nodes, pan.idx, 0.75, vs, ss if src else None)
+            assert got == open(fn, "rb").read(), (f, src)
 
 def test_vtu_round_trip(tmp_path):
     rng = np.random.default_rng(1)
