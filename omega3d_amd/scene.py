@@ -13,6 +13,9 @@ from_json / init_elements):
     in the reference (std::random_device, :561-562); here it takes a seed.  A
     particle emitter adds one particle per step (step_elements, :702-712) and none
     at start.  Disabled features are skipped;
+  * ``measurements`` ("tracer", "point", "tracer emitter", "tracer blob", "tracer
+    line", "measurement line", "measurement quad") become inert field points
+    (``fieldpts``) at the tracer spacing tracerScale x ips (MeasureFeature.cpp);
   * ``bodies``: each mesh of a body is made into flat triangles at the particle
     spacing by the reference's meshers (BoundaryFeature.cpp:39-60, GeomHelper.cpp:
     41-127): "sphere" (the 12-node icosahedron of radius 0.5, split 1 -> 4 and
@@ -475,6 +478,70 @@ def panel_records(nodes, faces):
     return 0.5 * base * height, n
 
 
+
+# ---------------------------------------------------------------------------
+# Measurement features: inert field points (MeasureFeature.cpp:44-68 dispatch,
+# each type's from_json and init_elements), at the tracer spacing tracerScale x ips
+# (main_batch.cpp:95; tracerScale from "drawparams", 0.15 by default).
+# ---------------------------------------------------------------------------
+
+def measure_points(m, tips, seed=0):
+    """Field points (3, n) float32 of one measurement feature, or None for a type
+    that names none (MeasureFeature.cpp:58-61: ignored).  "tracer blob" jitters its
+    lattice points by a uniform [-0.5, 0.5) x tips per axis; the reference draws that
+    from an unseeded generator (:276-278), here it takes a seed."""
+    t = m.get("type")
+    if t in ("tracer", "point", "tracer emitter"):                 # SinglePoint::init_elements (:164-176)
+        return np.array(m["center"], f32).reshape(3, 1)
+    if t == "tracer blob":                                        # MeasurementBlob::init_elements (:273-320)
+        c, rad = [_f(v) for v in m["center"]], _f(m["rad"])
+        irad = int(_f(1.0) + _f(rad / tips))
+        rng = np.random.default_rng(seed)
+        pts = []
+        for i in range(-irad, irad + 1):
+            for j in range(-irad, irad + 1):
+                for k in range(-irad, irad + 1):
+                    dr = _f(_f(math.sqrt(float(_f(i * i + j * j + k * k)))) * tips)
+                    if dr < rad:
+                        u = rng.uniform(-0.5, 0.5, 3).astype(f32)
+                        pts.append([_f(c[0] + _f(tips * _f(_f(i) + u[0]))), _f(c[1] + _f(tips * _f(_f(j) + u[1]))),
+                                    _f(c[2] + _f(tips * _f(_f(k) + u[2])))])
+        return np.array(pts, f32).reshape(-1, 3).T.copy()
+    if t in ("tracer line", "measurement line"):                  # MeasurementLine::init_elements (:415-445)
+        a, b = [_f(v) for v in m["center"]], [_f(v) for v in m["end"]]
+        llen = _f(math.sqrt(sum((float(b[d]) - float(a[d])) ** 2 for d in range(3))))
+        ilen = int(_f(_f(1.0) + _f(llen / tips)))
+        out = np.empty((3, ilen), f32)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            for i in range(ilen):
+                frac = _f(np.float32(i) / np.float32(ilen - 1))
+                for d in range(3):
+                    out[d, i] = _f((1.0 - float(frac)) * float(a[d]) + float(_f(frac * b[d])))
+        return out
+    if t == "measurement quad":                                   # Grid2dPoints::init_elements (:552-583)
+        x0, s1, s2 = [[_f(v) for v in m[k]] for k in ("start", "axis1", "axis2")]
+        ds, df = _f(m["dx"][0]), _f(m["dx"][1])
+        dist_s = _f(math.sqrt(sum(float(v) ** 2 for v in s1)))
+        dist_f = _f(math.sqrt(sum(float(v) ** 2 for v in s2)))
+        pts = []
+        sp = _f(0.5 * float(ds) / float(dist_s))
+        while float(sp) < 1.0 + 0.01 * float(ds) / float(dist_s):
+            fp = _f(0.5 * float(df) / float(dist_f))
+            while float(fp) < 1.0 + 0.01 * float(df) / float(dist_f):
+                pts.append([_f(_f(x0[d] + _f(s1[d] * sp)) + _f(s2[d] * fp)) for d in range(3)])
+                fp = _f(fp + _f(df / dist_f))
+            sp = _f(sp + _f(ds / dist_s))
+        return np.array(pts, f32).reshape(-1, 3).T.copy()
+    return None
+
+
+# (moves, emits) each type starts with before its "lagrangian" / "emits" keys
+# (MeasureFeature.cpp:52-58 with the constructors' positional arguments: "tracer
+# emitter" and "tracer line" pass (0, 0, false, true), i.e. z = 0 and moves = true)
+_MEASURE_DEFAULTS = {"tracer": (False, False), "point": (False, False), "tracer emitter": (True, False),
+                     "tracer blob": (False, False), "tracer line": (True, False),
+                     "measurement line": (False, False), "measurement quad": (False, False)}
+
 class Scene:
     """A parsed JSON scene: time step, Reynolds number, freestream, particle spacing
     and core radius, and the particle collections of its enabled flow structures."""
@@ -529,6 +596,20 @@ class Scene:
         self.pos = np.concatenate(pos, axis=1) if pos else np.zeros((3, 0), f32)
         self.str = np.concatenate(st, axis=1) if st else np.zeros((3, 0), f32)
         self.rad = np.full(self.pos.shape[1], self.vdelta, f32)
+        # measurement features: inert field points at the tracer spacing
+        dp = doc.get("drawparams", {})
+        self.tracer_scale = _f(dp.get("tracerScale", 0.15))
+        self.tips = _f(self.tracer_scale * self.ips)
+        self.measures = []       # (type, dict, moves, emits) of the enabled measurement features
+        fp_list = []
+        for k, m in enumerate(doc.get("measurements", [])):
+            if "type" not in m or not m.get("enabled", True) or m["type"] not in _MEASURE_DEFAULTS:
+                continue
+            x = measure_points(m, self.tips, seed + 1000 + k)
+            moves, emits = _MEASURE_DEFAULTS[m["type"]]
+            self.measures.append((m["type"], m, bool(m.get("lagrangian", moves)), bool(m.get("emits", emits))))
+            fp_list.append(x)
+        self.fieldpts = np.concatenate(fp_list, axis=1) if fp_list else np.zeros((3, 0), f32)
         # bodies: each mesh at the particle spacing (main_batch.cpp:85), placed by the
         # body's translation at t = 0; bodies that move by formula or start rotated
         # are listed, not meshed

@@ -280,3 +280,46 @@ def test_body_options_and_unmeshed_geometries():
     reasons = {(u[0], u[1]): u[2] for u in s.unmeshed}
     assert set(reasons) == {("b", "disk"), ("b", "wing.obj"), ("spun", "rect"), ("moving", "rect")}
     assert "decimate" in reasons[("b", "disk")] and "libigl" in reasons[("b", "wing.obj")]
+
+
+# --- measurement features ----------------------------------------------------
+
+def test_measurement_features():
+    doc = {"simparams": {"nominalDt": 0.01}, "flowparams": {"Re": 100.0}, "drawparams": {"tracerScale": 0.2},
+           "measurements": [
+               {"type": "point", "center": [0.1, 0.2, 0.3]},
+               {"type": "measurement line", "center": [0, 0, 0], "end": [0.3, 0.4, 0], "lagrangian": True},
+               {"type": "measurement quad", "start": [-1, -1, 0.5], "axis1": [2, 0, 0], "axis2": [0, 1, 0],
+                "dx": [0.1, 0.25]},
+               {"type": "tracer blob", "center": [0, 0, 1], "rad": 0.03},
+               {"type": "tracer emitter", "center": [0, 1, 0]},
+               {"type": "no such feature", "center": [0, 0, 0]},
+               {"type": "point", "center": [9, 9, 9], "enabled": False}]}
+    s = scene.Scene(doc)
+    tips = float(s.tips)
+    assert abs(tips - 0.2 * float(s.ips)) < 1e-9
+    assert [m[0] for m in s.measures] == ["point", "measurement line", "measurement quad", "tracer blob",
+                                          "tracer emitter"]
+    assert [m[2] for m in s.measures] == [False, True, False, False, True]     # the emitter starts lagrangian
+    x = s.fieldpts.astype(np.float64)
+    off = 0
+    assert np.allclose(x[:, 0], [0.1, 0.2, 0.3]) and np.all(x[:, 0] == np.float32([0.1, 0.2, 0.3]))
+    off += 1
+    nl = int(np.float32(1.0) + np.float32(0.5) / np.float32(tips))             # 1 + length / tips
+    line = x[:, off:off + nl]
+    assert np.allclose(line[:, 0], 0) and np.allclose(line[:, -1], [0.3, 0.4, 0])
+    assert np.allclose(np.diff(line, axis=1), (line[:, -1:] - line[:, :1]) / (nl - 1), atol=1e-7)
+    off += nl
+    quad = x[:, off:off + 20 * 4]                                                # 2 / 0.1 x 1 / 0.25 cells
+    assert np.allclose(quad[:, 0], [-1 + 0.05, -1 + 0.125, 0.5])
+    assert np.all((quad[0] > -1) & (quad[0] < 1) & (quad[1] > -1) & (quad[1] < 0) & (quad[2] == 0.5))
+    assert len(np.unique(quad[0].round(6))) == 20 and len(np.unique(quad[1].round(6))) == 4
+    off += 80
+    ijk = np.array([(i, j, k) for i in range(-9, 10) for j in range(-9, 10) for k in range(-9, 10)], np.float32)
+    inside = np.sqrt((ijk ** 2).sum(1)) * np.float32(tips) < np.float32(0.03)
+    nb = int(inside.sum())
+    blob = x[:, off:off + nb] - np.array([[0], [0], [1.0]])
+    site = ijk[inside].T * tips
+    assert np.all(np.abs(blob - site) <= 0.5 * tips + 1e-7)                     # each point jittered about its site
+    off += nb
+    assert np.allclose(x[:, off], [0, 1, 0]) and x.shape[1] == off + 1

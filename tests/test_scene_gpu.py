@@ -115,3 +115,22 @@ def test_example_sphere_scene_bem(mods):
     np.testing.assert_array_equal(dpu.to_host(), opu)
     np.testing.assert_array_equal(drhs.to_host(), orhs)
     assert np.abs(orhs).max() > 0.1                  # the freestream (1, 0, 0) projected on the panels
+
+
+def test_example_scene_measurement_points(mods):
+    """An example scene with a measurement quad and line added: the velocity and its
+    gradient at the inert field points (point targets, kernel_0v_0pg) from the
+    scene's particles, against the oracle."""
+    import json
+    infl, _ = mods
+    doc = json.load(open(os.path.join(GOLD, "scenes", "colliding_vortex_rings_nv.json")))
+    doc["measurements"] = [{"type": "measurement quad", "start": [-1.0, -1.0, 0.0], "axis1": [2.0, 0.0, 0.0],
+                            "axis2": [0.0, 0.0, 2.0], "dx": [0.05, 0.05]},
+                           {"type": "measurement line", "center": [0.0, 0.0, -1.0], "end": [0.0, 0.0, 1.0]}]
+    s = scene.Scene(doc)
+    assert s.fieldpts.shape[1] > 1600
+    v, g = infl.p2p("wl", s.pos, s.str, s.rad, s.fieldpts, None, True, "comp")
+    ov, og = oracle.p2p("wl", s.pos, s.str, s.rad, s.fieldpts, None, True, "f64")
+    ev, eg = rel_errors(v, ov), rel_errors(g, og)
+    print(f"{s.fieldpts.shape[1]} field points: comp vs fp64 {ev} {eg}")
+    assert ev[0] <= 2e-6 and eg[0] <= 2e-6
