@@ -555,3 +555,5 @@ if __name__ == "__main__":
         print(json.dumps(served()))
     elif what == "panels":
         print(json.dumps(panels_run()))
+    elif what == "single_ring":
+        print(json.dumps(single_ring_run(reps=50, log=lambda s: None)))
