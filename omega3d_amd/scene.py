@@ -494,44 +494,40 @@ def measure_points(m, tips, seed=0):
     if t in ("tracer", "point", "tracer emitter"):                 # SinglePoint::init_elements (:164-176)
         return np.array(m["center"], f32).reshape(3, 1)
     if t == "tracer blob":                                        # MeasurementBlob::init_elements (:273-320)
-        c, rad = [_f(v) for v in m["center"]], _f(m["rad"])
+        c, rad = np.array([_f(v) for v in m["center"]], f32), _f(m["rad"])
         irad = int(_f(1.0) + _f(rad / tips))
-        rng = np.random.default_rng(seed)
-        pts = []
-        for i in range(-irad, irad + 1):
-            for j in range(-irad, irad + 1):
-                for k in range(-irad, irad + 1):
-                    dr = _f(_f(math.sqrt(float(_f(i * i + j * j + k * k)))) * tips)
-                    if dr < rad:
-                        u = rng.uniform(-0.5, 0.5, 3).astype(f32)
-                        pts.append([_f(c[0] + _f(tips * _f(_f(i) + u[0]))), _f(c[1] + _f(tips * _f(_f(j) + u[1]))),
-                                    _f(c[2] + _f(tips * _f(_f(k) + u[2])))])
-        return np.array(pts, f32).reshape(-1, 3).T.copy()
+        r = np.arange(-irad, irad + 1)
+        ijk = np.stack(np.meshgrid(r, r, r, indexing="ij"), axis=-1).reshape(-1, 3)   # i, then j, then k
+        dr = np.sqrt((ijk ** 2).sum(1).astype(f32)) * f32(tips)
+        ijk = ijk[dr < rad].astype(f32)
+        u = np.random.default_rng(seed).uniform(-0.5, 0.5, ijk.shape).astype(f32)      # three per point, in order
+        return (c[None, :] + f32(tips) * (ijk + u)).astype(f32).T.copy()
     if t in ("tracer line", "measurement line"):                  # MeasurementLine::init_elements (:415-445)
         a, b = [_f(v) for v in m["center"]], [_f(v) for v in m["end"]]
         llen = _f(math.sqrt(sum((float(b[d]) - float(a[d])) ** 2 for d in range(3))))
         ilen = int(_f(_f(1.0) + _f(llen / tips)))
-        out = np.empty((3, ilen), f32)
-        with np.errstate(invalid="ignore", divide="ignore"):
-            for i in range(ilen):
-                frac = _f(np.float32(i) / np.float32(ilen - 1))
-                for d in range(3):
-                    out[d, i] = _f((1.0 - float(frac)) * float(a[d]) + float(_f(frac * b[d])))
-        return out
+        with np.errstate(invalid="ignore", divide="ignore"):           # ilen = 1: 0 / 0, as in the reference
+            frac = np.arange(ilen, dtype=f32) / f32(ilen - 1)
+        a64, b32 = np.array(a, np.float64), np.array(b, f32)
+        # (1.0 - frac) * a in double, frac * b in float, the sum rounded to float
+        out = (1.0 - frac.astype(np.float64))[None, :] * a64[:, None] + (frac[None, :] * b32[:, None]).astype(np.float64)
+        return out.astype(f32)
     if t == "measurement quad":                                   # Grid2dPoints::init_elements (:552-583)
-        x0, s1, s2 = [[_f(v) for v in m[k]] for k in ("start", "axis1", "axis2")]
+        x0, s1, s2 = [np.array([_f(v) for v in m[k]], f32) for k in ("start", "axis1", "axis2")]
         ds, df = _f(m["dx"][0]), _f(m["dx"][1])
-        dist_s = _f(math.sqrt(sum(float(v) ** 2 for v in s1)))
-        dist_f = _f(math.sqrt(sum(float(v) ** 2 for v in s2)))
-        pts = []
-        sp = _f(0.5 * float(ds) / float(dist_s))
-        while float(sp) < 1.0 + 0.01 * float(ds) / float(dist_s):
-            fp = _f(0.5 * float(df) / float(dist_f))
-            while float(fp) < 1.0 + 0.01 * float(df) / float(dist_f):
-                pts.append([_f(_f(x0[d] + _f(s1[d] * sp)) + _f(s2[d] * fp)) for d in range(3)])
-                fp = _f(fp + _f(df / dist_f))
-            sp = _f(sp + _f(ds / dist_s))
-        return np.array(pts, f32).reshape(-1, 3).T.copy()
+
+        def steps(d, dist):
+            # the loop variable of the reference: a float, started at 0.5 d / dist
+            # (in double) and stepped by d / dist (in float) while below 1 + 0.01 d / dist
+            out, v = [], _f(0.5 * float(d) / float(dist))
+            while float(v) < 1.0 + 0.01 * float(d) / float(dist):
+                out.append(v)
+                v = _f(v + _f(d / dist))
+            return np.array(out, f32)
+        sp = steps(ds, _f(math.sqrt(sum(float(v) ** 2 for v in s1))))
+        fp = steps(df, _f(math.sqrt(sum(float(v) ** 2 for v in s2))))
+        pts = (x0[None, None, :] + s1[None, None, :] * sp[:, None, None]) + s2[None, None, :] * fp[None, :, None]
+        return pts.astype(f32).reshape(-1, 3).T.copy()
     return None
 
 
