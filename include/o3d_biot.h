@@ -363,7 +363,7 @@ typedef struct o3d_batch o3d_batch;
 
 /* cores / precs / self_terms: per-instance arrays (NULL = the opts value for all).
  * opts->tile / opts->tpt: 0 = defaults (see o3d_batch_tune).
- * opts->method: AUTO sends every instance of >= 65,536 particles with an RM or EXP
+ * opts->method: AUTO sends every instance of >= 32,768 particles with an RM or EXP
  * core in FP32 or compensated precision to the factored far-field method; FACTORED
  * sends all instances there (an instance it cannot take is an error); DIRECT keeps
  * all in the persistent queue.  Factored instances run on up to 8 streams the

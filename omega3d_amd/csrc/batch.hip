@@ -7,8 +7,9 @@
 // item runs the same p2p_chunk as the single-collection kernel, so a target's
 // sums do not depend on the launch that computed it.
 //
-// Instances large enough for the factored far field (the same rule as a single
-// call with method AUTO, or all of them with method FACTORED) leave the queue and
+// Instances large enough for the factored far field (method AUTO: 32,768 particles
+// and more, where a single call takes it from 65,536; or all of them with method
+// FACTORED) leave the queue and
 // run that pipeline on FF_LANES (8) streams of the batch (longest first, each to the
 // least-loaded lane), each lane with scratch of its own.  The lanes fork from the
 // caller's stream after the zero fill and join it at the end, so they overlap
@@ -365,7 +366,7 @@ int o3d_batch_create(o3d_batch** out, int32_t ninst, const int64_t* off, const i
     const BatchInst& I = b->inst[i];
     const bool ffok = ff_supported(I.core, true, I.prec) && I.n <= INT32_MAX;
     if (I.n > 0 && ffok &&
-        (opts->method == O3D_METHOD_FACTORED || (opts->method == O3D_METHOD_AUTO && I.n >= FF_AUTO_MIN_N))) {
+        (opts->method == O3D_METHOD_FACTORED || (opts->method == O3D_METHOD_AUTO && I.n >= BATCH_FF_AUTO_MIN_N))) {
       factored[i] = 1;
       b->ff.push_back(i);
     }

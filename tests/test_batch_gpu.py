@@ -121,7 +121,7 @@ def test_batch_empty_and_tiny(mods):
 
 
 def test_batch_factored_instances(mods):
-    """Instances of >= 65,536 particles leave the queue for the factored method (AUTO):
+    """Instances of >= 32,768 particles leave the queue for the factored method (AUTO):
     each is bitwise the single-collection factored call, the small ones bitwise the
     direct call; run_separate and graph replays (with eager work between them) give
     the same bits; the large instances meet the compensated bar against the oracle."""
@@ -137,12 +137,12 @@ def test_batch_factored_instances(mods):
         _lib.check(_lib.lib.o3d_stream_synchronize(s))
         eager = b.results()
         for (v, g), p in zip(eager, inst):
-            m = "factored" if p.n >= 65536 else "direct"
+            m = "factored" if p.n >= 32768 else "direct"
             va, ga = infl.p2p("exp", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", tile=256 if m == "direct" else 0,
                               tpt=1 if m == "direct" else 0, method=m)
             np.testing.assert_array_equal(v, va)
             np.testing.assert_array_equal(g, ga)
-            if p.n >= 65536:
+            if p.n >= 32768:
                 sel = np.arange(0, p.n, p.n // 512)[:512]
                 ov, og = oracle.p2p("exp", p.pos, p.str, p.rad, p.pos[:, sel], p.rad[sel], True, "f64")
                 assert rel_errors(v[:, sel], ov)[0] <= 1e-6 and rel_errors(g[:, sel], og)[0] <= 1e-6

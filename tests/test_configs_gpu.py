@@ -2,7 +2,7 @@
 
 Batched: 256 independent rings, N log-uniform on [2,048, 262,144], R ~ U[0.5, 2],
 sigma ~ U[0.01, 0.05], RM core, compensated FP32, method AUTO (instances of
->= 65,536 particles take the factored method on the batch's lanes), through the
+>= 32,768 particles take the factored method on the batch's lanes), through the
 persistent launch and replays of the captured graph.  Every instance is bitwise
 its own single-collection call at the same tile and method; 128 spread targets of
 8 instances (the smallest, the largest, three factored, three direct) are within
@@ -24,6 +24,8 @@ from omega3d_amd import scenes
 from omega3d_amd.metrics import rel_errors
 
 pytestmark = pytest.mark.gpu
+# method AUTO in a batch: the factored method from this many particles (o3d_biot.h)
+BATCH_FACTORED_MIN_N = 32768
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +45,7 @@ def test_batched_256_rings(mods):
     s = batch.stream_create()
     try:
         cfg = b.config()
-        nf = int((n >= 65536).sum())
+        nf = int((n >= BATCH_FACTORED_MIN_N).sum())
         assert cfg["factored_instances"] == nf and nf > 0
         b.run(s)
         _lib.check(_lib.lib.o3d_stream_synchronize(s))
@@ -59,7 +61,7 @@ def test_batched_256_rings(mods):
             infl.p2p("wl", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp")   # eager work between replays
         print("batched: graph replays bitwise; single-collection calls", flush=True)
         for i, p in enumerate(inst):
-            fac = p.n >= 65536
+            fac = p.n >= BATCH_FACTORED_MIN_N
             va, ga = infl.p2p("rm", p.pos, p.str, p.rad, p.pos, p.rad, True, "comp", tile=0 if fac else 256,
                               tpt=0 if fac else 1, method="factored" if fac else "direct")
             v, g = eager[i]
@@ -67,8 +69,8 @@ def test_batched_256_rings(mods):
     finally:
         b.close()
         batch.stream_destroy(s)
-    fac_idx = [i for i in np.argsort(n) if n[i] >= 65536]
-    dir_idx = [i for i in np.argsort(n) if n[i] < 65536]
+    fac_idx = [i for i in np.argsort(n) if n[i] >= BATCH_FACTORED_MIN_N]
+    dir_idx = [i for i in np.argsort(n) if n[i] < BATCH_FACTORED_MIN_N]
     pick = {int(np.argmin(n)), int(np.argmax(n))}
     pick |= {int(fac_idx[k]) for k in np.linspace(0, len(fac_idx) - 1, 3).astype(int)}
     pick |= {int(dir_idx[k]) for k in np.linspace(0, len(dir_idx) - 1, 3).astype(int)}
