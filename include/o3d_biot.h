@@ -430,7 +430,7 @@ typedef struct o3d_server_opts {
   int32_t adaptive_tick;      /* raise the budget to (workgroup slots x the largest
                                  active chunk) so that large sessions fill the GPU      */
   int32_t method;             /* enum o3d_method.  AUTO (0): a step of a session of
-                                 >= 65,536 particles with the RM or EXP core evaluates
+                                 >= 32,768 particles with the RM or EXP core evaluates
                                  both stages by the factored far-field method on a
                                  second stream, overlapping the ticks; FACTORED: every
                                  RM/EXP session; DIRECT: ticks only.  The factored

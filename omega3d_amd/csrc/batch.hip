@@ -366,7 +366,7 @@ int o3d_batch_create(o3d_batch** out, int32_t ninst, const int64_t* off, const i
     const BatchInst& I = b->inst[i];
     const bool ffok = ff_supported(I.core, true, I.prec) && I.n <= INT32_MAX;
     if (I.n > 0 && ffok &&
-        (opts->method == O3D_METHOD_FACTORED || (opts->method == O3D_METHOD_AUTO && I.n >= BATCH_FF_AUTO_MIN_N))) {
+        (opts->method == O3D_METHOD_FACTORED || (opts->method == O3D_METHOD_AUTO && I.n >= LANES_FF_AUTO_MIN_N))) {
       factored[i] = 1;
       b->ff.push_back(i);
     }

@@ -69,11 +69,13 @@ int radix_sort_pairs(unsigned* ka, int* va, unsigned* kb, int* vb, int64_t n, in
 int exclusive_scan_i32(const int* in, int* out, int64_t n, hipStream_t st);
 // a collection large enough for the factored method to pay when the caller leaves the choice open
 constexpr int64_t FF_AUTO_MIN_N = 65536;
-// the same for an instance of a batch: its factored instances run on the batch's lanes while the
-// persistent kernel sums the rest, so the split point that balances the two is lower (256 RM rings of
-// 2,048-262,144 particles, compensated, graph replay: 682-685 ms at 65,536, 667-668 at 49,152, 665-666
-// at 40,960, 662-664 at 32,768; 741 at 98,304)
-constexpr int64_t BATCH_FF_AUTO_MIN_N = 32768;
+// the same for an instance of a batch or a session of the server: their factored work runs on
+// streams of its own while the persistent kernel sums the rest, so the split point that balances
+// the two is lower.  256 RM rings of 2,048-262,144 particles, compensated, graph replay: 682-685 ms
+// at 65,536, 667-668 at 49,152, 665-666 at 40,960, 662-664 at 32,768, 741 at 98,304.  16 served
+// sessions of 1,986-296,912 particles: at 32,768 against 65,536 saturated 2,881-2,905 against
+// 2,853-2,856 G interactions/s, Poisson p50 7.8 against 10.6-11.0 ms, p99 78.5 against 80.3 ms
+constexpr int64_t LANES_FF_AUTO_MIN_N = 32768;
 // direct EXP calls with at least this many targets visit them in Morton order
 constexpr int64_t TARGET_ORDER_MIN_N = 32768;
 int p2p_dev(const o3d_points* src, o3d_points* targ, const o3d_opts* o, hipStream_t st, const Tuning* tun);

@@ -17,7 +17,7 @@
 // The evaluation of a chunk is p2p_chunk, the same code as the single-collection
 // kernel, so a step's result does not depend on its companions.
 //
-// A step of a large session with the RM or EXP core (by default >= 65,536
+// A step of a large session with the RM or EXP core (by default >= 32,768
 // particles) skips the ticks: both of its stages run the factored far-field
 // method, each followed by its stage update, on one of a few further streams
 // ("lanes") that overlap the ticks and one another, each lane with scratch the
@@ -510,7 +510,7 @@ void finish_request(o3d_server* S, Request& r, int state, int status, double whe
 bool session_factored(const o3d_server* S, const Session& ses) {
   if (ses.core != CORE_RM && ses.core != CORE_EXP) return false;
   if (S->opts.method == O3D_METHOD_FACTORED) return true;
-  return S->opts.method == O3D_METHOD_AUTO && ses.n >= FF_AUTO_MIN_N;
+  return S->opts.method == O3D_METHOD_AUTO && ses.n >= LANES_FF_AUTO_MIN_N;
 }
 
 // admit queued requests (arrival order; a request of a session with an admitted

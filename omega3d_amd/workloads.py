@@ -306,7 +306,7 @@ def served(duration=20.0, sessions=16, lo=1024, hi=1048576, load=0.5, rate=None,
     sel = np.linspace(0, p.n - 1, 1024).astype(np.int64)
     vr, gr = _fp64_self(core, p, sel)
     spot = {"session": big, "n": p.n, "core": core, "targets": len(sel),
-            "method": "factored" if core in ("rm", "exp") and p.n >= 65536 else "direct"}
+            "method": "factored" if core in ("rm", "exp") and p.n >= 32768 else "direct"}
     for pr in ("comp", "fp32"):
         v, g = influence.p2p(core, p.pos, p.str, p.rad, p.pos, p.rad, True, pr)
         spot[pr] = field_errors(v[:, sel], g[:, sel], vr, gr)
