@@ -66,7 +66,7 @@ def test_flop_counts_match_reference():
                 oracle.flops_pair(core, False, True), oracle.flops_pair(core, False, False),
                 oracle.flops_pair(core, True, False, True), oracle.flops_pair(core, False, False, True)]
         assert list(f[core]) == want
-    # the survey's per-interaction table: RM 63, WL 70, EXP 68, V2 67
+    # kernel_0v_0bg's count per interaction (Kernels.h:155 plus each core's flops, CoreFunc.h): RM 63, WL 70, EXP 68, V2 67
     assert [oracle.flops_pair(c) for c in CORES] == [63, 70, 68, 67]
 
 
